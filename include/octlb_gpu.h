@@ -1,0 +1,234 @@
+/*
+ * octlb_gpu.h -- C ABI of the MI355X (gfx950) per-time-step AMR D3Q19
+ * lattice-Boltzmann update.
+ *
+ * Everything here is plain C: integers, doubles and pointers.  A C++ host
+ * (the reference's TimeLoop / GhostSchedule / LevelCoupler / BlockLattice
+ * call sites), a cgo/ctypes binding or a Python harness can all bind it.
+ *
+ * Conventions shared with the reference (gezengran/octlb):
+ *   - Face numbering  x-,x+,y-,y+,z-,z+ = 0..5      (src/common/types.h:15-22)
+ *   - BcKind byte per core cell, index (ix*n+iy)*n+iz (block_lattice.cpp:288-296)
+ *   - Populations are SHIFTED (f_i - w_i); rho = 1 + sum(f)  (block_lattice.h:113-125)
+ *   - D3Q19 velocity order of OpenLB  (descriptor/definition/common.h:166-179)
+ *   - Block population layout on upload/download is the reference's
+ *     halo-inclusive AoS: [(hx*NP+hy)*NP+hz][19], NP = n+2*halo
+ *     (block_lattice.h:294-304).
+ *
+ * Every function returns 0 on success or a negative OCTLB_E* code; the
+ * message of the last failure on the calling thread is octlb_last_error().
+ * One device handle owns one HIP stream; a handle is not thread-safe.
+ */
+#ifndef OCTLB_GPU_H_
+#define OCTLB_GPU_H_
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define OCTLB_Q 19
+
+/* error codes */
+#define OCTLB_OK 0
+#define OCTLB_EINVAL (-1)  /* bad argument / inconsistent tables          */
+#define OCTLB_ENOMEM (-2)  /* host or device allocation failed            */
+#define OCTLB_EHIP (-3)    /* HIP runtime error (message has the details) */
+#define OCTLB_ESTATE (-4)  /* call not valid in the current state         */
+
+/* FaceDir (reference src/common/types.h:15-22) */
+enum { OCTLB_XMIN = 0, OCTLB_XMAX, OCTLB_YMIN, OCTLB_YMAX, OCTLB_ZMIN, OCTLB_ZMAX };
+
+/* BcKind (reference src/solver/lbm/bc_kind.h:14-23) */
+enum {
+  OCTLB_KIND_BULK = 0,
+  OCTLB_KIND_SOLID,
+  OCTLB_KIND_BOUNCE_BACK,
+  OCTLB_KIND_MOVING_BOUNCE_BACK,
+  OCTLB_KIND_BOUZIDI,
+  OCTLB_KIND_VELOCITY_DIRICHLET,
+  OCTLB_KIND_PRESSURE_DIRICHLET,
+  OCTLB_KIND_OUTFLOW
+};
+
+/* DomainBcType (reference domain_boundary_handler.h:16-22) */
+enum {
+  OCTLB_BC_NO_SLIP = 0,
+  OCTLB_BC_MOVING_LID,
+  OCTLB_BC_INTERP_VELOCITY,
+  OCTLB_BC_OUTFLOW,
+  OCTLB_BC_INTERP_PRESSURE
+};
+
+/* InletVelocityField subclasses (reference boundary/inlet_velocity_field.h) */
+enum {
+  OCTLB_INLET_NONE = 0,
+  OCTLB_INLET_UNIFORM,     /* UniformInletProfile            :131-176 */
+  OCTLB_INLET_POISEUILLE,  /* PoiseuilleInletProfile (index) :47-129  */
+  OCTLB_INLET_CHANNEL      /* ChannelPoiseuilleInletProfile  :178-270 */
+};
+
+/* Which DomainBoundaryHandler the TimeLoop is given (time_loop.h:49-54). */
+enum {
+  OCTLB_HANDLER_NOOP = 0,     /* NoOpDomainBoundaryHandler (domain_boundary_handler.h:94-99) */
+  OCTLB_HANDLER_CONCRETE = 1  /* ConcreteDomainBoundaryHandler (.cpp:31-89)                  */
+};
+
+/* One DomainBcSpec (domain_boundary_handler.h:31-42) with its inlet field. */
+typedef struct {
+  int32_t face;         /* FaceDir the spec applies to                    */
+  int32_t type;         /* OCTLB_BC_*                                     */
+  double u_wall[3];     /* constant wall velocity                         */
+  double rho_target;    /* pressure-outlet density (default 1.0)          */
+  int32_t inlet;        /* OCTLB_INLET_*; NONE -> use u_wall              */
+  int32_t inlet_face;   /* face_dir of the profile (in-plane axes)        */
+  double inlet_u[3];    /* UNIFORM: u_const;  POISEUILLE/CHANNEL: flow_dir */
+  double u_peak;        /* POISEUILLE/CHANNEL peak velocity               */
+  int32_t n_inplane0;   /* POISEUILLE cells along in-plane axis 0         */
+  int32_t n_inplane1;   /* POISEUILLE cells along in-plane axis 1         */
+  double lo0, hi0;      /* CHANNEL physical extent, in-plane axis 0       */
+  double lo1, hi1;      /* CHANNEL physical extent, in-plane axis 1       */
+  double ramp_end_t;    /* smoothstep ramp end time (0 = no ramp)         */
+} octlb_bc_spec;
+
+/* One CoarseFineFace (reference src/mesh/topology/face_pair_list.h:22-38). */
+typedef struct {
+  int32_t coarse_id;
+  int32_t fine_ids[4];
+  int32_t normal;            /* coarse-side FaceDir pointing at the fine side */
+  int32_t coarse_level;
+  int32_t fine_level;
+  double coarse_bounds[6];   /* x_min,y_min,z_min,x_max,y_max,z_max          */
+  double fine_bounds[4][6];
+} octlb_cf_face;
+
+/*
+ * Flattened forest: what p4est's OctreeForest + FacePairList hand to the
+ * solver once per adapt.  All arrays are borrowed for the duration of the
+ * call that receives the struct (they are copied).
+ */
+typedef struct {
+  int32_t num_blocks;
+  int32_t n;                      /* cells per block edge (even)           */
+  int32_t halo;                   /* halo width h of BlockLattice(n,n,n,h) */
+  const int32_t* level;           /* [num_blocks]                          */
+  const double* bounds;           /* [num_blocks*6]                        */
+  int32_t num_same_level;
+  const int32_t* same_level;      /* [num_same_level*3] local,dir,remote   */
+  int32_t num_coarse_fine;
+  const octlb_cf_face* coarse_fine;
+  int32_t num_tree_boundary;
+  const int32_t* tree_boundary;   /* [num_tree_boundary*2] octant,dir      */
+} octlb_topology;
+
+/* Solver configuration: the TimeLoop constructor arguments plus the
+ * per-block stream mode and the domain BC specs. */
+typedef struct {
+  double omega;          /* single relaxation frequency (all levels)           */
+  int32_t precision;     /* 64: FP64 populations; 32: FP32 shifted populations */
+  int32_t handler;       /* OCTLB_HANDLER_*                                     */
+  int32_t const_rho;     /* use_const_rho_bgk (time_loop.h:52)                  */
+  int32_t stats_scope;   /* 0 kFluidAndBoundary, 1 kFluidOnly (time_loop.h:39)  */
+  int32_t bouzidi;       /* 1: blocks carry Bouzidi links -> per-cell stream    */
+  int32_t snapshot;      /* set_snapshot_post_collide (time_loop.h:84)          */
+  int32_t num_specs;
+  const octlb_bc_spec* specs;
+} octlb_params;
+
+const char* octlb_last_error(void);
+
+/* ------------------------------------------------------------------ */
+/* Host forest: p4est-free brick forest with refine / 2:1 face balance */
+/* and FacePairList enumeration.  Replaces, for brick domains,          */
+/*   OctreeForest(comm, domain, bx,by,bz)   octree_forest.cpp:204-229   */
+/*   OctreeForest::refine(bounds pred)      octree_forest.cpp:248-261   */
+/*   OctreeForest::balance()                octree_forest.cpp:263-271   */
+/*   FacePairList(forest)                   face_pair_list.cpp:480-570  */
+/* ------------------------------------------------------------------ */
+typedef struct octlb_forest octlb_forest;
+
+int octlb_forest_create(const double domain[6], int32_t bricks_x, int32_t bricks_y,
+                        int32_t bricks_z, octlb_forest** out);
+void octlb_forest_destroy(octlb_forest* forest);
+/* Refine every octant below max_level whose bounds overlap box (closed
+ * intervals), recursively, like refine(criterion=overlap(box), max_level). */
+int octlb_forest_refine_box(octlb_forest* forest, const double box[6], int32_t max_level);
+/* Refine every octant whose mark[id] != 0 by one level (ids of the current
+ * octant order). */
+int octlb_forest_refine_marked(octlb_forest* forest, const uint8_t* mark, int64_t count);
+int octlb_forest_balance(octlb_forest* forest);
+int32_t octlb_forest_num_octants(const octlb_forest* forest);
+/* Build the face lists and expose them as a topology for blocks of n^3
+ * cells with halo h.  The pointers stay valid until the forest changes. */
+int octlb_forest_topology(octlb_forest* forest, int32_t n, int32_t halo,
+                          octlb_topology* out);
+
+/* ------------------------------------------------------------------ */
+/* Device solver                                                       */
+/* ------------------------------------------------------------------ */
+typedef struct octlb_dev octlb_dev;
+
+/* Allocate device tables and lattices for a topology ("once per adapt").
+ * Replaces the construction of BlockCollection + GhostSchedule +
+ * LevelCoupler + DomainBoundaryHandler + TimeLoop (time_loop.h:123-164).
+ * All populations start at zero (shifted rest state). */
+int octlb_dev_create(const octlb_topology* topo, const octlb_params* params,
+                     octlb_dev** out);
+int octlb_dev_destroy(octlb_dev* dev);
+
+/* BlockLattice::set_bc_kind for all core cells: kinds[num_blocks*n^3]. */
+int octlb_dev_set_bc_kinds(octlb_dev* dev, const uint8_t* kinds);
+
+/* Bouzidi wall fractions (BouzidiLinkData::set_q_frac, bouzidi_link_data.h:21).
+ * link[3*i] = block, link[3*i+1] = cell (ix*n+iy)*n+iz, link[3*i+2] = iPop.
+ * Links not listed keep q = 1.0 (half-way bounce-back). */
+int octlb_dev_set_links(octlb_dev* dev, int64_t count, const int32_t* link,
+                        const double* q);
+
+/* BlockLattice::initialize / initialize_from_material (lattice_material_init.cpp:11-33):
+ * feq(rho0,u0) on core cells whose kind is in kind_mask (bit k = BcKind k);
+ * all other slots are left untouched. */
+int octlb_dev_init_equilibrium(octlb_dev* dev, double rho0, const double u0[3],
+                               uint32_t kind_mask);
+/* feq(rho, u) per core cell: rho[num_blocks*n^3], u[num_blocks*n^3*3]. */
+int octlb_dev_init_field(octlb_dev* dev, const double* rho, const double* u);
+
+/* Reference AoS halo-inclusive layout, FP64 on the host side. */
+int octlb_dev_upload_block(octlb_dev* dev, int32_t block, const double* aos);
+int octlb_dev_download_block(octlb_dev* dev, int32_t block, double* aos);
+/* Whole lattice, core cells only: out[num_blocks*n^3*19], (block, cell, pop). */
+int octlb_dev_download_core(octlb_dev* dev, double* out);
+
+/* TimeLoop::advance_one() n times (time_loop.h:242-246). */
+int octlb_dev_advance(octlb_dev* dev, int32_t n_coarse_steps);
+/* Same as octlb_dev_advance but replays a captured hipGraph of one coarse step. */
+int octlb_dev_advance_graph(octlb_dev* dev, int32_t n_coarse_steps);
+int octlb_dev_synchronize(octlb_dev* dev);
+int octlb_dev_set_step(octlb_dev* dev, int32_t step);
+
+/* MomentumExchangeDrag::force_on_fluid_if summed over blocks with the
+ * obstacle predicate "neighbour cell centre inside bbox"
+ * (drag_postprocessor.h:66-115, cylinder3d_case.h:473-503). FP64 sum. */
+int octlb_dev_drag(octlb_dev* dev, const double bbox[6], double out[3]);
+/* Sum of rho over kBulk cells (cylinder3d_case.h:601-622). FP64 sum. */
+int octlb_dev_mass(octlb_dev* dev, double* out);
+/* TimeLoop::average_rho() (time_loop.h:88). */
+int octlb_dev_average_rho(octlb_dev* dev, double* out);
+
+/* Fine-equivalent and actual cell updates per coarse step. */
+int octlb_dev_update_counts(octlb_dev* dev, double* actual, double* fine_equivalent);
+
+/* Timing of the dominant kernel (fused collide+stream) over the last
+ * octlb_dev_advance* call, measured with HIP events on the handle's stream:
+ * total milliseconds and number of launches. */
+int octlb_dev_kernel_timing(octlb_dev* dev, double* ms, int64_t* launches,
+                            double* cell_updates);
+int octlb_dev_enable_timing(octlb_dev* dev, int32_t on);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* OCTLB_GPU_H_ */

@@ -1,0 +1,233 @@
+"""Case assembly: topology, BC specs, BcKind stamping, Bouzidi links, init.
+
+Host-side setup that the reference does in its case constructors
+(examples/cavity3d/cavity3d_case.h:139-194,
+examples/cylinder3d/cylinder3d_case.h:392-456) and installers
+(bc_installer.h:72-138, lattice_material_init.cpp:11-33,
+bouzidi_link_data.cpp:139-233).  Geometry is analytic (exact circle, Gaussian
+blobs) instead of CGAL voxelisation.
+"""
+from __future__ import annotations
+
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from . import abi
+from .forest import Forest, single_block_topology
+
+C = np.array([[0, 0, 0], [-1, 0, 0], [0, -1, 0], [0, 0, -1], [-1, -1, 0], [-1, 1, 0], [-1, 0, -1],
+              [-1, 0, 1], [0, -1, -1], [0, -1, 1], [1, 0, 0], [0, 1, 0], [0, 0, 1], [1, 1, 0],
+              [1, -1, 0], [1, 0, 1], [1, 0, -1], [0, 1, 1], [0, 1, -1]], dtype=np.int64)
+OPP = np.array([0, 10, 11, 12, 13, 14, 15, 16, 17, 18, 1, 2, 3, 4, 5, 6, 7, 8, 9])
+
+MAT_FLUID, MAT_SOLID, MAT_BOUNDARY = 0, 1, 2  # MaterialKind (geometry_types.h:17-21)
+
+
+@dataclass
+class Case:
+    name: str
+    topo: abi.TopologyArrays
+    omega: float
+    handler: int
+    const_rho: int
+    bouzidi: int
+    snapshot: int
+    specs: list
+    kinds: np.ndarray                      # uint8 [nb, n, n, n]
+    link: np.ndarray = field(default_factory=lambda: np.zeros((0, 3), np.int32))
+    q: np.ndarray = field(default_factory=lambda: np.zeros(0))
+    init_rho0: float = 1.0
+    init_u0: tuple = (0.0, 0.0, 0.0)
+    init_mask: int = 1 << abi.KIND_BULK
+    init_field: tuple | None = None        # (rho [nb*n^3], u [nb*n^3*3])
+    drag_bbox: np.ndarray | None = None
+    area: float = 1.0
+    u_ref: float = 1.0
+    stats_scope: int = 0
+    info: dict = field(default_factory=dict)
+
+    def params(self, precision: int = 64) -> abi.Params:
+        return abi.make_params(self.omega, precision=precision, handler=self.handler,
+                               const_rho=self.const_rho, stats_scope=self.stats_scope,
+                               bouzidi=self.bouzidi, snapshot=self.snapshot, specs=self.specs)
+
+    @property
+    def n(self):
+        return self.topo.n
+
+    @property
+    def num_blocks(self):
+        return self.topo.num_blocks
+
+
+# ------------------------------------------------------------- installers
+def bc_kind_from_spec_type(t: int) -> int:
+    """BcKindFromSpecType (bc_installer.h:44-58)."""
+    return {abi.BC_NO_SLIP: abi.KIND_BOUNCE_BACK, abi.BC_MOVING_LID: abi.KIND_MOVING_BOUNCE_BACK,
+            abi.BC_INTERP_VELOCITY: abi.KIND_VELOCITY_DIRICHLET, abi.BC_OUTFLOW: abi.KIND_OUTFLOW,
+            abi.BC_INTERP_PRESSURE: abi.KIND_PRESSURE_DIRICHLET}[t]
+
+
+def find_spec_type(specs, face: int) -> int:
+    for s in specs:
+        if s.face == face:
+            return s.type
+    return abi.BC_NO_SLIP  # default DomainBcSpec{}
+
+
+def stamp_tree_boundary(kinds: np.ndarray, tree_boundary: np.ndarray, specs) -> None:
+    """StampTreeBoundaryCells (bc_installer.h:96-138): face slabs take the
+    spec's kind in list order, skipping kSolid cells."""
+    for b, d in tree_boundary:
+        kind = bc_kind_from_spec_type(find_spec_type(specs, int(d)))
+        kb = kinds[b]
+        axis, side = int(d) // 2, int(d) & 1
+        idx = [slice(None)] * 3
+        idx[axis] = -1 if side else 0
+        slab = kb[tuple(idx)]
+        slab[slab != abi.KIND_SOLID] = kind
+
+
+def material_to_kinds(material: np.ndarray) -> np.ndarray:
+    """BcKindFromMaterial (bc_installer.h:31-41)."""
+    lut = np.array([abi.KIND_BULK, abi.KIND_SOLID, abi.KIND_BOUZIDI], dtype=np.uint8)
+    return lut[material]
+
+
+# ------------------------------------------------------------- unit converter
+def unit_converter(resolution, tau, length, velocity, nu, rho=1.0):
+    """UnitConverter (unit_converter.cpp:8-26): returns (omega, u_lattice)."""
+    inv_cs2 = 3.0
+    dx = length / float(resolution)
+    dt = (tau - 0.5) / inv_cs2 * dx * dx / nu
+    conv_u = dx / dt
+    return 1.0 / tau, velocity / conv_u
+
+
+# ------------------------------------------------------------- cavity (C1)
+def cavity(n_lat: int = 64, tau: float = 0.6, u_lid: float = 0.05, halo: int = 3) -> Case:
+    """Lid-driven cavity in the reference's single-block form
+    (cavity3d_case.h:139-194): one n_lat^3 block with OpenLB padding 3,
+    ConstRhoBGK bulk, interpolated-velocity (Skordos FD) on all six faces,
+    lid = u on the y-max face, padded-block rotate streaming."""
+    res = n_lat - 1
+    nu = (tau - 0.5) / 3.0 * (1.0 / res) / u_lid  # gives u_lattice = u_lid
+    omega, u_lat = unit_converter(res, tau, 1.0, 1.0, nu)
+    specs = [abi.make_spec(abi.XMIN, abi.BC_INTERP_VELOCITY),
+             abi.make_spec(abi.XMAX, abi.BC_INTERP_VELOCITY),
+             abi.make_spec(abi.YMIN, abi.BC_INTERP_VELOCITY),
+             abi.make_spec(abi.ZMIN, abi.BC_INTERP_VELOCITY),
+             abi.make_spec(abi.ZMAX, abi.BC_INTERP_VELOCITY),
+             abi.make_spec(abi.YMAX, abi.BC_INTERP_VELOCITY, u_wall=(u_lat, 0.0, 0.0))]
+    topo = single_block_topology(n_lat, halo)
+    kinds = np.zeros((1, n_lat, n_lat, n_lat), np.uint8)
+    stamp_tree_boundary(kinds, topo.tree_boundary, specs)
+    return Case(name=f"cavity{n_lat}", topo=topo, omega=omega, handler=abi.HANDLER_CONCRETE,
+                const_rho=1, bouzidi=0, snapshot=0, specs=specs, kinds=kinds,
+                init_mask=0xFF, u_ref=u_lat, info={"u_lid": u_lat, "tau": tau})
+
+
+# ------------------------------------------------------------- cylinder (C2)
+def _ray_circle(px, py, dx, dy, xc, yc, r):
+    """Smallest s in (0,1] with |p + s d - c| = r, or nan."""
+    fx, fy = px - xc, py - yc
+    a = dx * dx + dy * dy
+    b = 2.0 * (fx * dx + fy * dy)
+    c = fx * fx + fy * fy - r * r
+    with np.errstate(invalid="ignore", divide="ignore"):
+        disc = b * b - 4.0 * a * c
+        sq = np.sqrt(np.maximum(disc, 0.0))
+        s1 = (-b - sq) / (2.0 * a)
+        s2 = (-b + sq) / (2.0 * a)
+    s = np.where((s1 > 0.0) & (s1 <= 1.0), s1, np.where((s2 > 0.0) & (s2 <= 1.0), s2, np.nan))
+    return np.where((a > 0.0) & (disc >= 0.0), s, np.nan)
+
+
+def cell_centres(topo: abi.TopologyArrays):
+    n = topo.n
+    b = topo.bounds
+    cw = (b[:, 3] - b[:, 0]) / n
+    idx = (np.arange(n) + 0.5)
+    x = b[:, 0, None] + cw[:, None] * idx[None, :]
+    y = b[:, 1, None] + cw[:, None] * idx[None, :]
+    z = b[:, 2, None] + cw[:, None] * idx[None, :]
+    X = np.broadcast_to(x[:, :, None, None], (len(b), n, n, n))
+    Y = np.broadcast_to(y[:, None, :, None], (len(b), n, n, n))
+    Z = np.broadcast_to(z[:, None, None, :], (len(b), n, n, n))
+    return X, Y, Z, cw
+
+
+def cylinder_channel(bricks_x: int = 4, base_level: int = 2, fine: bool = True, n: int = 8,
+                     bricks_yz: int = 1,
+                     xc: float = 1.0, radius: float = 0.125, u_in: float = 0.04, tau: float = 0.55,
+                     refine_lo: float = None, refine_hi: float = None, ramp_end_t: float = 0.0) -> Case:
+    """Channel with a z-spanning cylinder on a 2-level brick forest.
+
+    Coarse blocks at ``base_level`` fill [0,bricks_x]x[0,1]x[0,1]; blocks
+    overlapping x in [xc-2D, xc+6D] are refined once.  Inlet x-min =
+    interpolated velocity (uniform u_in), outlet x-max = interpolated pressure
+    (rho 1), lateral faces have no spec (default no-slip -> kBounceBack,
+    bc_installer.h:44-58).  Cylinder cells are solid, its surface cells
+    kBouzidi; q is the exact ray-circle fraction of every cut fluid link.
+    A single relaxation rate is used on all levels (time_loop.h:183-187)."""
+    dom = np.array([0.0, 0.0, 0.0, float(bricks_x) / bricks_yz, 1.0, 1.0])
+    f = Forest(dom, (bricks_x, bricks_yz, bricks_yz))
+    f.refine_uniform(base_level)
+    d = 2.0 * radius
+    lo = xc - 2.0 * d if refine_lo is None else refine_lo
+    hi = xc + 6.0 * d if refine_hi is None else refine_hi
+    if fine:
+        eps = 1e-9
+        f.refine_box([lo + eps, -1.0, -1.0, hi - eps, 2.0, 2.0], base_level + 1)
+        f.balance()
+    topo = f.topology(n, 1)
+    X, Y, Z, cw = cell_centres(topo)
+    inside = (X - xc) ** 2 + (Y - 0.5) ** 2 < radius * radius
+    nb = topo.num_blocks
+    # material: solid inside; solid with a fluid D3Q19 neighbour -> boundary
+    material = np.where(inside, MAT_SOLID, MAT_FLUID).astype(np.int64)
+    surf = np.zeros_like(inside)
+    for i in range(1, 19):
+        px = X + C[i, 0] * cw[:, None, None, None]
+        py = Y + C[i, 1] * cw[:, None, None, None]
+        nin = (px - xc) ** 2 + (py - 0.5) ** 2 < radius * radius
+        surf |= inside & ~nin
+    material[surf] = MAT_BOUNDARY
+    kinds = material_to_kinds(material)
+    specs = [abi.make_spec(abi.XMIN, abi.BC_INTERP_VELOCITY, inlet=abi.INLET_UNIFORM,
+                           inlet_u=(u_in, 0.0, 0.0), ramp_end_t=ramp_end_t),
+             abi.make_spec(abi.XMAX, abi.BC_INTERP_PRESSURE, rho_target=1.0)]
+    stamp_tree_boundary(kinds, topo.tree_boundary, specs)
+    # Bouzidi links of material-fluid cells whose in-block pull source x - c_i
+    # is solid-like (bouzidi_link_data.cpp:188-225), q in (1e-8, 1-1e-8)
+    links, qs = [], []
+    fluid = material == MAT_FLUID
+    for i in range(1, 19):
+        sx = np.arange(n)[:, None, None] - C[i, 0]
+        sy = np.arange(n)[None, :, None] - C[i, 1]
+        sz = np.arange(n)[None, None, :] - C[i, 2]
+        inb = (sx >= 0) & (sx < n) & (sy >= 0) & (sy < n) & (sz >= 0) & (sz < n)
+        sxc, syc, szc = np.clip(sx, 0, n - 1), np.clip(sy, 0, n - 1), np.clip(sz, 0, n - 1)
+        src_solid = material[:, sxc, syc, szc] != MAT_FLUID
+        cand = fluid & inb[None] & src_solid
+        if not cand.any():
+            continue
+        bb, ix, iy, iz = np.nonzero(cand)
+        s = _ray_circle(X[bb, ix, iy, iz], Y[bb, ix, iy, iz], -C[i, 0] * cw[bb], -C[i, 1] * cw[bb],
+                        xc, 0.5, radius)
+        ok = np.isfinite(s) & (s > 1e-8) & (s < 1.0 - 1e-8)
+        cell = (ix * n + iy) * n + iz
+        links.append(np.stack([bb[ok], cell[ok], np.full(ok.sum(), i)], axis=1))
+        qs.append(s[ok])
+    link = np.concatenate(links).astype(np.int32) if links else np.zeros((0, 3), np.int32)
+    q = np.concatenate(qs) if qs else np.zeros(0)
+    fine_dx = float(cw.min())
+    pad = 3.0 * fine_dx
+    bbox = np.array([xc - radius - pad, 0.5 - radius - pad, 0.0 - pad,
+                     xc + radius + pad, 0.5 + radius + pad, 1.0 + pad])
+    area = (d / fine_dx) * (1.0 / fine_dx)  # (D/dx)(z_span/dx), test_cylinder3d_amr.cpp:99-105
+    return Case(name="cylinder", topo=topo, omega=1.0 / tau, handler=abi.HANDLER_CONCRETE,
+                const_rho=0, bouzidi=1, snapshot=1, specs=specs, kinds=kinds, link=link, q=q,
+                init_mask=1 << abi.KIND_BULK, drag_bbox=bbox, area=area, u_ref=u_in,
+                info={"fine_dx": fine_dx, "D": d, "material_fluid": fluid})

@@ -1,0 +1,1182 @@
+// C ABI of the device solver (include/octlb_gpu.h): table flattening once per
+// adapt, the recursive level subcycle of TimeLoop (time_loop.h:201-246) as a
+// launch sequence on one HIP stream, hipGraph capture of a coarse step, and
+// FP64 drag / mass reductions.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <array>
+#include <cmath>
+#include <cstring>
+#include <map>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "lbm_device.h"
+#include "octlb_gpu.h"
+#include "octlb_internal.h"
+
+namespace octlb {
+
+namespace {
+thread_local std::string g_last_error;
+
+struct HipError : std::runtime_error {
+  using std::runtime_error::runtime_error;
+};
+struct ArgError : std::runtime_error {
+  using std::runtime_error::runtime_error;
+};
+struct StateError : std::runtime_error {
+  using std::runtime_error::runtime_error;
+};
+
+#define HIP_CHECK(expr)                                                                          \
+  do {                                                                                           \
+    hipError_t _e = (expr);                                                                      \
+    if (_e != hipSuccess)                                                                        \
+      throw HipError(std::string(#expr) + ": " + hipGetErrorString(_e) + " (api.hip:" +         \
+                     std::to_string(__LINE__) + ")");                                            \
+  } while (0)
+
+constexpr int kOpposite[6] = {1, 0, 3, 2, 5, 4};
+
+// Halo slot order of the blocked shell: padded (10^3) slots that are not core,
+// in padded linear order (same enumeration as the device apron table).
+std::vector<int> ShellSlots() {
+  std::vector<int> s;
+  for (int x = 0; x < kBox; ++x)
+    for (int y = 0; y < kBox; ++y)
+      for (int z = 0; z < kBox; ++z) {
+        const bool in = x >= 1 && x <= kTile && y >= 1 && y <= kTile && z >= 1 && z <= kTile;
+        if (!in) s.push_back((x * kBox + y) * kBox + z);
+      }
+  return s;
+}
+
+template <typename P>
+P* DevAlloc(size_t count) {
+  if (count == 0) count = 1;
+  void* p = nullptr;
+  HIP_CHECK(hipMalloc(&p, count * sizeof(P)));
+  return static_cast<P*>(p);
+}
+
+template <typename P>
+void Upload(P* dst, const std::vector<P>& src, hipStream_t s) {
+  if (!src.empty()) HIP_CHECK(hipMemcpyAsync(dst, src.data(), src.size() * sizeof(P), hipMemcpyHostToDevice, s));
+}
+
+}  // namespace
+
+int SetError(int code, const std::string& msg) {
+  g_last_error = msg;
+  return code;
+}
+
+}  // namespace octlb
+
+using namespace octlb;
+
+struct octlb_dev {
+  // ---- configuration
+  int nb = 0, n = 0, h = 0, NP = 0, ncore = 0, NS = 0, plane = 0;
+  bool padded = false;
+  int prec = 64;
+  size_t elem = 8;
+  SimConst sc{};
+  int max_level = 0;
+  bool snapshot = false;
+  std::vector<int> level;
+  std::vector<double> bounds;
+  std::vector<int> same;
+  std::vector<int> tb;
+  std::vector<octlb_cf_face> cf;
+  std::vector<int> tb_blocks;
+  std::vector<std::array<int, 6>> fn;
+  std::vector<BlockTab> bt;
+  std::vector<uint8_t> kinds;
+  std::map<int64_t, double> links;  // (block*ncore + cell)*19 + pop -> q
+  int nshell = 0;
+  // ---- coupling plan (level_coupler.cpp:70-149)
+  std::vector<int> plan;  // 12 ints per point
+  int npts = 0;
+  std::vector<std::vector<int>> prolong_active;   // per coarse level
+  std::vector<std::vector<int>> restrict_active;  // per fine level
+  // ---- tiles per level
+  std::vector<Tile> tiles;
+  std::vector<int> tile_begin, tile_count;
+  // ---- device
+  hipStream_t stream = nullptr;
+  void* pops[2] = {nullptr, nullptr};
+  void* shells[2] = {nullptr, nullptr};
+  uint8_t* d_kind = nullptr;
+  BlockTab* d_bt = nullptr;
+  Tile* d_tiles = nullptr;
+  int* d_level = nullptr;
+  uint32_t* d_qmask = nullptr;
+  int* d_qoff = nullptr;
+  double* d_qval = nullptr;
+  uint32_t* d_dmask = nullptr;
+  int* d_doff = nullptr;
+  void* d_snap = nullptr;
+  int4* d_dlinks = nullptr;
+  int* d_dblock_begin = nullptr;
+  double* d_dterm = nullptr;
+  uint8_t* d_duse = nullptr;
+  double* d_dblock_out = nullptr;
+  int ndlinks = 0, ndblocks = 0;
+  double* d_stats = nullptr;
+  double* d_avg = nullptr;
+  int* d_step = nullptr;
+  double* d_out = nullptr;
+  double* d_mass_partial = nullptr;
+  int* d_plan = nullptr;
+  void* d_prev = nullptr;
+  int* d_prolong = nullptr;
+  std::vector<int> prolong_off;
+  int* d_restrict = nullptr;
+  std::vector<int> restrict_off;
+  int2* d_fd_cells = nullptr;
+  int* d_fd_smap = nullptr;
+  int* d_fd_index = nullptr;
+  void* d_fd_post = nullptr;
+  int fd_phase[4] = {0, 0, 0, 0};
+  std::vector<void*> owned;
+  std::vector<void*> prep_owned;  // tables rebuilt by Prepare()
+  // ---- state
+  int par[kMaxLevels] = {0};
+  bool prepared = false;
+  bool coupler_ready = false;
+  int step = 0;
+  // ---- timing of the fused kernel
+  bool timing = false;
+  bool capturing = false;
+  std::vector<hipEvent_t> ev;
+  size_t ev_used = 0;
+  double tile_ms = 0.0;
+  long long tile_launches = 0;
+  double tile_updates = 0.0;
+  // ---- graphs (one per level-0 parity)
+  hipGraphExec_t gexec[2] = {nullptr, nullptr};
+
+  template <typename P>
+  P* Alloc(size_t count) {
+    P* p = DevAlloc<P>(count);
+    owned.push_back(p);
+    return p;
+  }
+  void* AllocBytes(size_t bytes) {
+    void* p = nullptr;
+    HIP_CHECK(hipMalloc(&p, bytes == 0 ? 1 : bytes));
+    owned.push_back(p);
+    return p;
+  }
+  template <typename P>
+  P* PAlloc(size_t count) {
+    P* p = DevAlloc<P>(count);
+    prep_owned.push_back(p);
+    return p;
+  }
+  void* PAllocBytes(size_t bytes) {
+    void* p = nullptr;
+    HIP_CHECK(hipMalloc(&p, bytes == 0 ? 1 : bytes));
+    prep_owned.push_back(p);
+    return p;
+  }
+  void FreePrepared() {
+    for (void* p : prep_owned) hipFree(p);
+    prep_owned.clear();
+  }
+  void FreeAll() {
+    FreePrepared();
+    for (auto& g : gexec)
+      if (g) hipGraphExecDestroy(g);
+    for (hipEvent_t e : ev) hipEventDestroy(e);
+    for (void* p : owned) hipFree(p);
+    owned.clear();
+    if (stream) hipStreamDestroy(stream);
+    stream = nullptr;
+  }
+  ParArray Par() const {
+    ParArray p{};
+    for (int l = 0; l < kMaxLevels; ++l) p.v[l] = par[l];
+    return p;
+  }
+  char* Core(int buf, int b) const {
+    return static_cast<char*>(pops[buf]) + static_cast<size_t>(b) * kQ * plane * elem;
+  }
+  void InvalidateGraphs() {
+    for (auto& g : gexec) {
+      if (g) hipGraphExecDestroy(g);
+      g = nullptr;
+    }
+  }
+
+  void Build(const octlb_topology* t, const octlb_params* p);
+  void Prepare();
+  template <typename T>
+  void Level(int l);
+  template <typename T>
+  void CoarseStep();
+  void Step();
+  void Sync() { HIP_CHECK(hipStreamSynchronize(stream)); }
+};
+
+namespace {
+
+DevSpec ResolveSpec(const octlb_params* p, int face) {
+  DevSpec d{};
+  d.type = OCTLB_BC_NO_SLIP;
+  d.rho_target = 1.0;
+  d.inlet = OCTLB_INLET_NONE;
+  d.inlet_face = face;
+  for (int i = 0; i < p->num_specs; ++i) {
+    const octlb_bc_spec& s = p->specs[i];
+    if (s.face != face) continue;  // FindSpec: first match (interpolated_velocity.h:38-45)
+    d.type = s.type;
+    d.inlet = s.inlet;
+    d.inlet_face = s.inlet_face;
+    d.n0 = s.n_inplane0;
+    d.n1 = s.n_inplane1;
+    for (int k = 0; k < 3; ++k) {
+      d.u_wall[k] = s.u_wall[k];
+      d.inlet_u[k] = s.inlet_u[k];
+    }
+    d.rho_target = s.rho_target;
+    d.u_peak = s.u_peak;
+    d.lo0 = s.lo0;
+    d.hi0 = s.hi0;
+    d.lo1 = s.lo1;
+    d.hi1 = s.hi1;
+    d.ramp_end_t = s.ramp_end_t;
+    break;
+  }
+  return d;
+}
+
+}  // namespace
+
+void octlb_dev::Build(const octlb_topology* t, const octlb_params* p) {
+  nb = t->num_blocks;
+  n = t->n;
+  h = t->halo;
+  if (nb < 1) throw ArgError("topology has no blocks");
+  if (n < 4 || (n % 2) != 0) throw ArgError("block edge n must be even and >= 4");
+  if (h < 1) throw ArgError("halo width must be >= 1");
+  if (p->precision != 64 && p->precision != 32) throw ArgError("precision must be 64 or 32");
+  if (!(p->omega > 0.0)) throw ArgError("omega must be positive");
+  NP = n + 2 * h;
+  ncore = n * n * n;
+  NS = NP * NP * NP;
+  padded = !(n == kTile && h == 1);
+  if (padded && (t->num_same_level > 0 || t->num_coarse_fine > 0))
+    throw ArgError("blocks other than 8^3 with halo 1 are supported only for forests without block faces");
+  plane = padded ? NS : ncore;
+  prec = p->precision;
+  elem = prec == 64 ? sizeof(double) : sizeof(float);
+  snapshot = p->snapshot != 0;
+  for (int f = 0; f < 6; ++f) sc.spec[f] = ResolveSpec(p, f);
+  sc.omega = p->omega;
+  sc.n = n;
+  sc.h = h;
+  sc.NP = NP;
+  sc.ncore = ncore;
+  sc.NS = NS;
+  sc.plane = plane;
+  sc.padded = padded ? 1 : 0;
+  sc.handler = p->handler;
+  sc.const_rho = p->const_rho;
+  sc.bouzidi = p->bouzidi;
+
+  level.assign(t->level, t->level + nb);
+  bounds.assign(t->bounds, t->bounds + static_cast<size_t>(nb) * 6);
+  max_level = 0;
+  for (int b = 0; b < nb; ++b) {
+    if (level[b] < 0 || level[b] >= kMaxLevels) throw ArgError("block level out of range");
+    max_level = std::max(max_level, level[b]);
+  }
+  same.assign(t->same_level, t->same_level + static_cast<size_t>(t->num_same_level) * 3);
+  tb.assign(t->tree_boundary, t->tree_boundary + static_cast<size_t>(t->num_tree_boundary) * 2);
+  cf.assign(t->coarse_fine, t->coarse_fine + t->num_coarse_fine);
+
+  // same-level face neighbours (GhostSchedule face_nbr, ghost_schedule.h:61-68)
+  fn.assign(nb, {-1, -1, -1, -1, -1, -1});
+  for (size_t e = 0; e < same.size() / 3; ++e) {
+    const int a = same[3 * e], d = same[3 * e + 1], r = same[3 * e + 2];
+    if (a < 0 || a >= nb || r < 0 || r >= nb || d < 0 || d > 5) throw ArgError("same-level face out of range");
+    if (level[a] != level[r]) throw ArgError("same-level face joins blocks of different levels");
+    fn[a][d] = r;
+  }
+  for (size_t e = 0; e < tb.size() / 2; ++e) {
+    const int b = tb[2 * e];
+    if (b < 0 || b >= nb || tb[2 * e + 1] < 0 || tb[2 * e + 1] > 5) throw ArgError("tree-boundary face out of range");
+    if (std::find(tb_blocks.begin(), tb_blocks.end(), b) == tb_blocks.end()) tb_blocks.push_back(b);
+  }
+
+  // Block table: exchanged face and edge neighbours.  The edge ghost (e1,e2)
+  // of block B is filled from fn[fn[B][e2]][e1] or fn[fn[B][e1]][e2]
+  // (composition of two same-level faces, ghost_schedule.h:69-91).
+  bt.assign(nb, BlockTab{});
+  nshell = 0;
+  for (int b = 0; b < nb; ++b) {
+    BlockTab& r = bt[b];
+    r.level = level[b];
+    r.flags = 0;
+    for (int d = 0; d < 6; ++d) r.nbr[d] = fn[b][d];
+    const int pairs[3][2] = {{0, 1}, {0, 2}, {1, 2}};
+    for (int pi = 0; pi < 3; ++pi)
+      for (int s1 = 0; s1 < 2; ++s1)
+        for (int s2 = 0; s2 < 2; ++s2) {
+          const int e1 = pairs[pi][0] * 2 + s1, e2 = pairs[pi][1] * 2 + s2;
+          int a = -1;
+          // the later of the two composition orders wins when both exist
+          const int via2 = fn[b][e2];
+          if (via2 >= 0 && fn[via2][e1] >= 0) a = fn[via2][e1];
+          const int via1 = fn[b][e1];
+          if (via1 >= 0 && fn[via1][e2] >= 0) a = fn[via1][e2];
+          r.nbr[6 + pi * 4 + s1 * 2 + s2] = a;
+        }
+    bool needs_shell = false;
+    for (int k = 0; k < 18; ++k) needs_shell |= (r.nbr[k] < 0);
+    r.shell = (!padded && needs_shell) ? nshell++ : -1;
+    const double* bb = &bounds[static_cast<size_t>(b) * 6];
+    r.origin[0] = bb[0];
+    r.origin[1] = bb[1];
+    r.origin[2] = bb[2];
+    r.cw = (bb[3] - bb[0]) / static_cast<double>(n);  // cylinder3d_case.h:404-409
+  }
+
+  // coupling plan
+  plan.clear();
+  for (const octlb_cf_face& f : cf) {
+    if (f.coarse_id < 0 || f.coarse_id >= nb) throw ArgError("coarse-fine face out of range");
+    const double* cb = f.coarse_bounds;
+    const double cc[3] = {0.5 * (cb[0] + cb[3]), 0.5 * (cb[1] + cb[4]), 0.5 * (cb[2] + cb[5])};
+    const int axis = f.normal / 2, is_max = f.normal & 1;
+    const int t0 = (axis + 1) % 3, t1 = (axis + 2) % 3;
+    const int cni = is_max ? n - 1 : 0, fni = is_max ? 0 : n - 1;
+    const int half = n / 2;
+    for (int sl = 0; sl < 4; ++sl) {
+      if (f.fine_ids[sl] < 0 || f.fine_ids[sl] >= nb) throw ArgError("coarse-fine face out of range");
+      const double* fb = f.fine_bounds[sl];
+      const double fc[3] = {0.5 * (fb[0] + fb[3]), 0.5 * (fb[1] + fb[4]), 0.5 * (fb[2] + fb[5])};
+      const int b0 = (fc[t0] < cc[t0] ? 0 : 1) * half, b1 = (fc[t1] < cc[t1] ? 0 : 1) * half;
+      for (int a0 = b0; a0 < b0 + half; ++a0)
+        for (int a1 = b1; a1 < b1 + half; ++a1) {
+          int ci[3], fi[3];
+          ci[axis] = cni;
+          fi[axis] = fni;
+          ci[t0] = a0;
+          ci[t1] = a1;
+          fi[t0] = a0 - b0;
+          fi[t1] = a1 - b1;
+          const int row[12] = {f.coarse_id, ci[0], ci[1], ci[2], f.fine_ids[sl], fi[0], fi[1],
+                               fi[2],       f.normal, f.coarse_level, f.fine_level, sl};
+          plan.insert(plan.end(), row, row + 12);
+        }
+    }
+  }
+  npts = static_cast<int>(plan.size() / 12);
+  // LevelRangeBegin/End (level_coupler.cpp:309-323): one [min,max+1) range
+  // per coarse level; points inside a range are prolonged in plan order, so
+  // the last point writing a fine cell wins.
+  prolong_active.assign(kMaxLevels, {});
+  restrict_active.assign(kMaxLevels, {});
+  {
+    int maxc = -1;
+    for (int i = 0; i < npts; ++i) maxc = std::max(maxc, plan[12 * i + 9]);
+    for (int l = 0; l <= maxc; ++l) {
+      int beg = npts, end = 0;
+      for (int i = 0; i < npts; ++i)
+        if (plan[12 * i + 9] == l) {
+          beg = std::min(beg, i);
+          end = std::max(end, i + 1);
+        }
+      std::map<int64_t, int> last;
+      for (int i = beg; i < end; ++i) {
+        const int* r = &plan[12 * i];
+        last[static_cast<int64_t>(r[4]) * ncore + (r[5] * n + r[6]) * n + r[7]] = i;
+      }
+      for (int i = beg; i < end; ++i) {
+        const int* r = &plan[12 * i];
+        if (last[static_cast<int64_t>(r[4]) * ncore + (r[5] * n + r[6]) * n + r[7]] == i)
+          prolong_active[l].push_back(i);
+      }
+    }
+    for (int fl = 1; fl <= max_level; ++fl) {
+      std::map<int64_t, int> last;
+      for (int i = 0; i < npts; ++i) {
+        const int* r = &plan[12 * i];
+        if (r[10] != fl) continue;
+        last[static_cast<int64_t>(r[0]) * ncore + (r[1] * n + r[2]) * n + r[3]] = i;
+      }
+      for (int i = 0; i < npts; ++i) {
+        const int* r = &plan[12 * i];
+        if (r[10] != fl) continue;
+        if (last[static_cast<int64_t>(r[0]) * ncore + (r[1] * n + r[2]) * n + r[3]] == i)
+          restrict_active[fl].push_back(i);
+      }
+    }
+  }
+
+  // tiles per level (octants_by_level_, time_loop.h:155-160)
+  tile_begin.assign(kMaxLevels, 0);
+  tile_count.assign(kMaxLevels, 0);
+  tiles.clear();
+  const int tpa = padded ? (NP + kTile - 1) / kTile : 1;
+  for (int l = 0; l <= max_level; ++l) {
+    tile_begin[l] = static_cast<int>(tiles.size());
+    for (int b = 0; b < nb; ++b) {
+      if (level[b] != l) continue;
+      for (int x = 0; x < tpa; ++x)
+        for (int y = 0; y < tpa; ++y)
+          for (int z = 0; z < tpa; ++z) tiles.push_back(Tile{b, x * kTile, y * kTile, z * kTile});
+    }
+    tile_count[l] = static_cast<int>(tiles.size()) - tile_begin[l];
+  }
+
+  kinds.assign(static_cast<size_t>(nb) * ncore, 0);
+
+  // ---- device allocation
+  HIP_CHECK(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+  const size_t pop_bytes = static_cast<size_t>(nb) * kQ * plane * elem;
+  for (int k = 0; k < 2; ++k) {
+    pops[k] = AllocBytes(pop_bytes);
+    HIP_CHECK(hipMemsetAsync(pops[k], 0, pop_bytes, stream));
+    const size_t sb = static_cast<size_t>(nshell) * kQ * kApron * elem;
+    shells[k] = AllocBytes(sb);
+    if (sb) HIP_CHECK(hipMemsetAsync(shells[k], 0, sb, stream));
+  }
+  d_kind = Alloc<uint8_t>(kinds.size());
+  HIP_CHECK(hipMemsetAsync(d_kind, 0, kinds.size(), stream));
+  d_bt = Alloc<BlockTab>(nb);
+  Upload(d_bt, bt, stream);
+  d_tiles = Alloc<Tile>(tiles.size());
+  Upload(d_tiles, tiles, stream);
+  d_level = Alloc<int>(nb);
+  Upload(d_level, level, stream);
+  d_stats = Alloc<double>(2 * tiles.size());
+  d_avg = Alloc<double>(1);
+  const double one = 1.0;
+  HIP_CHECK(hipMemcpyAsync(d_avg, &one, sizeof(double), hipMemcpyHostToDevice, stream));
+  d_step = Alloc<int>(1);
+  HIP_CHECK(hipMemsetAsync(d_step, 0, sizeof(int), stream));
+  d_out = Alloc<double>(8);
+  d_mass_partial = Alloc<double>(nb);
+  d_plan = Alloc<int>(plan.size());
+  Upload(d_plan, plan, stream);
+  d_prev = AllocBytes(static_cast<size_t>(std::max(npts, 1)) * kMacro * elem);
+  std::vector<int> cat;
+  prolong_off.assign(kMaxLevels + 1, 0);
+  for (int l = 0; l < kMaxLevels; ++l) {
+    prolong_off[l] = static_cast<int>(cat.size());
+    cat.insert(cat.end(), prolong_active[l].begin(), prolong_active[l].end());
+  }
+  prolong_off[kMaxLevels] = static_cast<int>(cat.size());
+  d_prolong = Alloc<int>(cat.size());
+  Upload(d_prolong, cat, stream);
+  cat.clear();
+  restrict_off.assign(kMaxLevels + 1, 0);
+  for (int l = 0; l < kMaxLevels; ++l) {
+    restrict_off[l] = static_cast<int>(cat.size());
+    cat.insert(cat.end(), restrict_active[l].begin(), restrict_active[l].end());
+  }
+  restrict_off[kMaxLevels] = static_cast<int>(cat.size());
+  d_restrict = Alloc<int>(cat.size());
+  Upload(d_restrict, cat, stream);
+  Sync();
+}
+
+// Tables that depend on the BcKind bytes and Bouzidi links.
+void octlb_dev::Prepare() {
+  Sync();
+  FreePrepared();
+  if (!links.empty() && !sc.bouzidi) throw StateError("Bouzidi links set on a solver built without them");
+  if (!links.empty() && padded) throw StateError("Bouzidi links need 8^3 blocks with halo 1");
+  for (BlockTab& r : bt) r.flags = 0;
+  // ---- Bouzidi links -> per-cell mask + compact q list (block, cell, pop order)
+  std::vector<uint32_t> qmask(static_cast<size_t>(nb) * ncore, 0);
+  std::vector<int> qoff(static_cast<size_t>(nb) * ncore, 0);
+  std::vector<double> qval;
+  {
+    auto it = links.begin();
+    for (size_t cellg = 0; cellg < qmask.size(); ++cellg) {
+      qoff[cellg] = static_cast<int>(qval.size());
+      while (it != links.end() && it->first / kQ == static_cast<int64_t>(cellg)) {
+        const int pop = static_cast<int>(it->first % kQ);
+        qmask[cellg] |= 1u << pop;
+        qval.push_back(it->second);
+        bt[cellg / ncore].flags |= kFlagLinks;
+        ++it;
+      }
+    }
+  }
+  // ---- drag links: kBulk cells with an in-block kSolid/kBouzidi neighbour at
+  // x + c_i, in the order of drag_postprocessor.h:77-110 (iz, iy, ix, iPop)
+  std::vector<uint32_t> dmask(static_cast<size_t>(nb) * ncore, 0);
+  std::vector<int> doff(static_cast<size_t>(nb) * ncore, 0);
+  std::vector<int4> dlinks;
+  std::vector<int> dblock_begin;
+  {
+    std::vector<std::vector<std::pair<int, int>>> per_cell(static_cast<size_t>(nb) * ncore);
+    int nsnap = 0;
+    for (int b = 0; b < nb; ++b) {
+      const uint8_t* kb = &kinds[static_cast<size_t>(b) * ncore];
+      bool any = false;
+      const size_t first = dlinks.size();
+      for (int iz = 0; iz < n; ++iz)
+        for (int iy = 0; iy < n; ++iy)
+          for (int ix = 0; ix < n; ++ix) {
+            const int c = (ix * n + iy) * n + iz;
+            if (kb[c] != kBulk) continue;
+            for (int i = 1; i < kQ; ++i) {
+              const int x = ix + Cx(i), y = iy + Cy(i), z = iz + Cz(i);
+              if (x < 0 || x >= n || y < 0 || y >= n || z < 0 || z >= n) continue;
+              const int nk = kb[(x * n + y) * n + z];
+              if (nk != kSolid && nk != kBouzidi) continue;
+              dmask[static_cast<size_t>(b) * ncore + c] |= 1u << i;
+              dlinks.push_back(make_int4(b, c, i, -1));
+              any = true;
+            }
+          }
+      if (any) {
+        dblock_begin.push_back(static_cast<int>(first));
+        // snapshot slots: per cell, pops ascending (matches the kernel's packing)
+        for (int c = 0; c < ncore; ++c) {
+          const uint32_t m = dmask[static_cast<size_t>(b) * ncore + c];
+          doff[static_cast<size_t>(b) * ncore + c] = nsnap;
+          nsnap += __builtin_popcount(m);
+        }
+        for (size_t j = first; j < dlinks.size(); ++j) {
+          const int c = dlinks[j].y, i = dlinks[j].z;
+          const uint32_t m = dmask[static_cast<size_t>(b) * ncore + c];
+          dlinks[j].w = doff[static_cast<size_t>(b) * ncore + c] + __builtin_popcount(m & ((1u << i) - 1u));
+        }
+        if (snapshot) bt[b].flags |= kFlagDrag;
+      }
+    }
+    dblock_begin.push_back(static_cast<int>(dlinks.size()));
+    ndlinks = static_cast<int>(dlinks.size());
+    ndblocks = static_cast<int>(dblock_begin.size()) - 1;
+    d_snap = PAllocBytes(static_cast<size_t>(std::max(nsnap, 1)) * elem);
+    HIP_CHECK(hipMemsetAsync(d_snap, 0, static_cast<size_t>(std::max(nsnap, 1)) * elem, stream));
+  }
+  // ---- shell liveness: a non-exchanged halo slot is read by some core cell
+  if (!padded) {
+    for (int b = 0; b < nb; ++b) {
+      BlockTab& r = bt[b];
+      if (r.shell < 0) continue;
+      const uint8_t* kb = &kinds[static_cast<size_t>(b) * ncore];
+      bool live = false;
+      for (int c = 0; c < ncore && !live; ++c) {
+        const int ix = c / (n * n), iy = (c / n) % n, iz = c % n;
+        if (ix > 0 && ix < n - 1 && iy > 0 && iy < n - 1 && iz > 0 && iz < n - 1) continue;
+        const int k = kb[c];
+        if (sc.bouzidi && (k == kSolid || ReflectsOnPull(k))) continue;
+        for (int i = 1; i < kQ && !live; ++i) {
+          const int x = ix - Cx(i), y = iy - Cy(i), z = iz - Cz(i);
+          const int ox = (x < 0 || x >= n), oy = (y < 0 || y >= n), oz = (z < 0 || z >= n);
+          const int nout = ox + oy + oz;
+          if (nout == 0) continue;
+          int nbk;
+          if (nout == 1) {
+            const int axis = ox ? 0 : (oy ? 1 : 2);
+            const int v = axis == 0 ? x : (axis == 1 ? y : z);
+            nbk = r.nbr[axis * 2 + (v >= n ? 1 : 0)];
+          } else {
+            const int a1 = ox ? 0 : 1, a2 = oz ? 2 : 1;
+            const int v1 = a1 == 0 ? x : y, v2 = a2 == 2 ? z : y;
+            const int pair = (a1 == 0) ? (a2 == 1 ? 0 : 1) : 2;
+            nbk = r.nbr[6 + pair * 4 + (v1 >= n ? 1 : 0) * 2 + (v2 >= n ? 1 : 0)];
+          }
+          if (nbk < 0) live = true;
+        }
+      }
+      if (live) r.flags |= kFlagShellLive;
+    }
+  }
+  // ---- post-stream FD cell lists (concrete handler, tree-boundary blocks)
+  std::vector<int2> fd_cells[3];
+  std::vector<int> fd_index(nb, -1);
+  std::vector<int> smap;
+  if (sc.handler == OCTLB_HANDLER_CONCRETE) {
+    int nfd = 0;
+    for (int b : tb_blocks) fd_index[b] = nfd++;
+    smap.assign(static_cast<size_t>(std::max(nfd, 1)) * ncore, -1);
+    for (int b : tb_blocks) {
+      const uint8_t* kb = &kinds[static_cast<size_t>(b) * ncore];
+      for (int c = 0; c < ncore; ++c) {
+        if (!IsFd(kb[c])) continue;
+        const int ix = c / (n * n), iy = (c / n) % n, iz = c % n;
+        const int fc = (ix == 0) + (ix == n - 1) + (iy == 0) + (iy == n - 1) + (iz == 0) + (iz == n - 1);
+        const int ph = fc == 3 ? 2 : (fc == 2 ? 1 : 0);
+        fd_cells[ph].push_back(make_int2(b, c));
+      }
+    }
+    fd_phase[0] = 0;
+    fd_phase[1] = static_cast<int>(fd_cells[0].size());
+    fd_phase[2] = fd_phase[1] + static_cast<int>(fd_cells[1].size());
+    fd_phase[3] = fd_phase[2] + static_cast<int>(fd_cells[2].size());
+    std::vector<int2> all;
+    for (int ph = 0; ph < 3; ++ph) all.insert(all.end(), fd_cells[ph].begin(), fd_cells[ph].end());
+    for (size_t e = 0; e < all.size(); ++e)
+      smap[static_cast<size_t>(fd_index[all[e].x]) * ncore + all[e].y] = static_cast<int>(e);
+    d_fd_cells = PAlloc<int2>(all.size());
+    Upload(d_fd_cells, all, stream);
+    d_fd_smap = PAlloc<int>(smap.size());
+    Upload(d_fd_smap, smap, stream);
+    d_fd_index = PAlloc<int>(nb);
+    Upload(d_fd_index, fd_index, stream);
+    d_fd_post = PAllocBytes(std::max<size_t>(all.size(), 1) * kQ * elem);
+  } else {
+    fd_phase[0] = fd_phase[1] = fd_phase[2] = fd_phase[3] = 0;
+  }
+
+  d_qmask = PAlloc<uint32_t>(qmask.size());
+  Upload(d_qmask, qmask, stream);
+  d_qoff = PAlloc<int>(qoff.size());
+  Upload(d_qoff, qoff, stream);
+  d_qval = PAlloc<double>(qval.size());
+  Upload(d_qval, qval, stream);
+  d_dmask = PAlloc<uint32_t>(dmask.size());
+  Upload(d_dmask, dmask, stream);
+  d_doff = PAlloc<int>(doff.size());
+  Upload(d_doff, doff, stream);
+  d_dlinks = PAlloc<int4>(dlinks.size());
+  Upload(d_dlinks, dlinks, stream);
+  d_dblock_begin = PAlloc<int>(dblock_begin.size());
+  Upload(d_dblock_begin, dblock_begin, stream);
+  d_dterm = PAlloc<double>(dlinks.size());
+  d_duse = PAlloc<uint8_t>(dlinks.size());
+  d_dblock_out = PAlloc<double>(3 * static_cast<size_t>(std::max(ndblocks, 1)));
+  Upload(d_bt, bt, stream);
+  Sync();
+  prepared = true;
+  InvalidateGraphs();
+}
+
+template <typename T>
+void octlb_dev::Level(int l) {
+  const int cnt = tile_count[l];
+  if (cnt > 0) {
+    TileArgs<T> a{};
+    a.tiles = d_tiles + tile_begin[l];
+    a.ntiles = cnt;
+    a.bt = d_bt;
+    a.kind = d_kind;
+    a.src = static_cast<const T*>(pops[par[l]]);
+    a.dst = static_cast<T*>(pops[par[l] ^ 1]);
+    a.shell_src = static_cast<const T*>(shells[par[l]]);
+    a.shell_dst = static_cast<T*>(shells[par[l] ^ 1]);
+    a.qmask = d_qmask;
+    a.qoff = d_qoff;
+    a.qval = d_qval;
+    a.dmask = d_dmask;
+    a.doff = d_doff;
+    a.snap = static_cast<T*>(d_snap);
+    a.stats = d_stats;
+    a.avg_rho = d_avg;
+    a.step = d_step;
+    a.want_stats = (sc.const_rho && l >= max_level) ? 1 : 0;
+    a.sc = sc;
+    const bool timed = timing && !capturing;
+    if (timed) {
+      while (ev.size() < ev_used + 2) {
+        hipEvent_t e;
+        HIP_CHECK(hipEventCreate(&e));
+        ev.push_back(e);
+      }
+      HIP_CHECK(hipEventRecord(ev[ev_used], stream));
+    }
+    LaunchTile<T>(a, stream);
+    HIP_CHECK(hipGetLastError());
+    if (timed) {
+      HIP_CHECK(hipEventRecord(ev[ev_used + 1], stream));
+      ev_used += 2;
+      tile_launches += 1;
+      const int tpa = (NP + kTile - 1) / kTile;
+      const int tpb = padded ? tpa * tpa * tpa : 1;
+      tile_updates += static_cast<double>(cnt / tpb) * static_cast<double>(ncore);
+    }
+    par[l] ^= 1;
+    if (a.want_stats) LaunchStatsFinalize<T>(d_stats, cnt, d_avg, stream);
+  }
+  if (fd_phase[3] > 0) {
+    FdArgs<T> f{};
+    f.cells = d_fd_cells;
+    f.smap = d_fd_smap;
+    f.fd_index = d_fd_index;
+    f.level = d_level;
+    f.bt = d_bt;
+    f.kind = d_kind;
+    f.pops0 = static_cast<T*>(pops[0]);
+    f.pops1 = static_cast<T*>(pops[1]);
+    f.post = static_cast<T*>(d_fd_post);
+    f.step = d_step;
+    f.par = Par();
+    f.sc = sc;
+    LaunchFd<T>(f, fd_phase, stream);
+  }
+  if (l >= max_level) return;
+  CoupleArgs<T> c{};
+  c.plan = d_plan;
+  c.pops0 = static_cast<T*>(pops[0]);
+  c.pops1 = static_cast<T*>(pops[1]);
+  c.prev = static_cast<T*>(d_prev);
+  const double tau = 1.0 / sc.omega;  // level_coupler.cpp:226-228
+  c.prolong_scale = static_cast<T>((tau - 0.25) / tau);
+  c.restrict_scale = static_cast<T>(tau / (tau - 0.25));
+  c.par = Par();
+  LaunchProlong<T>(c, d_prolong + prolong_off[l], static_cast<int>(prolong_active[l].size()), 1, stream);
+  Level<T>(l + 1);
+  c.par = Par();
+  LaunchProlong<T>(c, d_prolong + prolong_off[l], static_cast<int>(prolong_active[l].size()), 0, stream);
+  Level<T>(l + 1);
+  c.par = Par();
+  LaunchRestrict<T>(c, d_restrict + restrict_off[l + 1], static_cast<int>(restrict_active[l + 1].size()), stream);
+}
+
+template <typename T>
+void octlb_dev::CoarseStep() {
+  Level<T>(0);
+  LaunchStepInc(d_step, stream);
+}
+
+void octlb_dev::Step() {
+  if (prec == 64) {
+    CoarseStep<double>();
+  } else {
+    CoarseStep<float>();
+  }
+  ++step;
+}
+
+// ----------------------------------------------------------------- C ABI
+namespace {
+
+template <typename F>
+int Guard(F&& f) {
+  try {
+    f();
+    return OCTLB_OK;
+  } catch (const ArgError& e) {
+    return SetError(OCTLB_EINVAL, e.what());
+  } catch (const StateError& e) {
+    return SetError(OCTLB_ESTATE, e.what());
+  } catch (const HipError& e) {
+    return SetError(OCTLB_EHIP, e.what());
+  } catch (const std::bad_alloc&) {
+    return SetError(OCTLB_ENOMEM, "host allocation failed");
+  } catch (const std::exception& e) {
+    return SetError(OCTLB_EINVAL, e.what());
+  }
+}
+
+void EnsureReady(octlb_dev* d) {
+  if (!d->prepared) d->Prepare();
+  if (!d->coupler_ready) {
+    if (d->npts > 0) {
+      if (d->prec == 64) {
+        CoupleArgs<double> c{};
+        c.plan = d->d_plan;
+        c.pops0 = static_cast<double*>(d->pops[0]);
+        c.pops1 = static_cast<double*>(d->pops[1]);
+        c.prev = static_cast<double*>(d->d_prev);
+        c.par = d->Par();
+        LaunchCouplerInit<double>(c, d->npts, d->stream);
+      } else {
+        CoupleArgs<float> c{};
+        c.plan = d->d_plan;
+        c.pops0 = static_cast<float*>(d->pops[0]);
+        c.pops1 = static_cast<float*>(d->pops[1]);
+        c.prev = static_cast<float*>(d->d_prev);
+        c.par = d->Par();
+        LaunchCouplerInit<float>(c, d->npts, d->stream);
+      }
+      HIP_CHECK(hipGetLastError());
+    }
+    d->coupler_ready = true;
+  }
+}
+
+template <typename T>
+void ToHostBlock(octlb_dev* d, int b, double* aos) {
+  const int buf = d->par[d->level[b]];
+  const size_t NS = static_cast<size_t>(d->NS);
+  std::vector<T> tmp(static_cast<size_t>(kQ) * d->plane);
+  HIP_CHECK(hipMemcpy(tmp.data(), d->Core(buf, b), tmp.size() * sizeof(T), hipMemcpyDeviceToHost));
+  std::fill(aos, aos + NS * kQ, 0.0);
+  if (d->padded) {
+    for (size_t s = 0; s < NS; ++s)
+      for (int i = 0; i < kQ; ++i) aos[s * kQ + i] = static_cast<double>(tmp[i * NS + s]);
+    return;
+  }
+  for (int c = 0; c < d->ncore; ++c) {
+    const int x = c / 64, y = (c / 8) % 8, z = c % 8;
+    const size_t s = static_cast<size_t>(((x + 1) * kBox + y + 1) * kBox + z + 1);
+    for (int i = 0; i < kQ; ++i) aos[s * kQ + i] = static_cast<double>(tmp[static_cast<size_t>(i) * d->ncore + c]);
+  }
+  const int sh = d->bt[b].shell;
+  if (sh >= 0) {
+    std::vector<T> st(static_cast<size_t>(kQ) * kApron);
+    HIP_CHECK(hipMemcpy(st.data(), static_cast<char*>(d->shells[buf]) + static_cast<size_t>(sh) * kQ * kApron * sizeof(T),
+                        st.size() * sizeof(T), hipMemcpyDeviceToHost));
+    const std::vector<int> slots = ShellSlots();
+    for (int j = 0; j < kApron; ++j)
+      for (int i = 0; i < kQ; ++i) aos[static_cast<size_t>(slots[j]) * kQ + i] = static_cast<double>(st[static_cast<size_t>(i) * kApron + j]);
+  }
+}
+
+template <typename T>
+void FromHostBlock(octlb_dev* d, int b, const double* aos) {
+  const int buf = d->par[d->level[b]];
+  const size_t NS = static_cast<size_t>(d->NS);
+  std::vector<T> tmp(static_cast<size_t>(kQ) * d->plane);
+  if (d->padded) {
+    for (size_t s = 0; s < NS; ++s)
+      for (int i = 0; i < kQ; ++i) tmp[i * NS + s] = static_cast<T>(aos[s * kQ + i]);
+  } else {
+    for (int c = 0; c < d->ncore; ++c) {
+      const int x = c / 64, y = (c / 8) % 8, z = c % 8;
+      const size_t s = static_cast<size_t>(((x + 1) * kBox + y + 1) * kBox + z + 1);
+      for (int i = 0; i < kQ; ++i) tmp[static_cast<size_t>(i) * d->ncore + c] = static_cast<T>(aos[s * kQ + i]);
+    }
+  }
+  HIP_CHECK(hipMemcpy(d->Core(buf, b), tmp.data(), tmp.size() * sizeof(T), hipMemcpyHostToDevice));
+  const int sh = d->bt[b].shell;
+  if (!d->padded && sh >= 0) {
+    std::vector<T> st(static_cast<size_t>(kQ) * kApron);
+    const std::vector<int> slots = ShellSlots();
+    for (int j = 0; j < kApron; ++j)
+      for (int i = 0; i < kQ; ++i) st[static_cast<size_t>(i) * kApron + j] = static_cast<T>(aos[static_cast<size_t>(slots[j]) * kQ + i]);
+    HIP_CHECK(hipMemcpy(static_cast<char*>(d->shells[buf]) + static_cast<size_t>(sh) * kQ * kApron * sizeof(T), st.data(),
+                        st.size() * sizeof(T), hipMemcpyHostToDevice));
+  }
+}
+
+template <typename T>
+void CoreToHost(octlb_dev* d, double* out) {
+  std::vector<T> tmp(static_cast<size_t>(kQ) * d->plane);
+  for (int b = 0; b < d->nb; ++b) {
+    const int buf = d->par[d->level[b]];
+    HIP_CHECK(hipMemcpy(tmp.data(), d->Core(buf, b), tmp.size() * sizeof(T), hipMemcpyDeviceToHost));
+    for (int c = 0; c < d->ncore; ++c) {
+      const int s = CoreSlot(d->sc, c);
+      double* o = out + (static_cast<size_t>(b) * d->ncore + c) * kQ;
+      for (int i = 0; i < kQ; ++i) o[i] = static_cast<double>(tmp[static_cast<size_t>(i) * d->plane + s]);
+    }
+  }
+}
+
+template <typename T>
+void Drag(octlb_dev* d, const double* bbox, double* out) {
+  DragArgs<T> a{};
+  a.links = d->d_dlinks;
+  a.bt = d->d_bt;
+  a.pops0 = static_cast<const T*>(d->pops[0]);
+  a.pops1 = static_cast<const T*>(d->pops[1]);
+  a.snap = d->snapshot ? static_cast<const T*>(d->d_snap) : nullptr;
+  a.term = d->d_dterm;
+  a.use = d->d_duse;
+  for (int k = 0; k < 6; ++k) a.bbox[k] = bbox[k];
+  a.par = d->Par();
+  a.sc = d->sc;
+  LaunchDrag<T>(a, d->ndlinks, d->d_dblock_begin, d->ndblocks, d->d_dblock_out, d->d_out, d->stream);
+  HIP_CHECK(hipGetLastError());
+  HIP_CHECK(hipMemcpyAsync(out, d->d_out, 3 * sizeof(double), hipMemcpyDeviceToHost, d->stream));
+  d->Sync();
+}
+
+template <typename T>
+void Mass(octlb_dev* d, double* out) {
+  LaunchMass<T>(static_cast<const T*>(d->pops[0]), static_cast<const T*>(d->pops[1]), d->Par(), d->d_bt,
+                d->d_kind, d->sc, d->nb, d->d_mass_partial, d->d_out + 3, d->stream);
+  HIP_CHECK(hipGetLastError());
+  HIP_CHECK(hipMemcpyAsync(out, d->d_out + 3, sizeof(double), hipMemcpyDeviceToHost, d->stream));
+  d->Sync();
+}
+
+void CheckDev(const octlb_dev* d) {
+  if (d == nullptr) throw ArgError("null device handle");
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* octlb_last_error(void) { return g_last_error.c_str(); }
+
+int octlb_dev_create(const octlb_topology* topo, const octlb_params* params, octlb_dev** out) {
+  if (out) *out = nullptr;
+  if (!topo || !params || !out) return SetError(OCTLB_EINVAL, "null argument");
+  octlb_dev* d = new (std::nothrow) octlb_dev();
+  if (!d) return SetError(OCTLB_ENOMEM, "host allocation failed");
+  const int rc = Guard([&] { d->Build(topo, params); });
+  if (rc != OCTLB_OK) {
+    d->FreeAll();
+    delete d;
+    return rc;
+  }
+  *out = d;
+  return OCTLB_OK;
+}
+
+int octlb_dev_destroy(octlb_dev* dev) {
+  if (!dev) return OCTLB_OK;
+  if (dev->stream) hipStreamSynchronize(dev->stream);
+  dev->FreeAll();
+  delete dev;
+  return OCTLB_OK;
+}
+
+int octlb_dev_set_bc_kinds(octlb_dev* d, const uint8_t* kinds) {
+  return Guard([&] {
+    CheckDev(d);
+    if (!kinds) throw ArgError("null kinds");
+    for (size_t i = 0; i < d->kinds.size(); ++i)
+      if (kinds[i] > OCTLB_KIND_OUTFLOW) throw ArgError("BcKind byte out of range");
+    d->kinds.assign(kinds, kinds + d->kinds.size());
+    HIP_CHECK(hipMemcpyAsync(d->d_kind, d->kinds.data(), d->kinds.size(), hipMemcpyHostToDevice, d->stream));
+    d->Sync();
+    d->prepared = false;
+  });
+}
+
+int octlb_dev_set_links(octlb_dev* d, int64_t count, const int32_t* link, const double* q) {
+  return Guard([&] {
+    CheckDev(d);
+    if (count < 0 || (count > 0 && (!link || !q))) throw ArgError("bad link arrays");
+    if (!d->sc.bouzidi) throw StateError("solver built without Bouzidi links (params.bouzidi = 0)");
+    for (int64_t i = 0; i < count; ++i) {
+      const int b = link[3 * i], c = link[3 * i + 1], p = link[3 * i + 2];
+      if (b < 0 || b >= d->nb || c < 0 || c >= d->ncore || p < 0 || p >= kQ) throw ArgError("link out of range");
+      d->links[(static_cast<int64_t>(b) * d->ncore + c) * kQ + p] = q[i];
+    }
+    d->prepared = false;
+  });
+}
+
+int octlb_dev_init_equilibrium(octlb_dev* d, double rho0, const double u0[3], uint32_t mask) {
+  return Guard([&] {
+    CheckDev(d);
+    if (!u0) throw ArgError("null u0");
+    for (int buf = 0; buf < 2; ++buf) {
+      // both copies: the state is then defined whatever each level's parity
+      if (d->prec == 64) {
+        LaunchInitEq<double>(static_cast<double*>(d->pops[buf]), d->d_kind, d->nb, d->sc, rho0, u0, mask, d->stream);
+      } else {
+        LaunchInitEq<float>(static_cast<float*>(d->pops[buf]), d->d_kind, d->nb, d->sc, rho0, u0, mask, d->stream);
+      }
+    }
+    HIP_CHECK(hipGetLastError());
+    d->Sync();
+    d->coupler_ready = false;
+  });
+}
+
+int octlb_dev_init_field(octlb_dev* d, const double* rho, const double* u) {
+  return Guard([&] {
+    CheckDev(d);
+    if (!rho || !u) throw ArgError("null field");
+    const size_t nc = static_cast<size_t>(d->nb) * d->ncore;
+    double* dr = DevAlloc<double>(nc);
+    double* du = DevAlloc<double>(3 * nc);
+    HIP_CHECK(hipMemcpy(dr, rho, nc * sizeof(double), hipMemcpyHostToDevice));
+    HIP_CHECK(hipMemcpy(du, u, 3 * nc * sizeof(double), hipMemcpyHostToDevice));
+    for (int buf = 0; buf < 2; ++buf) {
+      if (d->prec == 64) {
+        LaunchInitField<double>(static_cast<double*>(d->pops[buf]), d->nb, d->sc, dr, du, d->stream);
+      } else {
+        LaunchInitField<float>(static_cast<float*>(d->pops[buf]), d->nb, d->sc, dr, du, d->stream);
+      }
+    }
+    HIP_CHECK(hipGetLastError());
+    d->Sync();
+    hipFree(dr);
+    hipFree(du);
+    d->coupler_ready = false;
+  });
+}
+
+int octlb_dev_upload_block(octlb_dev* d, int32_t b, const double* aos) {
+  return Guard([&] {
+    CheckDev(d);
+    if (b < 0 || b >= d->nb || !aos) throw ArgError("bad block");
+    d->Sync();
+    if (d->prec == 64) {
+      FromHostBlock<double>(d, b, aos);
+    } else {
+      FromHostBlock<float>(d, b, aos);
+    }
+    d->coupler_ready = false;
+  });
+}
+
+int octlb_dev_download_block(octlb_dev* d, int32_t b, double* aos) {
+  return Guard([&] {
+    CheckDev(d);
+    if (b < 0 || b >= d->nb || !aos) throw ArgError("bad block");
+    d->Sync();
+    if (d->prec == 64) {
+      ToHostBlock<double>(d, b, aos);
+    } else {
+      ToHostBlock<float>(d, b, aos);
+    }
+  });
+}
+
+int octlb_dev_download_core(octlb_dev* d, double* out) {
+  return Guard([&] {
+    CheckDev(d);
+    if (!out) throw ArgError("null output");
+    d->Sync();
+    if (d->prec == 64) {
+      CoreToHost<double>(d, out);
+    } else {
+      CoreToHost<float>(d, out);
+    }
+  });
+}
+
+int octlb_dev_advance(octlb_dev* d, int32_t steps) {
+  return Guard([&] {
+    CheckDev(d);
+    if (steps < 0) throw ArgError("negative step count");
+    EnsureReady(d);
+    d->ev_used = 0;
+    d->tile_ms = 0.0;
+    d->tile_launches = 0;
+    d->tile_updates = 0.0;
+    for (int s = 0; s < steps; ++s) d->Step();
+    HIP_CHECK(hipGetLastError());
+    d->Sync();
+    if (d->timing) {
+      double ms = 0.0;
+      for (size_t k = 0; k + 1 < d->ev_used; k += 2) {
+        float e = 0.0f;
+        HIP_CHECK(hipEventElapsedTime(&e, d->ev[k], d->ev[k + 1]));
+        ms += e;
+      }
+      d->tile_ms = ms;
+    }
+  });
+}
+
+int octlb_dev_advance_graph(octlb_dev* d, int32_t steps) {
+  return Guard([&] {
+    CheckDev(d);
+    if (steps < 0) throw ArgError("negative step count");
+    EnsureReady(d);
+    for (int s = 0; s < steps; ++s) {
+      const int p0 = d->par[0];
+      if (!d->gexec[p0]) {
+        int saved[kMaxLevels];
+        std::memcpy(saved, d->par, sizeof(saved));
+        hipGraph_t g = nullptr;
+        d->capturing = true;
+        HIP_CHECK(hipStreamBeginCapture(d->stream, hipStreamCaptureModeThreadLocal));
+        if (d->prec == 64) {
+          d->CoarseStep<double>();
+        } else {
+          d->CoarseStep<float>();
+        }
+        HIP_CHECK(hipStreamEndCapture(d->stream, &g));
+        d->capturing = false;
+        std::memcpy(d->par, saved, sizeof(saved));
+        HIP_CHECK(hipGraphInstantiate(&d->gexec[p0], g, nullptr, nullptr, 0));
+        HIP_CHECK(hipGraphDestroy(g));
+      }
+      HIP_CHECK(hipGraphLaunch(d->gexec[p0], d->stream));
+      d->par[0] ^= 1;  // finer levels step an even number of times per coarse step
+      ++d->step;
+    }
+    d->Sync();
+  });
+}
+
+int octlb_dev_synchronize(octlb_dev* d) {
+  return Guard([&] {
+    CheckDev(d);
+    d->Sync();
+  });
+}
+
+int octlb_dev_set_step(octlb_dev* d, int32_t step) {
+  return Guard([&] {
+    CheckDev(d);
+    d->step = step;
+    HIP_CHECK(hipMemcpyAsync(d->d_step, &d->step, sizeof(int), hipMemcpyHostToDevice, d->stream));
+    d->Sync();
+  });
+}
+
+int octlb_dev_drag(octlb_dev* d, const double bbox[6], double out[3]) {
+  return Guard([&] {
+    CheckDev(d);
+    if (!bbox || !out) throw ArgError("null argument");
+    if (!d->prepared) d->Prepare();
+    d->Sync();
+    if (d->prec == 64) {
+      Drag<double>(d, bbox, out);
+    } else {
+      Drag<float>(d, bbox, out);
+    }
+  });
+}
+
+int octlb_dev_mass(octlb_dev* d, double* out) {
+  return Guard([&] {
+    CheckDev(d);
+    if (!out) throw ArgError("null argument");
+    d->Sync();
+    if (d->prec == 64) {
+      Mass<double>(d, out);
+    } else {
+      Mass<float>(d, out);
+    }
+  });
+}
+
+int octlb_dev_average_rho(octlb_dev* d, double* out) {
+  return Guard([&] {
+    CheckDev(d);
+    if (!out) throw ArgError("null argument");
+    d->Sync();
+    HIP_CHECK(hipMemcpy(out, d->d_avg, sizeof(double), hipMemcpyDeviceToHost));
+  });
+}
+
+int octlb_dev_update_counts(octlb_dev* d, double* actual, double* feq) {
+  return Guard([&] {
+    CheckDev(d);
+    double a = 0.0, f = 0.0;
+    for (int b = 0; b < d->nb; ++b) {
+      const int l = d->level[b];
+      a += static_cast<double>(d->ncore) * std::ldexp(1.0, l);
+      f += static_cast<double>(d->ncore) * std::ldexp(1.0, 3 * (d->max_level - l));
+    }
+    f *= std::ldexp(1.0, d->max_level);
+    if (actual) *actual = a;
+    if (feq) *feq = f;
+  });
+}
+
+int octlb_dev_kernel_timing(octlb_dev* d, double* ms, int64_t* launches, double* updates) {
+  return Guard([&] {
+    CheckDev(d);
+    if (ms) *ms = d->tile_ms;
+    if (launches) *launches = d->tile_launches;
+    if (updates) *updates = d->tile_updates;
+  });
+}
+
+int octlb_dev_enable_timing(octlb_dev* d, int32_t on) {
+  return Guard([&] {
+    CheckDev(d);
+    d->timing = on != 0;
+  });
+}
+
+}  // extern "C"

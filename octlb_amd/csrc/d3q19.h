@@ -1,0 +1,200 @@
+// D3Q19 arithmetic shared by every kernel, written once for host and device.
+//
+// Each function keeps the operation order of the reference source it cites so
+// that an FP64 build without FMA contraction (-ffp-contract=off) reproduces
+// the reference's rounding; with T = float the same code is the FP32 path.
+#ifndef OCTLB_AMD_CSRC_D3Q19_H_
+#define OCTLB_AMD_CSRC_D3Q19_H_
+
+#include <hip/hip_runtime.h>
+
+namespace octlb {
+
+constexpr int kQ = 19;
+
+// Velocity set, opposite table and weight denominators in OpenLB order
+// (descriptor/definition/common.h:166-198).
+__host__ __device__ constexpr int Cx(int i) {
+  constexpr int v[kQ] = {0, -1, 0, 0, -1, -1, -1, -1, 0, 0, 1, 0, 0, 1, 1, 1, 1, 0, 0};
+  return v[i];
+}
+__host__ __device__ constexpr int Cy(int i) {
+  constexpr int v[kQ] = {0, 0, -1, 0, -1, 1, 0, 0, -1, -1, 0, 1, 0, 1, -1, 0, 0, 1, 1};
+  return v[i];
+}
+__host__ __device__ constexpr int Cz(int i) {
+  constexpr int v[kQ] = {0, 0, 0, -1, 0, 0, -1, 1, -1, 1, 0, 0, 1, 0, 0, 1, -1, 1, -1};
+  return v[i];
+}
+__host__ __device__ constexpr int Ci(int i, int d) { return d == 0 ? Cx(i) : (d == 1 ? Cy(i) : Cz(i)); }
+__host__ __device__ constexpr int Opp(int i) { return i == 0 ? 0 : (i <= 9 ? i + 9 : i - 9); }
+__host__ __device__ constexpr int WDen(int i) { return i == 0 ? 3 : ((i <= 3 || (i >= 10 && i <= 12)) ? 18 : 36); }
+
+// Fraction::as<T>() = T(1)/T(den) (utilities/fraction.h:72-75).
+template <typename T>
+__host__ __device__ inline T W(int i) {
+  return T(1) / T(WDen(i));
+}
+
+// BcKind (bc_kind.h:14-23).
+enum : int {
+  kBulk = 0,
+  kSolid = 1,
+  kBounceBack = 2,
+  kMovingBounceBack = 3,
+  kBouzidi = 4,
+  kVelocity = 5,
+  kPressure = 6,
+  kOutflow = 7
+};
+
+__host__ __device__ inline bool ReflectsOnPull(int k) { return k == kVelocity || k == kPressure || k == kBouzidi; }
+__host__ __device__ inline bool IsFd(int k) { return k == kVelocity || k == kPressure; }
+
+// equilibrium::secondOrder on shifted populations (dynamics/lbm.h:51-62).
+template <typename T>
+__host__ __device__ inline T Feq(int i, T rho, const T* u, T usqr) {
+  T cu = T(0);
+  cu += T(Cx(i)) * u[0];
+  cu += T(Cy(i)) * u[1];
+  cu += T(Cz(i)) * u[2];
+  const T t = W<T>(i);
+  return rho * t * (T(1) + T(3) * cu + T(3) * T(3) * T(0.5) * cu * cu - T(3) * T(0.5) * usqr) - t;
+}
+
+template <typename T>
+__host__ __device__ inline T NormSqr(const T* u) {
+  T s = T(0);
+  s += u[0] * u[0];
+  s += u[1] * u[1];
+  s += u[2] * u[2];
+  return s;
+}
+
+// CellProxy::computeRhoU: rho starts at 1 (block_lattice.h:113-125).
+template <typename T>
+__host__ __device__ inline void ProxyRhoU(const T* f, T& rho, T* u) {
+  T r = T(1);
+  u[0] = u[1] = u[2] = T(0);
+#pragma unroll
+  for (int i = 0; i < kQ; ++i) {
+    r += f[i];
+    u[0] += f[i] * T(Cx(i));
+    u[1] += f[i] * T(Cy(i));
+    u[2] += f[i] * T(Cz(i));
+  }
+  u[0] /= r;
+  u[1] /= r;
+  u[2] /= r;
+  rho = r;
+}
+
+// olb::lbm::computeRhoU: rho = sum(f) + 1 (dynamics/lbm.h:290-331).
+template <typename T>
+__host__ __device__ inline void OlbRhoU(const T* f, T& rho, T* u) {
+  T r = T(0);
+#pragma unroll
+  for (int i = 0; i < kQ; ++i) r += f[i];
+  r += T(1);
+  u[0] = u[1] = u[2] = T(0);
+#pragma unroll
+  for (int i = 0; i < kQ; ++i) {
+    u[0] += f[i] * T(Cx(i));
+    u[1] += f[i] * T(Cy(i));
+    u[2] += f[i] * T(Cz(i));
+  }
+  u[0] /= r;
+  u[1] /= r;
+  u[2] /= r;
+  rho = r;
+}
+
+// olb::lbm::bgkCollision (dynamics/lbm.h:395-403).
+template <typename T>
+__host__ __device__ inline void OlbBgk(T* f, T rho, const T* u, T omega) {
+  const T usqr = NormSqr(u);
+#pragma unroll
+  for (int i = 0; i < kQ; ++i) {
+    f[i] *= T(1) - omega;
+    f[i] += omega * Feq(i, rho, u, usqr);
+  }
+}
+
+// CollideBgkAt (block_lattice.cpp:328-341).
+template <typename T>
+__host__ __device__ inline void CollideBgkAt(T* f, T omega) {
+  T rho, u[3];
+  ProxyRhoU(f, rho, u);
+  const T usqr = NormSqr(u);
+#pragma unroll
+  for (int i = 0; i < kQ; ++i) {
+    const T fe = Feq(i, rho, u, usqr);
+    f[i] += omega * (fe - f[i]);
+  }
+}
+
+// CollideConstRhoBgkWithMacroscopic (block_lattice.h:133-151).
+template <typename T>
+__host__ __device__ inline void ConstRhoBgk(T* f, T omega, T avg, T rho, const T* u) {
+  const T usqr = NormSqr(u);
+  const T delta = T(1) - avg;
+  const T ratio = rho > T(1e-12) ? T(1) + delta / rho : T(1);
+#pragma unroll
+  for (int i = 0; i < kQ; ++i) {
+    const T fe = Feq(i, rho, u, usqr);
+    const T t = W<T>(i);
+    f[i] = ratio * (fe + t) - t + (T(1) - omega) * (f[i] - fe);
+  }
+}
+
+// equilibrium::fromPiToFneq (dynamics/lbm.h:248-270); pi = xx,xy,xz,yy,yz,zz.
+template <typename T>
+__host__ __device__ inline T PiToFneq(int i, const T* pi) {
+  T fneq = T(0);
+  int ipi = 0;
+  for (int a = 0; a < 3; ++a) {
+    for (int b = a; b < 3; ++b) {
+      T to_add = T(Ci(i, a) * Ci(i, b));
+      if (a == b) {
+        to_add -= T(1) / T(3);
+      } else {
+        to_add *= T(2);
+      }
+      to_add *= pi[ipi++];
+      fneq += to_add;
+    }
+  }
+  fneq *= W<T>(i) * T(3) * T(3) / T(2);
+  return fneq;
+}
+
+// BouzidiPostCollisionPull (boundary/bouzidi_pull.h:8-26); evaluated in FP64
+// for every storage type, as the reference does.
+__host__ __device__ inline double BouzidiPull(double f_bb_opp, double f_interior, double f_same, double q) {
+  if (q >= 1.0 - 1.0e-14) return f_bb_opp;
+  if (q <= 1.0e-14) return f_bb_opp;
+  if (q >= 0.5 - 1.0e-14) {
+    const double inv2q = 1.0 / (2.0 * q);
+    return inv2q * f_bb_opp + (1.0 - inv2q) * f_interior;
+  }
+  return 2.0 * q * f_bb_opp + (1.0 - 2.0 * q) * f_same;
+}
+
+// VelocityBoundaryRhoFromPop (interpolated_velocity.h:170-211); a zero
+// orientation takes the +1 instantiation like the reference's dispatch.
+template <typename T>
+__host__ __device__ inline T VelRhoFromPop(int direction, int orientation, const T* f, const T* u) {
+  const int o = orientation < 0 ? -1 : 1;
+  T on_wall = T(0), normal = T(0);
+#pragma unroll
+  for (int i = 0; i < kQ; ++i)
+    if (Ci(i, direction) == 0) on_wall += f[i];
+#pragma unroll
+  for (int i = 0; i < kQ; ++i)
+    if (Ci(i, direction) == o) normal += f[i];
+  return (T(2) * normal + on_wall + T(1)) / (T(1) + T(o) * u[direction]);
+}
+
+}  // namespace octlb
+
+#endif  // OCTLB_AMD_CSRC_D3Q19_H_

@@ -1,0 +1,985 @@
+// MI355X (gfx950) per-time-step AMR D3Q19 lattice-Boltzmann update: device
+// kernels.  The C ABI that drives them is in api.hip.
+//
+// One coarse step replays TimeLoop::advance_one (time_loop.h:201-246):
+//   advance(l): fused collide + halo + stream of the level-l blocks
+//               -> [ConstRho statistics] -> post-stream FD boundaries
+//               -> if l < max: prolong(half) advance(l+1) prolong(full)
+//                              advance(l+1) restrict(l+1)
+// The fused kernels keep the reference's phase semantics: they read the
+// post-stream state S of step n-1, collide the tile's cells and its one-cell
+// apron (neighbour cells are re-collided from their own pre-collide state;
+// halo slots the reference's ghost exchange does not refresh come from the
+// block's shell store), stream through LDS one population plane at a time
+// and write S of step n to the other buffer.
+#include "lbm_device.h"
+
+namespace octlb {
+
+// ------------------------------------------------------------------ apron map
+struct ApronTable {
+  short idx[kApron];
+};
+constexpr ApronTable MakeApron() {
+  ApronTable t{};
+  int k = 0;
+  for (int x = 0; x < kBox; ++x)
+    for (int y = 0; y < kBox; ++y)
+      for (int z = 0; z < kBox; ++z) {
+        const bool in = x >= 1 && x <= kTile && y >= 1 && y <= kTile && z >= 1 && z <= kTile;
+        if (!in) t.idx[k++] = static_cast<short>((x * kBox + y) * kBox + z);
+      }
+  return t;
+}
+__constant__ ApronTable kApronTab = MakeApron();
+
+__device__ inline int EdgeSlot(int a1, int s1, int a2, int s2) {
+  // a1 < a2; axis pair (x,y)=0, (x,z)=1, (y,z)=2
+  const int pair = (a1 == 0) ? (a2 == 1 ? 0 : 1) : 2;
+  return 6 + pair * 4 + s1 * 2 + s2;
+}
+
+__device__ inline int XcdTile(int w, int nt) {
+  // Consecutive tiles (Morton-ordered blocks) go to the same XCD, so the
+  // apron re-reads of neighbouring blocks hit that XCD's L2.
+  const int q8 = nt >> 3;
+  return (w < (q8 << 3)) ? (w & 7) * q8 + (w >> 3) : w;
+}
+
+template <typename T>
+__device__ inline void BlockReduceStats(double s, int c, double* red_s, int* red_c, double* out) {
+  const int tid = threadIdx.x;
+  red_s[tid] = s;
+  red_c[tid] = c;
+  __syncthreads();
+  for (int w = blockDim.x / 2; w > 0; w >>= 1) {
+    if (tid < w) {
+      red_s[tid] += red_s[tid + w];
+      red_c[tid] += red_c[tid + w];
+    }
+    __syncthreads();
+  }
+  if (tid == 0) {
+    out[0] = red_s[0];
+    out[1] = static_cast<double>(red_c[0]);
+  }
+}
+
+// Pull-stream rule of one core cell (block_lattice.cpp:579-674), given the
+// LDS plane of population i at stream time.  `own` holds the cell's
+// post-collide populations.
+template <typename T, bool PERCELL, int I>
+__device__ inline T StreamPop(const T* pl, const uint8_t* kbox, int own_l, int k, const T* own,
+                              bool src_interior, uint32_t qm, const double* qv) {
+  constexpr int off = (Cx(I) * kBox + Cy(I)) * kBox + Cz(I);
+  if (!PERCELL) return pl[own_l - off];
+  if (k == kSolid) return own[I];
+  if (ReflectsOnPull(k) && !src_interior) return own[Opp(I)];
+  const int sk = kbox[own_l - off];
+  if (sk == kSolid || sk == kBouzidi) {
+    double q = 1.0;
+    if ((qm >> I) & 1u) q = qv[__popc(qm & ((1u << I) - 1u))];
+    if (q > 1.0e-10 && q < 1.0 - 1.0e-10) {
+      const int nl = own_l + off;
+      const double f_int = (kbox[nl] == kBulk) ? static_cast<double>(pl[nl]) : static_cast<double>(own[I]);
+      return static_cast<T>(BouzidiPull(static_cast<double>(own[Opp(I)]), f_int, static_cast<double>(own[I]), q));
+    }
+    return own[Opp(I)];
+  }
+  return pl[own_l - off];
+}
+
+template <typename T, bool PERCELL>
+struct PopLoop {
+  template <int I>
+  __device__ static inline void Run(T (*plane)[kBoxCells], const uint8_t* kbox, int own_l, int ap_l,
+                                    int k, const T* f, const T* g, T* out, T* sout, bool live,
+                                    int ix, int iy, int iz, int n, uint32_t qm, const double* qv) {
+    if constexpr (I < kQ) {
+      T* pl = plane[I & 1];
+      pl[own_l] = f[I];
+      if (ap_l >= 0) pl[ap_l] = g[I];
+      __syncthreads();
+      const int sx = ix - Cx(I), sy = iy - Cy(I), sz = iz - Cz(I);
+      const bool src_int = sx >= 0 && sx < n && sy >= 0 && sy < n && sz >= 0 && sz < n;
+      out[I] = StreamPop<T, PERCELL, I>(pl, kbox, own_l, k, f, src_int, qm, qv);
+      if (live) {
+        // padded-block torus pull of a halo slot; in-halo sources and wrapped
+        // sources are both (d - off) mod NP^3 (block_lattice.cpp:595-611)
+        constexpr int off = (Cx(I) * kBox + Cy(I)) * kBox + Cz(I);
+        int s = ap_l - off;
+        if (s < 0) s += kBoxCells;
+        if (s >= kBoxCells) s -= kBoxCells;
+        sout[I] = pl[s];
+      }
+      Run<I + 1>(plane, kbox, own_l, ap_l, k, f, g, out, sout, live, ix, iy, iz, n, qm, qv);
+    }
+  }
+};
+
+// ------------------------------------------------------- blocked fused kernel
+template <typename T, bool PERCELL>
+__global__ __launch_bounds__(512) void KBlock(TileArgs<T> a) {
+  __shared__ T plane[2][kBoxCells];
+  __shared__ uint8_t kbox[kBoxCells];
+  __shared__ double red_s[512];
+  __shared__ int red_c[512];
+
+  const int ti = XcdTile(blockIdx.x, a.ntiles);
+  const int b = a.tiles[ti].block;
+  const SimConst& sc = a.sc;
+  constexpr int n = kTile, ncore = kTileCells;
+  const int tid = threadIdx.x;
+  const int ix = tid >> 6, iy = (tid >> 3) & 7, iz = tid & 7;
+  const size_t bbase = static_cast<size_t>(b) * kQ * ncore;
+  const double t = static_cast<double>(*a.step);
+  const T avg = sc.const_rho ? static_cast<T>(*a.avg_rho) : T(1);
+  const BlockTab& bt = a.bt[b];
+  const int bflags = bt.flags;
+  const int bshell = bt.shell;
+  const double borg[3] = {bt.origin[0], bt.origin[1], bt.origin[2]};
+  const double bcw = bt.cw;
+
+  T f[kQ];
+#pragma unroll
+  for (int i = 0; i < kQ; ++i) f[i] = a.src[bbase + static_cast<size_t>(i) * ncore + tid];
+  const int k = a.kind[static_cast<size_t>(b) * ncore + tid];
+  double ssum = 0.0;
+  int scnt = 0;
+  CollideCell<T>(f, k, sc, borg, bcw, ix, iy, iz, t, avg, a.want_stats ? &ssum : nullptr, &scnt);
+  const int own_l = ((ix + 1) * kBox + iy + 1) * kBox + iz + 1;
+  kbox[own_l] = static_cast<uint8_t>(k);
+
+  // apron = the block's halo shell; apron index == shell index
+  T g[kQ];
+#pragma unroll
+  for (int i = 0; i < kQ; ++i) g[i] = T(0);
+  int ap_l = -1;
+  bool live = false;
+  if (tid < kApron) {
+    ap_l = kApronTab.idx[tid];
+    kbox[ap_l] = kBulk;  // kind_at_halo -> kBulk (block_lattice.cpp:510-518)
+    const int px = ap_l / (kBox * kBox), py = (ap_l / kBox) % kBox, pz = ap_l % kBox;
+    const int x = px - 1, y = py - 1, z = pz - 1;
+    const int ox = (x < 0 || x >= n), oy = (y < 0 || y >= n), oz = (z < 0 || z >= n);
+    const int nout = ox + oy + oz;
+    int nb = -1;
+    if (nout == 1) {
+      const int axis = ox ? 0 : (oy ? 1 : 2);
+      const int v = axis == 0 ? x : (axis == 1 ? y : z);
+      nb = bt.nbr[axis * 2 + (v >= n ? 1 : 0)];
+    } else if (nout == 2) {
+      const int a1 = ox ? 0 : 1;
+      const int a2 = oz ? 2 : 1;
+      const int v1 = a1 == 0 ? x : y;
+      const int v2 = a2 == 2 ? z : y;
+      nb = bt.nbr[EdgeSlot(a1, v1 >= n ? 1 : 0, a2, v2 >= n ? 1 : 0)];
+    }
+    if (nb >= 0) {
+      const int wx = x < 0 ? n - 1 : (x >= n ? 0 : x);
+      const int wy = y < 0 ? n - 1 : (y >= n ? 0 : y);
+      const int wz = z < 0 ? n - 1 : (z >= n ? 0 : z);
+      const int cc = (wx * n + wy) * n + wz;
+      const size_t nbase = static_cast<size_t>(nb) * kQ * ncore;
+#pragma unroll
+      for (int i = 0; i < kQ; ++i) g[i] = a.src[nbase + static_cast<size_t>(i) * ncore + cc];
+      const int kk = a.kind[static_cast<size_t>(nb) * ncore + cc];
+      const BlockTab& nt = a.bt[nb];
+      const double norg[3] = {nt.origin[0], nt.origin[1], nt.origin[2]};
+      CollideCell<T>(g, kk, sc, norg, nt.cw, wx, wy, wz, t, avg, nullptr, nullptr);
+    } else if (bshell >= 0) {
+      const size_t sbase = static_cast<size_t>(bshell) * kQ * kApron;
+#pragma unroll
+      for (int i = 0; i < kQ; ++i) g[i] = a.shell_src[sbase + static_cast<size_t>(i) * kApron + tid];
+    }
+    live = (bflags & kFlagShellLive) != 0;
+  }
+
+  uint32_t qm = 0;
+  const double* qv = a.qval;
+  if (PERCELL && (bflags & kFlagLinks)) {
+    qm = a.qmask[static_cast<size_t>(b) * ncore + tid];
+    qv = a.qval + a.qoff[static_cast<size_t>(b) * ncore + tid];
+  }
+
+  T out[kQ];
+  T sout[kQ];
+  PopLoop<T, PERCELL>::template Run<0>(plane, kbox, own_l, ap_l, k, f, g, out, sout, live, ix, iy,
+                                       iz, n, qm, qv);
+
+#pragma unroll
+  for (int i = 0; i < kQ; ++i) a.dst[bbase + static_cast<size_t>(i) * ncore + tid] = out[i];
+  if (live) {
+    const size_t sbase = static_cast<size_t>(bshell) * kQ * kApron;
+#pragma unroll
+    for (int i = 0; i < kQ; ++i) a.shell_dst[sbase + static_cast<size_t>(i) * kApron + tid] = sout[i];
+  }
+  if (bflags & kFlagDrag) {
+    // post-collide snapshot of the outgoing populations on wall links
+    // (take_post_collide_snapshot, block_lattice.cpp:309-316)
+    const uint32_t dm = a.dmask[static_cast<size_t>(b) * ncore + tid];
+    if (dm) {
+      T* sp = a.snap + a.doff[static_cast<size_t>(b) * ncore + tid];
+      int j = 0;
+#pragma unroll
+      for (int i = 0; i < kQ; ++i)
+        if ((dm >> i) & 1u) sp[j++] = f[i];
+    }
+  }
+  if (a.want_stats) BlockReduceStats<T>(ssum, scnt, red_s, red_c, a.stats + 2 * ti);
+}
+
+// -------------------------------------------------------- padded fused kernel
+// Tiles of 8^3 padded slots cover the whole padded block; every slot is
+// updated.  Core slots stream by the cell rule, padding slots by the padded
+// torus pull (block_lattice.cpp:595-611 / 551-577), whose wrapped sources
+// are padding slots (h >= 1) read straight from the previous state.
+template <typename T, bool PERCELL>
+__global__ __launch_bounds__(512) void KPadded(TileArgs<T> a) {
+  __shared__ T plane[2][kBoxCells];
+  __shared__ uint8_t kbox[kBoxCells];  // 0..7 core kinds, 254 padding, 255 outside
+  __shared__ double red_s[512];
+  __shared__ int red_c[512];
+
+  const int ti = XcdTile(blockIdx.x, a.ntiles);
+  const Tile tl = a.tiles[ti];
+  const int b = tl.block;
+  const SimConst& sc = a.sc;
+  const int n = sc.n, h = sc.h, NP = sc.NP, NS = sc.NS;
+  const int tid = threadIdx.x;
+  const size_t bbase = static_cast<size_t>(b) * kQ * NS;
+  const double t = static_cast<double>(*a.step);
+  const T avg = sc.const_rho ? static_cast<T>(*a.avg_rho) : T(1);
+  const BlockTab& bt = a.bt[b];
+  const double borg[3] = {bt.origin[0], bt.origin[1], bt.origin[2]};
+  const double bcw = bt.cw;
+  double ssum = 0.0;
+  int scnt = 0;
+
+  const int lx = tid >> 6, ly = (tid >> 3) & 7, lz = tid & 7;
+  const int hx = tl.ox + lx, hy = tl.oy + ly, hz = tl.oz + lz;
+  const bool active = hx < NP && hy < NP && hz < NP;
+  const int own_l = ((lx + 1) * kBox + ly + 1) * kBox + lz + 1;
+  const int d = (hx * NP + hy) * NP + hz;
+  const int x = hx - h, y = hy - h, z = hz - h;
+  const bool core = active && x >= 0 && x < n && y >= 0 && y < n && z >= 0 && z < n;
+
+  T f[kQ];
+#pragma unroll
+  for (int i = 0; i < kQ; ++i) f[i] = T(0);
+  int k = kBulk;
+  if (active) {
+#pragma unroll
+    for (int i = 0; i < kQ; ++i) f[i] = a.src[bbase + static_cast<size_t>(i) * NS + d];
+    if (core) {
+      k = a.kind[static_cast<size_t>(b) * sc.ncore + (x * n + y) * n + z];
+      CollideCell<T>(f, k, sc, borg, bcw, x, y, z, t, avg, a.want_stats ? &ssum : nullptr, &scnt);
+      kbox[own_l] = static_cast<uint8_t>(k);
+    } else {
+      kbox[own_l] = 254;
+    }
+  } else {
+    kbox[own_l] = 255;
+  }
+
+  T g[kQ];
+#pragma unroll
+  for (int i = 0; i < kQ; ++i) g[i] = T(0);
+  int ap_l = -1;
+  if (tid < kApron) {
+    ap_l = kApronTab.idx[tid];
+    const int px = ap_l / (kBox * kBox), py = (ap_l / kBox) % kBox, pz = ap_l % kBox;
+    const int gx = tl.ox - 1 + px, gy = tl.oy - 1 + py, gz = tl.oz - 1 + pz;
+    if (gx >= 0 && gx < NP && gy >= 0 && gy < NP && gz >= 0 && gz < NP) {
+      const int dg = (gx * NP + gy) * NP + gz;
+#pragma unroll
+      for (int i = 0; i < kQ; ++i) g[i] = a.src[bbase + static_cast<size_t>(i) * NS + dg];
+      const int cx = gx - h, cy = gy - h, cz = gz - h;
+      if (cx >= 0 && cx < n && cy >= 0 && cy < n && cz >= 0 && cz < n) {
+        const int kk = a.kind[static_cast<size_t>(b) * sc.ncore + (cx * n + cy) * n + cz];
+        CollideCell<T>(g, kk, sc, borg, bcw, cx, cy, cz, t, avg, nullptr, nullptr);
+        kbox[ap_l] = static_cast<uint8_t>(kk);
+      } else {
+        kbox[ap_l] = 254;
+      }
+    } else {
+      kbox[ap_l] = 255;
+    }
+  }
+
+  T out[kQ];
+#pragma unroll
+  for (int i = 0; i < kQ; ++i) {
+    T* pl = plane[i & 1];
+    pl[own_l] = f[i];
+    if (ap_l >= 0) pl[ap_l] = g[i];
+    __syncthreads();
+    const int cxi = Cx(i), cyi = Cy(i), czi = Cz(i);
+    const int box_off = (cxi * kBox + cyi) * kBox + czi;
+    const int sx = hx - cxi, sy = hy - cyi, sz = hz - czi;
+    T v = T(0);
+    if (active) {
+      if (core && PERCELL) {
+        const bool src_int = sx - h >= 0 && sx - h < n && sy - h >= 0 && sy - h < n && sz - h >= 0 &&
+                             sz - h < n;
+        if (k == kSolid) {
+          v = f[i];
+        } else if (ReflectsOnPull(k) && !src_int) {
+          v = f[Opp(i)];
+        } else {
+          const int sk = kbox[own_l - box_off];
+          if (src_int && (sk == kSolid || sk == kBouzidi)) {
+            v = f[Opp(i)];  // padded blocks carry no Bouzidi links: q = 1 -> half-way bounce-back
+          } else {
+            v = pl[own_l - box_off];
+          }
+        }
+      } else if (sx >= 0 && sx < NP && sy >= 0 && sy < NP && sz >= 0 && sz < NP) {
+        v = pl[own_l - box_off];
+      } else {
+        const int off = (cxi * NP + cyi) * NP + czi;
+        int s = (d - off) % NS;
+        if (s < 0) s += NS;
+        v = a.src[bbase + static_cast<size_t>(i) * NS + s];
+      }
+    }
+    out[i] = v;
+  }
+  if (active) {
+#pragma unroll
+    for (int i = 0; i < kQ; ++i) a.dst[bbase + static_cast<size_t>(i) * NS + d] = out[i];
+    if (core && (bt.flags & kFlagDrag)) {
+      const int c = (x * n + y) * n + z;
+      const uint32_t dm = a.dmask[static_cast<size_t>(b) * sc.ncore + c];
+      if (dm) {
+        T* sp = a.snap + a.doff[static_cast<size_t>(b) * sc.ncore + c];
+        int j = 0;
+#pragma unroll
+        for (int i = 0; i < kQ; ++i)
+          if ((dm >> i) & 1u) sp[j++] = f[i];
+      }
+    }
+  }
+  if (a.want_stats) BlockReduceStats<T>(ssum, scnt, red_s, red_c, a.stats + 2 * ti);
+}
+
+// ------------------------------------------------------------ small kernels
+__global__ void KStatsFinalize(const double* partial, int ntiles, double* avg) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  double s = 0.0, c = 0.0;
+  for (int i = 0; i < ntiles; ++i) {
+    s += partial[2 * i];
+    c += partial[2 * i + 1];
+  }
+  *avg = c > 0.0 ? s / c : 1.0;  // CollideRhoStats::average (block_lattice.h:31)
+}
+
+__global__ void KStepInc(int* step) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) *step += 1;
+}
+
+template <typename T>
+__global__ void KInitEq(T* pops, const uint8_t* kind, int nb, SimConst sc, double rho0, double u0x,
+                        double u0y, double u0z, uint32_t mask) {
+  const size_t gid = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  const size_t total = static_cast<size_t>(nb) * sc.ncore;
+  if (gid >= total) return;
+  const int b = static_cast<int>(gid / sc.ncore), c = static_cast<int>(gid % sc.ncore);
+  if (!((mask >> kind[gid]) & 1u)) return;
+  const double u[3] = {u0x, u0y, u0z};
+  const double usqr = NormSqr(u);
+  const int slot = CoreSlot(sc, c);
+  T* base = pops + static_cast<size_t>(b) * kQ * sc.plane;
+  for (int i = 0; i < kQ; ++i)
+    base[static_cast<size_t>(i) * sc.plane + slot] = static_cast<T>(Feq<double>(i, rho0, u, usqr));
+}
+
+template <typename T>
+__global__ void KInitField(T* pops, int nb, SimConst sc, const double* rho, const double* u) {
+  const size_t gid = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  const size_t total = static_cast<size_t>(nb) * sc.ncore;
+  if (gid >= total) return;
+  const int b = static_cast<int>(gid / sc.ncore), c = static_cast<int>(gid % sc.ncore);
+  const double uc[3] = {u[3 * gid], u[3 * gid + 1], u[3 * gid + 2]};
+  const double usqr = NormSqr(uc);
+  const int slot = CoreSlot(sc, c);
+  T* base = pops + static_cast<size_t>(b) * kQ * sc.plane;
+  for (int i = 0; i < kQ; ++i)
+    base[static_cast<size_t>(i) * sc.plane + slot] = static_cast<T>(Feq<double>(i, rho[gid], uc, usqr));
+}
+
+// ------------------------------------------------------- post-stream FD
+// ApplyInterpolatedVelocityBoundaryCells (interpolated_velocity.h:950-1036)
+// processes FD cells in (ix,iy,iz) order in place.  Plane cells read only
+// pre-pass populations, edge cells read plane cells' densities and corners
+// read edge cells' densities (n >= 4), so three phases over all FD cells
+// reproduce the sequential order: a read of an FD cell that precedes the
+// reader in (ix,iy,iz) order takes its new value from `post`.
+template <typename T>
+struct FdView {
+  const T* cur;
+  const T* post;
+  const int* smap;  // this block's FD cell -> entry (or -1)
+  const uint8_t* kind;
+  const SimConst* sc;
+  const double* org;
+  double cw;
+  double t;
+  int self;
+
+  __device__ bool InDomain(int x, int y, int z) const {
+    const int n = sc->n;
+    return x >= 0 && x < n && y >= 0 && y < n && z >= 0 && z < n;
+  }
+  // IsPaddingOverlapCell (interpolated_velocity.h:238-240); h >= 1 and every
+  // stencil read lies within one cell of the block.
+  __device__ bool Padding(int x, int y, int z) const { return !InDomain(x, y, z); }
+  __device__ int Kind(int x, int y, int z) const { return kind[(x * sc->n + y) * sc->n + z]; }
+  __device__ void Pops(int x, int y, int z, T* f) const {
+    const int c = (x * sc->n + y) * sc->n + z;
+    const int e = smap[c];
+    if (e >= 0 && c < self) {
+#pragma unroll
+      for (int i = 0; i < kQ; ++i) f[i] = post[static_cast<size_t>(e) * kQ + i];
+    } else {
+      const int s = CoreSlot(*sc, c);
+#pragma unroll
+      for (int i = 0; i < kQ; ++i) f[i] = cur[static_cast<size_t>(i) * sc->plane + s];
+    }
+  }
+  __device__ void Prescribed(int x, int y, int z, T* u) const {
+    double ud[3];
+    PrescribedU(*sc, org, cw, x, y, z, t, ud);
+    u[0] = static_cast<T>(ud[0]);
+    u[1] = static_cast<T>(ud[1]);
+    u[2] = static_cast<T>(ud[2]);
+  }
+  // BoundaryLatticeView::ComputeU with ExtrapolatedU unrolled into a loop
+  __device__ void ComputeU(int x, int y, int z, T* u) const {
+    for (;;) {
+      if (Padding(x, y, z)) {
+        u[0] = u[1] = u[2] = T(0);
+        return;
+      }
+      const int k = Kind(x, y, z);
+      if (k == kVelocity) {
+        Prescribed(x, y, z, u);
+        return;
+      }
+      if (k == kPressure) {
+        int dir, ori;
+        if (!FlatInfo(sc->n, x, y, z, dir, ori)) {
+          u[0] = u[1] = u[2] = T(0);
+          return;
+        }
+        if (dir == 0) {
+          x -= ori;
+        } else if (dir == 1) {
+          y -= ori;
+        } else {
+          z -= ori;
+        }
+        continue;
+      }
+      T f[kQ], rho;
+      Pops(x, y, z, f);
+      ProxyRhoU(f, rho, u);
+      return;
+    }
+  }
+  __device__ T PrescribedRho(int x, int y, int z) const {
+    int dir, ori;
+    if (FlatInfo(sc->n, x, y, z, dir, ori)) return static_cast<T>(sc->spec[dir * 2 + (ori < 0 ? 0 : 1)].rho_target);
+    return T(1);
+  }
+  __device__ T ComputeRho(int x, int y, int z) const {
+    if (Padding(x, y, z)) return T(1);
+    const int k = Kind(x, y, z);
+    if (k == kPressure) return PrescribedRho(x, y, z);
+    T f[kQ];
+    Pops(x, y, z, f);
+    if (k == kVelocity) {
+      int dir, ori;
+      if (FlatInfo(sc->n, x, y, z, dir, ori)) {
+        T u[3];
+        Prescribed(x, y, z, u);
+        return VelRhoFromPop(dir, ori, f, u);
+      }
+    }
+    T rho, u[3];
+    ProxyRhoU(f, rho, u);
+    return rho;
+  }
+  __device__ void ComputeRhoU(int x, int y, int z, T& rho, T* u) const {
+    const int k = Kind(x, y, z);
+    if (k == kVelocity) {
+      Prescribed(x, y, z, u);
+      rho = ComputeRho(x, y, z);
+      return;
+    }
+    if (k == kPressure) {
+      rho = PrescribedRho(x, y, z);
+      int dir, ori;
+      if (!FlatInfo(sc->n, x, y, z, dir, ori)) {
+        u[0] = u[1] = u[2] = T(0);
+        return;
+      }
+      ComputeU(x - (dir == 0 ? ori : 0), y - (dir == 1 ? ori : 0), z - (dir == 2 ? ori : 0), u);
+      return;
+    }
+    T f[kQ];
+    Pops(x, y, z, f);
+    ProxyRhoU(f, rho, u);
+  }
+  __device__ void NormalGrad(int x, int y, int z, int axis, int normal, T* g) const {
+    const int ox = axis == 0 ? -normal : 0, oy = axis == 1 ? -normal : 0, oz = axis == 2 ? -normal : 0;
+    T u0[3], u1[3], u2[3];
+    ComputeU(x, y, z, u0);
+    ComputeU(x + ox, y + oy, z + oz, u1);
+    ComputeU(x + 2 * ox, y + 2 * oy, z + 2 * oz, u2);
+    for (int d = 0; d < 3; ++d)
+      g[d] = T(-normal) * ((-T(3) * u0[d] + T(4) * u1[d] - T(1) * u2[d]) / T(2));
+  }
+  __device__ void TangGrad(int x, int y, int z, int tx, int ty, int tz, T* g) const {
+    T up[3], um[3];
+    ComputeU(x + tx, y + ty, z + tz, up);
+    ComputeU(x - tx, y - ty, z - tz, um);
+    for (int d = 0; d < 3; ++d) g[d] = (up[d] - um[d]) / T(2);
+  }
+};
+
+template <typename T>
+__device__ inline void WriteFromPi(T* f, T rho, const T* u, const T* pi) {
+  const T usqr = u[0] * u[0] + u[1] * u[1] + u[2] * u[2];
+  for (int i = 0; i < kQ; ++i) f[i] = Feq(i, rho, u, usqr) + PiToFneq(i, pi);
+}
+
+template <typename T>
+__device__ inline void PiFromGrad(T rho, T omega, const T* dx, const T* dy, const T* dz, T* pi) {
+  const T s = -rho / T(3) / omega;
+  pi[0] = T(2) * dx[0] * s;
+  pi[3] = T(2) * dy[1] * s;
+  pi[5] = T(2) * dz[2] * s;
+  pi[1] = (dx[1] + dy[0]) * s;
+  pi[2] = (dx[2] + dz[0]) * s;
+  pi[4] = (dy[2] + dz[1]) * s;
+}
+
+template <typename T>
+__global__ void KFd(FdArgs<T> a, int begin, int count) {
+  const int e = begin + static_cast<int>(blockIdx.x * blockDim.x + threadIdx.x);
+  if (e >= begin + count) return;
+  const int b = a.cells[e].x, c = a.cells[e].y;
+  const SimConst& sc = a.sc;
+  const int n = sc.n;
+  const int ix = c / (n * n), iy = (c / n) % n, iz = c % n;
+  const int lvl = a.level[b];
+  const T* cur = (a.par.v[lvl] ? a.pops1 : a.pops0) + static_cast<size_t>(b) * kQ * sc.plane;
+  FdView<T> v;
+  v.cur = cur;
+  v.post = a.post;
+  v.smap = a.smap + static_cast<size_t>(a.fd_index[b]) * sc.ncore;
+  v.kind = a.kind + static_cast<size_t>(b) * sc.ncore;
+  v.sc = &a.sc;
+  v.org = a.bt[b].origin;
+  v.cw = a.bt[b].cw;
+  v.t = static_cast<double>(*a.step);
+  v.self = c;
+  const T omega = static_cast<T>(sc.omega);
+  const int xm = ix == 0, xp = ix == n - 1, ym = iy == 0, yp = iy == n - 1, zm = iz == 0, zp = iz == n - 1;
+  const int fc = xm + xp + ym + yp + zm + zp;
+  const int xn = xm ? -1 : (xp ? 1 : 0), yn = ym ? -1 : (yp ? 1 : 0), zn = zm ? -1 : (zp ? 1 : 0);
+  T fnew[kQ];
+  T pi[6];
+  if (fc == 3) {
+    // ApplyCornerBoundary (:638-685)
+    const T r100 = v.ComputeRho(ix - xn, iy, iz);
+    const T r010 = v.ComputeRho(ix, iy - yn, iz);
+    const T r001 = v.ComputeRho(ix, iy, iz - zn);
+    const T r200 = v.ComputeRho(ix - 2 * xn, iy, iz);
+    const T r020 = v.ComputeRho(ix, iy - 2 * yn, iz);
+    const T r002 = v.ComputeRho(ix, iy, iz - 2 * zn);
+    const T rho = T(4) / T(9) * (r001 + r010 + r100) - T(1) / T(9) * (r002 + r020 + r200);
+    T dx[3], dy[3], dz[3], u[3];
+    v.NormalGrad(ix, iy, iz, 0, xn, dx);
+    v.NormalGrad(ix, iy, iz, 1, yn, dy);
+    v.NormalGrad(ix, iy, iz, 2, zn, dz);
+    v.ComputeU(ix, iy, iz, u);
+    PiFromGrad(rho, omega, dx, dy, dz, pi);
+    WriteFromPi(fnew, rho, u, pi);
+  } else if (fc == 2) {
+    // ApplyEdgeBoundary (:687-752)
+    int plane, n1, n2;
+    if (xn != 0 && yn != 0) {
+      plane = 2;
+      n1 = xn;
+      n2 = yn;
+    } else if (xn != 0 && zn != 0) {
+      plane = 1;
+      n1 = xn;
+      n2 = zn;
+    } else {
+      plane = 0;
+      n1 = yn;
+      n2 = zn;
+    }
+    const int d1 = (plane + 1) % 3, d2 = (plane + 2) % 3;
+    T rs[4];
+    const int st[4][2] = {{1, 0}, {0, 1}, {2, 0}, {0, 2}};
+    for (int q = 0; q < 4; ++q) {
+      int o[3] = {0, 0, 0};
+      o[d1] = -n1 * st[q][0];
+      o[d2] = -n2 * st[q][1];
+      rs[q] = v.ComputeRho(ix + o[0], iy + o[1], iz + o[2]);
+    }
+    const T rho = T(2) / T(3) * (rs[1] + rs[0]) - T(1) / T(6) * (rs[3] + rs[2]);
+    T g[3][3];
+    for (int p = 0; p < 3; ++p)
+      for (int q = 0; q < 3; ++q) g[p][q] = T(0);
+    v.TangGrad(ix, iy, iz, plane == 0, plane == 1, plane == 2, g[plane]);
+    v.NormalGrad(ix, iy, iz, d1, n1, g[d1]);
+    v.NormalGrad(ix, iy, iz, d2, n2, g[d2]);
+    T u[3];
+    v.ComputeU(ix, iy, iz, u);
+    const T s = -rho / T(3) / omega;
+    pi[0] = T(2) * g[0][0] * s;
+    pi[3] = T(2) * g[1][1] * s;
+    pi[5] = T(2) * g[2][2] * s;
+    pi[1] = (g[0][1] + g[1][0]) * s;
+    pi[2] = (g[0][2] + g[2][0]) * s;
+    pi[4] = (g[1][2] + g[2][1]) * s;
+    WriteFromPi(fnew, rho, u, pi);
+  } else {
+    // ApplyPlaneFdBoundary (:570-636)
+    int face = OCTLB_XMIN;
+    if (xp) {
+      face = OCTLB_XMAX;
+    } else if (yp) {
+      face = OCTLB_YMAX;
+    } else if (ym) {
+      face = OCTLB_YMIN;
+    } else if (zp) {
+      face = OCTLB_ZMAX;
+    } else if (zm) {
+      face = OCTLB_ZMIN;
+    }
+    const int direction = face / 2, orientation = (face & 1) ? 1 : -1;
+    T rho, u[3];
+    v.ComputeRhoU(ix, iy, iz, rho, u);
+    T dx[3] = {T(0), T(0), T(0)}, dy[3] = {T(0), T(0), T(0)}, dz[3] = {T(0), T(0), T(0)};
+    if (direction == 0) {
+      v.NormalGrad(ix, iy, iz, 0, orientation, dx);
+      v.TangGrad(ix, iy, iz, 0, 1, 0, dy);
+      v.TangGrad(ix, iy, iz, 0, 0, 1, dz);
+    } else if (direction == 1) {
+      v.TangGrad(ix, iy, iz, 1, 0, 0, dx);
+      v.NormalGrad(ix, iy, iz, 1, orientation, dy);
+      v.TangGrad(ix, iy, iz, 0, 0, 1, dz);
+    } else {
+      v.TangGrad(ix, iy, iz, 1, 0, 0, dx);
+      v.TangGrad(ix, iy, iz, 0, 1, 0, dy);
+      v.NormalGrad(ix, iy, iz, 2, orientation, dz);
+    }
+    PiFromGrad(rho, omega, dx, dy, dz, pi);
+    WriteFromPi(fnew, rho, u, pi);
+  }
+  T* po = a.post + static_cast<size_t>(e) * kQ;
+  for (int i = 0; i < kQ; ++i) po[i] = fnew[i];
+}
+
+template <typename T>
+__global__ void KFdCommit(FdArgs<T> a, int count) {
+  const int e = static_cast<int>(blockIdx.x * blockDim.x + threadIdx.x);
+  if (e >= count) return;
+  const int b = a.cells[e].x, c = a.cells[e].y;
+  const SimConst& sc = a.sc;
+  T* cur = (a.par.v[a.level[b]] ? a.pops1 : a.pops0) + static_cast<size_t>(b) * kQ * sc.plane;
+  const int s = CoreSlot(sc, c);
+  const T* po = a.post + static_cast<size_t>(e) * kQ;
+  for (int i = 0; i < kQ; ++i) cur[static_cast<size_t>(i) * sc.plane + s] = po[i];
+}
+
+// ------------------------------------------------------ Lagrava coupling
+// ReadMacro (level_coupler.cpp:183-189): olb computeRhoU + computeFneq.
+template <typename T>
+__device__ inline void ReadMacro(const T* base, int plane, int c, T* m) {
+  T f[kQ];
+#pragma unroll
+  for (int i = 0; i < kQ; ++i) f[i] = base[static_cast<size_t>(i) * plane + c];
+  T rho, u[3];
+  OlbRhoU(f, rho, u);
+  const T usqr = NormSqr(u);
+  m[0] = rho;
+  m[1] = u[0];
+  m[2] = u[1];
+  m[3] = u[2];
+#pragma unroll
+  for (int i = 0; i < kQ; ++i) m[4 + i] = f[i] - Feq(i, rho, u, usqr);
+}
+
+// WriteProlongation (level_coupler.cpp:191-201)
+template <typename T>
+__device__ inline void WriteMacro(T* base, int plane, int c, const T* m, T scaling) {
+  const T u[3] = {m[1], m[2], m[3]};
+  const T usqr = NormSqr(u);
+#pragma unroll
+  for (int i = 0; i < kQ; ++i) base[static_cast<size_t>(i) * plane + c] = Feq(i, m[0], u, usqr) + scaling * m[4 + i];
+}
+
+template <typename T>
+__global__ void KCouplerInit(CoupleArgs<T> a, int npts) {
+  const int p = static_cast<int>(blockIdx.x * blockDim.x + threadIdx.x);
+  if (p >= npts) return;
+  const int* r = a.plan + 12 * p;
+  const int cl = r[9];
+  const T* cb = (a.par.v[cl] ? a.pops1 : a.pops0) + static_cast<size_t>(r[0]) * kQ * kTileCells;
+  ReadMacro(cb, kTileCells, (r[1] * kTile + r[2]) * kTile + r[3], a.prev + static_cast<size_t>(p) * kMacro);
+}
+
+// ApplyProlongation (level_coupler.cpp:438-478) over the deduplicated points
+// of one coarse level (a later point writing the same fine cell wins).
+template <typename T>
+__global__ void KProlong(CoupleArgs<T> a, const int* list, int count, int half) {
+  const int j = static_cast<int>(blockIdx.x * blockDim.x + threadIdx.x);
+  if (j >= count) return;
+  const int p = list[j];
+  const int* r = a.plan + 12 * p;
+  const int cl = r[9], fl = r[10];
+  const T* cb = (a.par.v[cl] ? a.pops1 : a.pops0) + static_cast<size_t>(r[0]) * kQ * kTileCells;
+  T* fb = (a.par.v[fl] ? a.pops1 : a.pops0) + static_cast<size_t>(r[4]) * kQ * kTileCells;
+  T m[kMacro];
+  const int cc = (r[1] * kTile + r[2]) * kTile + r[3];
+  ReadMacro(cb, kTileCells, cc, m);
+  T* pv = a.prev + static_cast<size_t>(p) * kMacro;
+  if (half) {
+    for (int q = 0; q < kMacro; ++q) m[q] = T(0.5) * (pv[q] + m[q]);
+  }
+  WriteMacro(fb, kTileCells, (r[5] * kTile + r[6]) * kTile + r[7], m, a.prolong_scale);
+  if (!half) {
+    for (int q = 0; q < kMacro; ++q) pv[q] = m[q];
+  }
+}
+
+// LevelCoupler::restrict (level_coupler.cpp:613-654) over the deduplicated
+// points of one fine level (a later point writing the same coarse cell wins).
+template <typename T>
+__global__ void KRestrict(CoupleArgs<T> a, const int* list, int count) {
+  const int j = static_cast<int>(blockIdx.x * blockDim.x + threadIdx.x);
+  if (j >= count) return;
+  const int p = list[j];
+  const int* r = a.plan + 12 * p;
+  const int cl = r[9], fl = r[10];
+  T* cb = (a.par.v[cl] ? a.pops1 : a.pops0) + static_cast<size_t>(r[0]) * kQ * kTileCells;
+  const T* fb = (a.par.v[fl] ? a.pops1 : a.pops0) + static_cast<size_t>(r[4]) * kQ * kTileCells;
+  const int fi = r[5], fj = r[6], fk = r[7];
+  T fm[kMacro];
+  ReadMacro(fb, kTileCells, (fi * kTile + fj) * kTile + fk, fm);
+  T sum[kQ];
+  for (int q = 0; q < kQ; ++q) sum[q] = T(0);
+  int cnt = 0;
+  for (int jp = 1; jp < kQ; ++jp) {
+    const int x = fi + Cx(jp), y = fj + Cy(jp), z = fk + Cz(jp);
+    if (x < 0 || x >= kTile || y < 0 || y >= kTile || z < 0 || z >= kTile) continue;
+    T nm[kMacro];
+    ReadMacro(fb, kTileCells, (x * kTile + y) * kTile + z, nm);
+    for (int q = 0; q < kQ; ++q) sum[q] += nm[4 + q];
+    ++cnt;
+  }
+  for (int q = 0; q < kQ; ++q) fm[4 + q] = (fm[4 + q] + sum[q]) / static_cast<T>(cnt + 1);
+  WriteMacro(cb, kTileCells, (r[1] * kTile + r[2]) * kTile + r[3], fm, a.restrict_scale);
+}
+
+// ------------------------------------------------------ drag and mass
+// MomentumExchangeDrag::force_on_fluid_if (drag_postprocessor.h:66-115):
+// one term per wall link, summed per block in the reference's loop order
+// (iz, iy, ix, iPop) and over blocks in block order, all in FP64.
+template <typename T>
+__global__ void KDragTerms(DragArgs<T> a, int nlinks) {
+  const int j = static_cast<int>(blockIdx.x * blockDim.x + threadIdx.x);
+  if (j >= nlinks) return;
+  const int4 L = a.links[j];  // block, cell, pop, snapshot index
+  const int b = L.x, c = L.y, i = L.z;
+  const SimConst& sc = a.sc;
+  const int n = sc.n;
+  const int ix = c / (n * n), iy = (c / n) % n, iz = c % n;
+  const double* org = a.bt[b].origin;
+  const double cw = a.bt[b].cw;
+  const double px = org[0] + (ix + Cx(i) + 0.5) * cw;
+  const double py = org[1] + (iy + Cy(i) + 0.5) * cw;
+  const double pz = org[2] + (iz + Cz(i) + 0.5) * cw;
+  const bool in = px >= a.bbox[0] && px <= a.bbox[3] && py >= a.bbox[1] && py <= a.bbox[4] &&
+                  pz >= a.bbox[2] && pz <= a.bbox[5];
+  a.use[j] = in ? 1 : 0;
+  const T* live = (a.par.v[a.bt[b].level] ? a.pops1 : a.pops0) + static_cast<size_t>(b) * kQ * sc.plane;
+  const int s = CoreSlot(sc, c);
+  const double fbar = static_cast<double>(live[static_cast<size_t>(Opp(i)) * sc.plane + s]);
+  const double fout = a.snap ? static_cast<double>(a.snap[L.w]) : static_cast<double>(live[static_cast<size_t>(i) * sc.plane + s]);
+  a.term[j] = fout + fbar;
+}
+
+__global__ void KDragBlocks(const int4* links, const int* block_begin, const int* blocks, int nblk,
+                            const double* term, const uint8_t* use, double* out) {
+  const int j = static_cast<int>(blockIdx.x * blockDim.x + threadIdx.x);
+  if (j >= nblk) return;
+  double fo[3] = {0.0, 0.0, 0.0};
+  for (int e = block_begin[j]; e < block_begin[j + 1]; ++e) {
+    if (!use[e]) continue;
+    const int i = links[e].z;
+    fo[0] += term[e] * static_cast<double>(Cx(i));
+    fo[1] += term[e] * static_cast<double>(Cy(i));
+    fo[2] += term[e] * static_cast<double>(Cz(i));
+  }
+  out[3 * j] = -fo[0];
+  out[3 * j + 1] = -fo[1];
+  out[3 * j + 2] = -fo[2];
+  (void)blocks;
+}
+
+__global__ void KSum3(const double* v, int count, double* out) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  double s[3] = {0.0, 0.0, 0.0};
+  for (int j = 0; j < count; ++j)
+    for (int d = 0; d < 3; ++d) s[d] += v[3 * j + d];
+  out[0] = s[0];
+  out[1] = s[1];
+  out[2] = s[2];
+}
+
+// Cylinder3dCase::total_mass (cylinder3d_case.h:601-622): rho over kBulk
+// cells, FP64 partial sums per block.
+template <typename T>
+__global__ __launch_bounds__(256) void KMass(const T* pops0, const T* pops1, ParArray par,
+                                             const BlockTab* bt, const uint8_t* kind, SimConst sc,
+                                             double* partial) {
+  __shared__ double red[256];
+  const int b = blockIdx.x;
+  const T* base = (par.v[bt[b].level] ? pops1 : pops0) + static_cast<size_t>(b) * kQ * sc.plane;
+  double s = 0.0;
+  for (int c = threadIdx.x; c < sc.ncore; c += blockDim.x) {
+    if (kind[static_cast<size_t>(b) * sc.ncore + c] != kBulk) continue;
+    T f[kQ];
+    const int sl = CoreSlot(sc, c);
+    for (int i = 0; i < kQ; ++i) f[i] = base[static_cast<size_t>(i) * sc.plane + sl];
+    T rho, u[3];
+    ProxyRhoU(f, rho, u);
+    s += static_cast<double>(rho);
+  }
+  red[threadIdx.x] = s;
+  __syncthreads();
+  for (int w = blockDim.x / 2; w > 0; w >>= 1) {
+    if (threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) partial[b] = red[0];
+}
+
+__global__ void KSum1(const double* v, int count, double* out) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  double s = 0.0;
+  for (int j = 0; j < count; ++j) s += v[j];
+  *out = s;
+}
+
+// ---------------------------------------------------------- instantiations
+template <typename T>
+void LaunchTile(const TileArgs<T>& a, hipStream_t s) {
+  if (a.ntiles <= 0) return;
+  if (a.sc.padded) {
+    if (a.sc.bouzidi) {
+      hipLaunchKernelGGL((KPadded<T, true>), dim3(a.ntiles), dim3(512), 0, s, a);
+    } else {
+      hipLaunchKernelGGL((KPadded<T, false>), dim3(a.ntiles), dim3(512), 0, s, a);
+    }
+  } else if (a.sc.bouzidi) {
+    hipLaunchKernelGGL((KBlock<T, true>), dim3(a.ntiles), dim3(512), 0, s, a);
+  } else {
+    hipLaunchKernelGGL((KBlock<T, false>), dim3(a.ntiles), dim3(512), 0, s, a);
+  }
+}
+
+static inline dim3 Grid(long long n, int bs) { return dim3(static_cast<unsigned>((n + bs - 1) / bs)); }
+
+template <typename T>
+void LaunchStatsFinalize(const double* partial, int ntiles, double* avg, hipStream_t s) {
+  hipLaunchKernelGGL(KStatsFinalize, dim3(1), dim3(1), 0, s, partial, ntiles, avg);
+}
+
+void LaunchStepInc(int* step, hipStream_t s) { hipLaunchKernelGGL(KStepInc, dim3(1), dim3(1), 0, s, step); }
+
+template <typename T>
+void LaunchInitEq(T* pops, const uint8_t* kind, int nb, const SimConst& sc, double rho0,
+                  const double* u0, uint32_t mask, hipStream_t s) {
+  const long long total = static_cast<long long>(nb) * sc.ncore;
+  hipLaunchKernelGGL(KInitEq<T>, Grid(total, 256), dim3(256), 0, s, pops, kind, nb, sc, rho0, u0[0],
+                     u0[1], u0[2], mask);
+}
+
+template <typename T>
+void LaunchInitField(T* pops, int nb, const SimConst& sc, const double* rho, const double* u,
+                     hipStream_t s) {
+  const long long total = static_cast<long long>(nb) * sc.ncore;
+  hipLaunchKernelGGL(KInitField<T>, Grid(total, 256), dim3(256), 0, s, pops, nb, sc, rho, u);
+}
+
+template <typename T>
+void LaunchFd(const FdArgs<T>& a, const int* phase_begin, hipStream_t s) {
+  for (int ph = 0; ph < 3; ++ph) {
+    const int cnt = phase_begin[ph + 1] - phase_begin[ph];
+    if (cnt > 0) hipLaunchKernelGGL(KFd<T>, Grid(cnt, 128), dim3(128), 0, s, a, phase_begin[ph], cnt);
+  }
+  const int total = phase_begin[3];
+  if (total > 0) hipLaunchKernelGGL(KFdCommit<T>, Grid(total, 256), dim3(256), 0, s, a, total);
+}
+
+template <typename T>
+void LaunchCouplerInit(const CoupleArgs<T>& a, int npts, hipStream_t s) {
+  if (npts > 0) hipLaunchKernelGGL(KCouplerInit<T>, Grid(npts, 128), dim3(128), 0, s, a, npts);
+}
+
+template <typename T>
+void LaunchProlong(const CoupleArgs<T>& a, const int* list, int count, int half, hipStream_t s) {
+  if (count > 0) hipLaunchKernelGGL(KProlong<T>, Grid(count, 128), dim3(128), 0, s, a, list, count, half);
+}
+
+template <typename T>
+void LaunchRestrict(const CoupleArgs<T>& a, const int* list, int count, hipStream_t s) {
+  if (count > 0) hipLaunchKernelGGL(KRestrict<T>, Grid(count, 64), dim3(64), 0, s, a, list, count);
+}
+
+template <typename T>
+void LaunchDrag(const DragArgs<T>& a, int nlinks, const int* block_begin, int nblk, double* block_out,
+                double* out, hipStream_t s) {
+  if (nlinks > 0) hipLaunchKernelGGL(KDragTerms<T>, Grid(nlinks, 256), dim3(256), 0, s, a, nlinks);
+  if (nblk > 0)
+    hipLaunchKernelGGL(KDragBlocks, Grid(nblk, 64), dim3(64), 0, s, a.links, block_begin, nullptr, nblk,
+                       a.term, a.use, block_out);
+  hipLaunchKernelGGL(KSum3, dim3(1), dim3(1), 0, s, block_out, nblk, out);
+}
+
+template <typename T>
+void LaunchMass(const T* p0, const T* p1, const ParArray& par, const BlockTab* bt, const uint8_t* kind,
+                const SimConst& sc, int nb, double* partial, double* out, hipStream_t s) {
+  hipLaunchKernelGGL(KMass<T>, dim3(nb), dim3(256), 0, s, p0, p1, par, bt, kind, sc, partial);
+  hipLaunchKernelGGL(KSum1, dim3(1), dim3(1), 0, s, partial, nb, out);
+}
+
+#define OCTLB_INSTANTIATE(T)                                                                      \
+  template void LaunchTile<T>(const TileArgs<T>&, hipStream_t);                                   \
+  template void LaunchStatsFinalize<T>(const double*, int, double*, hipStream_t);                 \
+  template void LaunchInitEq<T>(T*, const uint8_t*, int, const SimConst&, double, const double*,  \
+                                uint32_t, hipStream_t);                                           \
+  template void LaunchInitField<T>(T*, int, const SimConst&, const double*, const double*,        \
+                                   hipStream_t);                                                  \
+  template void LaunchFd<T>(const FdArgs<T>&, const int*, hipStream_t);                           \
+  template void LaunchCouplerInit<T>(const CoupleArgs<T>&, int, hipStream_t);                     \
+  template void LaunchProlong<T>(const CoupleArgs<T>&, const int*, int, int, hipStream_t);        \
+  template void LaunchRestrict<T>(const CoupleArgs<T>&, const int*, int, hipStream_t);            \
+  template void LaunchDrag<T>(const DragArgs<T>&, int, const int*, int, double*, double*,         \
+                              hipStream_t);                                                       \
+  template void LaunchMass<T>(const T*, const T*, const ParArray&, const BlockTab*,               \
+                              const uint8_t*, const SimConst&, int, double*, double*, hipStream_t);
+
+OCTLB_INSTANTIATE(double)
+OCTLB_INSTANTIATE(float)
+
+}  // namespace octlb

@@ -1,0 +1,351 @@
+// Device data structures, boundary helpers and kernel launchers of the
+// per-time-step AMR D3Q19 update (gfx950).
+//
+// Two storage modes, both with two ping-pong copies and one parity bit per
+// refinement level:
+//  * blocked (8^3 blocks, halo 1 -- the AMR path):
+//      core  [block][19][512]  SoA, cell (ix*8+iy)*8+iz, shifted populations
+//      shell [shell][19][488]  halo slots of blocks that own a halo face or
+//            edge the same-level ghost exchange does not refresh (coarse-fine
+//            faces, domain faces, unexchanged edges).  Exchanged halo slots
+//            are never stored: the fused kernel reads the neighbour's cell
+//            and collides it on the fly, which is the value
+//            GhostSchedule::exchange copies there (ghost_schedule.h:121-255).
+//  * padded (one large block, any halo -- the single-block cavity path):
+//      [block][19][NP^3] halo-inclusive SoA, so the reference's rotation of
+//      the whole padded storage (block_lattice.cpp:551-577) is reproduced.
+#ifndef OCTLB_AMD_CSRC_LBM_DEVICE_H_
+#define OCTLB_AMD_CSRC_LBM_DEVICE_H_
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "d3q19.h"
+#include "octlb_gpu.h"
+
+namespace octlb {
+
+constexpr int kTile = 8;                       // tile edge (cells)
+constexpr int kBox = kTile + 2;                // tile + one-cell apron
+constexpr int kBoxCells = kBox * kBox * kBox;  // 1000
+constexpr int kTileCells = kTile * kTile * kTile;
+constexpr int kApron = kBoxCells - kTileCells;  // 488
+constexpr int kMaxLevels = 16;
+constexpr int kMacro = 4 + kQ;  // rho, u[3], f_neq[19]
+
+// Resolved DomainBcSpec of one face (FindSpec, interpolated_velocity.h:38-45).
+struct DevSpec {
+  int type;
+  int inlet;
+  int inlet_face;
+  int n0, n1;
+  double u_wall[3];
+  double rho_target;
+  double inlet_u[3];
+  double u_peak;
+  double lo0, hi0, lo1, hi1;
+  double ramp_end_t;
+};
+
+struct SimConst {
+  DevSpec spec[6];
+  double omega;
+  int n, h, NP, ncore, NS;
+  int plane;      // elements per population plane of one block (512 or NP^3)
+  int padded;     // storage mode
+  int handler;    // 0 NoOp, 1 concrete
+  int const_rho;
+  int bouzidi;    // per-cell stream path (else padded-block rotate)
+};
+
+enum : int {
+  kFlagLinks = 1,      // block has Bouzidi links
+  kFlagShellLive = 2,  // block's shell is read by its core: evolve and persist it
+  kFlagDrag = 4,       // block has drag-snapshot links
+};
+
+struct BlockTab {
+  int level;
+  int shell;    // shell index or -1
+  int flags;
+  int pad_;
+  int nbr[18];  // faces 0..5, edges 6..17; -1 = halo part not exchanged
+  double origin[3];
+  double cw;
+};
+
+struct Tile {
+  int block;
+  int ox, oy, oz;  // blocked: unused (tile = block); padded: padded-box origin
+};
+
+struct ParArray {
+  int v[kMaxLevels];
+};
+
+template <typename T>
+struct TileArgs {
+  const Tile* tiles;
+  int ntiles;
+  const BlockTab* bt;
+  const uint8_t* kind;
+  const T* src;
+  T* dst;
+  const T* shell_src;
+  T* shell_dst;
+  const uint32_t* qmask;
+  const int* qoff;
+  const double* qval;
+  const uint32_t* dmask;
+  const int* doff;
+  T* snap;
+  double* stats;          // [ntiles*2]
+  const double* avg_rho;  // previous step's ConstRho average
+  const int* step;        // coarse-step counter (time of the inlet ramps)
+  int want_stats;
+  SimConst sc;
+};
+
+template <typename T>
+struct FdArgs {
+  const int2* cells;     // (block, cell) per FD entry, phases plane|edge|corner
+  const int* smap;       // [fd block][ncore] -> entry or -1
+  const int* fd_index;   // block -> fd block or -1
+  const int* level;      // block -> level
+  const BlockTab* bt;
+  const uint8_t* kind;
+  T* pops0;
+  T* pops1;
+  T* post;               // [entries][19]
+  const int* step;
+  ParArray par;
+  SimConst sc;
+};
+
+template <typename T>
+struct CoupleArgs {
+  const int* plan;  // 12 ints per coupling point
+  T* pops0;
+  T* pops1;
+  T* prev;          // [points][kMacro]
+  ParArray par;
+  T prolong_scale;
+  T restrict_scale;
+};
+
+template <typename T>
+struct DragArgs {
+  const int4* links;  // block, cell, pop, snapshot index (reference loop order)
+  const BlockTab* bt;
+  const T* pops0;
+  const T* pops1;
+  const T* snap;      // nullptr: no snapshot, read the live populations
+  double* term;
+  uint8_t* use;
+  double bbox[6];
+  ParArray par;
+  SimConst sc;
+};
+
+__host__ __device__ inline int CoreSlot(const SimConst& sc, int c) {
+  if (!sc.padded) return c;
+  const int n = sc.n;
+  const int x = c / (n * n), y = (c / n) % n, z = c % n;
+  return ((x + sc.h) * sc.NP + (y + sc.h)) * sc.NP + z + sc.h;
+}
+
+// ------------------------------------------------------------- BC helpers
+// smoothstep ramp x^3(10+x(-15+6x)) (inlet_velocity_field.h:77-90)
+__device__ inline double RampD(double t, double ramp_end) {
+  if (ramp_end <= 0.0) return 1.0;
+  double x = t / ramp_end;
+  if (x < 0.0) {
+    x = 0.0;
+  } else if (x > 1.0) {
+    x = 1.0;
+  }
+  const double x2 = x * x;
+  const double x3 = x2 * x;
+  return x3 * (10.0 + x * (-15.0 + 6.0 * x));
+}
+
+__device__ inline double Clamp01(double c) { return c < 0.0 ? 0.0 : (c > 1.0 ? 1.0 : c); }
+
+// InletVelocityField::velocity / velocity_phys (inlet_velocity_field.h:19-270)
+__device__ inline void InletU(const DevSpec& sp, int ix, int iy, int iz, double t, const double* org,
+                              double cw, double* u) {
+  const bool xf = sp.inlet_face == OCTLB_XMIN || sp.inlet_face == OCTLB_XMAX;
+  const bool zf = sp.inlet_face == OCTLB_ZMIN || sp.inlet_face == OCTLB_ZMAX;
+  if (sp.inlet == OCTLB_INLET_CHANNEL) {
+    const double px = org[0] + (ix + 0.5) * cw;
+    const double py = org[1] + (iy + 0.5) * cw;
+    const double pz = org[2] + (iz + 0.5) * cw;
+    const double r = RampD(t, sp.ramp_end_t);
+    const double a0 = xf ? py : px;
+    const double a1 = zf ? py : pz;
+    const double c0 = Clamp01((a0 - sp.lo0) / (sp.hi0 - sp.lo0));
+    const double c1 = Clamp01((a1 - sp.lo1) / (sp.hi1 - sp.lo1));
+    const double p0 = 4.0 * c0 * (1.0 - c0);
+    const double p1 = 4.0 * c1 * (1.0 - c1);
+    const double mag = r * sp.u_peak * p0 * p1;
+    for (int d = 0; d < 3; ++d) u[d] = mag * sp.inlet_u[d];
+    return;
+  }
+  if (sp.inlet == OCTLB_INLET_POISEUILLE) {
+    const double r = RampD(t, sp.ramp_end_t);
+    const int i0 = xf ? iy : ix;
+    const int i1 = zf ? iy : iz;
+    const double c0 = (static_cast<double>(i0) + 0.5) / static_cast<double>(sp.n0);
+    const double c1 = (static_cast<double>(i1) + 0.5) / static_cast<double>(sp.n1);
+    const double p0 = 4.0 * c0 * (1.0 - c0);
+    const double p1 = 4.0 * c1 * (1.0 - c1);
+    const double mag = r * sp.u_peak * p0 * p1;
+    for (int d = 0; d < 3; ++d) u[d] = mag * sp.inlet_u[d];
+    return;
+  }
+  const double r = RampD(t, sp.ramp_end_t);
+  for (int d = 0; d < 3; ++d) u[d] = r * sp.inlet_u[d];
+}
+
+// detail::PrescribedBoundaryU (interpolated_velocity.h:69-117), block-local faces.
+__device__ inline void PrescribedU(const SimConst& sc, const double* org, double cw, int ix, int iy,
+                                   int iz, double t, double* u) {
+  const int n = sc.n;
+  u[0] = u[1] = u[2] = 0.0;
+  const bool on[6] = {ix == 0, ix == n - 1, iy == 0, iy == n - 1, iz == 0, iz == n - 1};
+  for (int f = 0; f < 6; ++f) {
+    if (!on[f]) continue;
+    if (sc.spec[f].inlet != OCTLB_INLET_NONE) {
+      InletU(sc.spec[f], ix, iy, iz, t, org, cw, u);
+      return;
+    }
+  }
+  if (iy == n - 1 && ix > 0 && ix < n - 1 && iz > 0 && iz < n - 1) {
+    for (int d = 0; d < 3; ++d) u[d] = sc.spec[OCTLB_YMAX].u_wall[d];
+  }
+}
+
+// detail::FlatBoundaryFaceInfo (interpolated_velocity.h:134-167)
+__device__ inline bool FlatInfo(int n, int ix, int iy, int iz, int& dir, int& ori) {
+  int count = 0;
+  dir = 0;
+  ori = 0;
+  if (ix == 0) {
+    ++count;
+    dir = 0;
+    ori = -1;
+  } else if (ix == n - 1) {
+    ++count;
+    dir = 0;
+    ori = 1;
+  }
+  if (iy == 0) {
+    ++count;
+    dir = 1;
+    ori = -1;
+  } else if (iy == n - 1) {
+    ++count;
+    dir = 1;
+    ori = 1;
+  }
+  if (iz == 0) {
+    ++count;
+    dir = 2;
+    ori = -1;
+  } else if (iz == n - 1) {
+    ++count;
+    dir = 2;
+    ori = 1;
+  }
+  return count == 1;
+}
+
+// BcDispatcher::collide (bc_dispatcher.h:32-81) for the concrete handler;
+// BlockLattice::collide / collide_const_rho (block_lattice.cpp:376-499) for
+// the NoOp handler.  Statistics follow CollideRhoStats (block_lattice.h:22-32).
+template <typename T>
+__device__ inline void CollideCell(T* f, int k, const SimConst& sc, const double* org, double cw,
+                                   int ix, int iy, int iz, double t, T avg, double* ssum,
+                                   int* scnt) {
+  const T omega = static_cast<T>(sc.omega);
+  if (k == kBulk) {
+    if (sc.const_rho) {
+      T rho, u[3];
+      ProxyRhoU(f, rho, u);
+      if (ssum) {
+        *ssum += static_cast<double>(rho + (T(1) - avg));
+        *scnt += 1;
+      }
+      ConstRhoBgk(f, omega, avg, rho, u);
+    } else if (sc.handler == OCTLB_HANDLER_CONCRETE) {
+      T rho, u[3];
+      ProxyRhoU(f, rho, u);
+      OlbBgk(f, rho, u, omega);
+    } else {
+      CollideBgkAt(f, omega);
+    }
+    return;
+  }
+  if (sc.handler != OCTLB_HANDLER_CONCRETE) return;
+  if (k == kBounceBack) {
+    T tmp[kQ];
+#pragma unroll
+    for (int i = 0; i < kQ; ++i) tmp[i] = f[i];
+#pragma unroll
+    for (int i = 0; i < kQ; ++i) f[i] = tmp[Opp(i)];
+    return;
+  }
+  if (k == kVelocity) {
+    // CollideDirichletBoundaryCellAt (interpolated_velocity.h:858-884)
+    double ud[3];
+    PrescribedU(sc, org, cw, ix, iy, iz, t, ud);
+    T u[3] = {static_cast<T>(ud[0]), static_cast<T>(ud[1]), static_cast<T>(ud[2])};
+    const int n = sc.n;
+    const int fc = (ix == 0) + (ix == n - 1) + (iy == 0) + (iy == n - 1) + (iz == 0) + (iz == n - 1);
+    T rho;
+    if (fc >= 2) {
+      T ut[3];
+      ProxyRhoU(f, rho, ut);
+    } else {
+      int dir, ori;
+      FlatInfo(n, ix, iy, iz, dir, ori);
+      rho = VelRhoFromPop(dir, ori, f, u);
+    }
+    OlbBgk(f, rho, u, omega);
+    if (ssum) {
+      *ssum += static_cast<double>(rho);
+      *scnt += 1;
+    }
+  }
+}
+
+// --------------------------------------------------------------- launchers
+template <typename T>
+void LaunchTile(const TileArgs<T>& a, hipStream_t s);
+template <typename T>
+void LaunchStatsFinalize(const double* partial, int ntiles, double* avg, hipStream_t s);
+void LaunchStepInc(int* step, hipStream_t s);
+template <typename T>
+void LaunchInitEq(T* pops, const uint8_t* kind, int nb, const SimConst& sc, double rho0,
+                  const double* u0, uint32_t mask, hipStream_t s);
+template <typename T>
+void LaunchInitField(T* pops, int nb, const SimConst& sc, const double* rho, const double* u,
+                     hipStream_t s);
+template <typename T>
+void LaunchFd(const FdArgs<T>& a, const int* phase_begin, hipStream_t s);
+template <typename T>
+void LaunchCouplerInit(const CoupleArgs<T>& a, int npts, hipStream_t s);
+template <typename T>
+void LaunchProlong(const CoupleArgs<T>& a, const int* list, int count, int half, hipStream_t s);
+template <typename T>
+void LaunchRestrict(const CoupleArgs<T>& a, const int* list, int count, hipStream_t s);
+template <typename T>
+void LaunchDrag(const DragArgs<T>& a, int nlinks, const int* block_begin, int nblk, double* block_out,
+                double* out, hipStream_t s);
+template <typename T>
+void LaunchMass(const T* p0, const T* p1, const ParArray& par, const BlockTab* bt, const uint8_t* kind,
+                const SimConst& sc, int nb, double* partial, double* out, hipStream_t s);
+
+}  // namespace octlb
+
+#endif  // OCTLB_AMD_CSRC_LBM_DEVICE_H_

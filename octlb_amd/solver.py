@@ -1,0 +1,126 @@
+"""Device solver handle over liboctlb_gpu.so.
+
+``GpuSolver`` stands where the reference constructs BlockCollection +
+GhostSchedule + LevelCoupler + DomainBoundaryHandler + TimeLoop
+(examples/*/..._case.h) and calls ``TimeLoop::advance_one`` (time_loop.h:56).
+Every call goes to the HIP library; there is no CPU path.
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from . import abi
+
+
+class GpuSolver:
+    def __init__(self, topo: abi.TopologyArrays, params: abi.Params):
+        self._lib = abi.lib()
+        self.topo = topo
+        self.params = params
+        self.n = topo.n
+        self.halo = topo.halo
+        self.nb = topo.num_blocks
+        self.ncore = self.n ** 3
+        self.np_ = self.n + 2 * self.halo
+        h = C.c_void_p()
+        t = topo.struct()
+        abi.check(self._lib.octlb_dev_create(C.byref(t), C.byref(params), C.byref(h)), "dev create")
+        self._h = h
+
+    def close(self):
+        if getattr(self, "_h", None) is not None and self._h.value:
+            self._lib.octlb_dev_destroy(self._h)
+        self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # ---- setup
+    def set_bc_kinds(self, kinds):
+        k = np.ascontiguousarray(kinds, dtype=np.uint8).reshape(-1)
+        assert k.size == self.nb * self.ncore
+        abi.check(self._lib.octlb_dev_set_bc_kinds(self._h, abi.u8p(k)), "set_bc_kinds")
+
+    def set_links(self, link, q):
+        link = np.ascontiguousarray(link, dtype=np.int32).reshape(-1, 3)
+        q = np.ascontiguousarray(q, dtype=np.float64).reshape(-1)
+        assert link.shape[0] == q.size
+        abi.check(self._lib.octlb_dev_set_links(self._h, int(q.size), abi.i32p(link), abi.f64p(q)),
+                  "set_links")
+
+    def init_equilibrium(self, rho0=1.0, u0=(0.0, 0.0, 0.0), kind_mask=0xFF):
+        u = np.ascontiguousarray(u0, dtype=np.float64)
+        abi.check(self._lib.octlb_dev_init_equilibrium(self._h, float(rho0), abi.f64p(u),
+                                                       int(kind_mask)), "init")
+
+    def init_field(self, rho, u):
+        r = np.ascontiguousarray(rho, dtype=np.float64).reshape(-1)
+        uu = np.ascontiguousarray(u, dtype=np.float64).reshape(-1)
+        abi.check(self._lib.octlb_dev_init_field(self._h, abi.f64p(r), abi.f64p(uu)), "init_field")
+
+    def upload_block(self, b, aos):
+        a = np.ascontiguousarray(aos, dtype=np.float64).reshape(-1)
+        assert a.size == self.np_ ** 3 * abi.Q
+        abi.check(self._lib.octlb_dev_upload_block(self._h, int(b), abi.f64p(a)), "upload")
+
+    def download_block(self, b):
+        a = np.empty(self.np_ ** 3 * abi.Q)
+        abi.check(self._lib.octlb_dev_download_block(self._h, int(b), abi.f64p(a)), "download")
+        return a.reshape(self.np_, self.np_, self.np_, abi.Q)
+
+    def download_core(self):
+        a = np.empty(self.nb * self.ncore * abi.Q)
+        abi.check(self._lib.octlb_dev_download_core(self._h, abi.f64p(a)), "download_core")
+        return a.reshape(self.nb, self.n, self.n, self.n, abi.Q)
+
+    # ---- stepping
+    def advance(self, steps=1):
+        abi.check(self._lib.octlb_dev_advance(self._h, int(steps)), "advance")
+
+    def advance_graph(self, steps=1):
+        abi.check(self._lib.octlb_dev_advance_graph(self._h, int(steps)), "advance_graph")
+
+    def synchronize(self):
+        abi.check(self._lib.octlb_dev_synchronize(self._h), "synchronize")
+
+    def set_step(self, step):
+        abi.check(self._lib.octlb_dev_set_step(self._h, int(step)), "set_step")
+
+    # ---- observables
+    def drag(self, bbox):
+        b = np.ascontiguousarray(bbox, dtype=np.float64)
+        out = np.zeros(3)
+        abi.check(self._lib.octlb_dev_drag(self._h, abi.f64p(b), abi.f64p(out)), "drag")
+        return out
+
+    def mass(self):
+        out = np.zeros(1)
+        abi.check(self._lib.octlb_dev_mass(self._h, abi.f64p(out)), "mass")
+        return float(out[0])
+
+    def average_rho(self):
+        out = np.zeros(1)
+        abi.check(self._lib.octlb_dev_average_rho(self._h, abi.f64p(out)), "average_rho")
+        return float(out[0])
+
+    def update_counts(self):
+        a = np.zeros(1)
+        f = np.zeros(1)
+        abi.check(self._lib.octlb_dev_update_counts(self._h, abi.f64p(a), abi.f64p(f)), "counts")
+        return float(a[0]), float(f[0])
+
+    def enable_timing(self, on=True):
+        abi.check(self._lib.octlb_dev_enable_timing(self._h, 1 if on else 0), "timing")
+
+    def kernel_timing(self):
+        ms = C.c_double()
+        launches = C.c_int64()
+        upd = C.c_double()
+        abi.check(self._lib.octlb_dev_kernel_timing(self._h, C.byref(ms), C.byref(launches),
+                                                    C.byref(upd)), "timing")
+        return ms.value, launches.value, upd.value

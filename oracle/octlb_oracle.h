@@ -1,0 +1,70 @@
+/*
+ * octlb_oracle.h -- CPU restatement of the reference's per-time-step AMR
+ * D3Q19 LBM update (gezengran/octlb, TimeLoop::advance_one and everything it
+ * calls).  TEST INFRASTRUCTURE ONLY: tests, the smoke check and the bench's
+ * CPU-baseline leg use it as the checker; the product never links it.
+ *
+ * Storage mirrors the reference BlockLattice exactly: per block an AoS array
+ * of (n+2h)^3 halo-inclusive slots x 19 shifted FP64 populations
+ * (block_lattice.h:159-163, 294-304), so block dumps compare 1:1 with the
+ * reference harness in oracle/_ref and with the GPU path's downloads.
+ */
+#ifndef OCTLB_ORACLE_H_
+#define OCTLB_ORACLE_H_
+
+#include <stdint.h>
+
+#include "../include/octlb_gpu.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct ol_sim ol_sim;
+
+ol_sim* ol_create(const octlb_topology* topo, const octlb_params* params);
+void ol_destroy(ol_sim* sim);
+const char* ol_last_error(void);
+
+int ol_set_bc_kinds(ol_sim* sim, const uint8_t* kinds);
+int ol_set_links(ol_sim* sim, int64_t count, const int32_t* link, const double* q);
+int ol_init_equilibrium(ol_sim* sim, double rho0, const double u0[3], uint32_t kind_mask);
+int ol_init_field(ol_sim* sim, const double* rho, const double* u);
+int ol_set_block(ol_sim* sim, int32_t block, const double* aos);
+int ol_get_block(const ol_sim* sim, int32_t block, double* aos);
+int ol_get_core(const ol_sim* sim, double* out);
+int ol_get_snapshot_block(const ol_sim* sim, int32_t block, double* aos);
+
+int ol_advance(ol_sim* sim, int32_t n_coarse_steps);
+int ol_set_threads(ol_sim* sim, int32_t nthreads);
+int ol_set_step(ol_sim* sim, int32_t step);
+
+int ol_drag(const ol_sim* sim, const double bbox[6], double out[3]);
+double ol_mass(const ol_sim* sim);
+double ol_average_rho(const ol_sim* sim);
+
+/* TimeLoopCounters (time_loop.h:19-34): per-level collide/stream counts and
+ * the coupler call log as (phase, coarse_level) pairs. */
+int ol_counters(const ol_sim* sim, int32_t* collide, int32_t* stream, int32_t max_levels,
+                int32_t* calls, int32_t max_calls, int32_t* num_calls);
+
+/* Coupling plan (level_coupler.cpp:70-149): rows of 12 ints
+ * coarse_id,ci,cj,ck,fine_id,fi,fj,fk,normal,coarse_level,fine_level,fine_slot. */
+int32_t ol_plan_size(const ol_sim* sim);
+int ol_plan(const ol_sim* sim, int32_t* rows);
+
+/* Same-rank edge exchange list (ghost_schedule.h:56-91): rows local,d1,d2,remote. */
+int32_t ol_num_edges(const ol_sim* sim);
+int ol_edges(const ol_sim* sim, int32_t* rows);
+
+/* Single-block primitives used by the unit checks against the reference
+ * harness: one phase of the loop on one block. */
+int ol_block_collide(ol_sim* sim, int32_t block, double t);
+int ol_block_stream(ol_sim* sim, int32_t block);
+int ol_block_post_stream(ol_sim* sim, int32_t block, double t);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* OCTLB_ORACLE_H_ */
