@@ -135,6 +135,9 @@ typedef struct {
   int32_t snapshot;      /* set_snapshot_post_collide (time_loop.h:84)          */
   int32_t num_specs;
   const octlb_bc_spec* specs;
+  int32_t fast_stats;    /* ConstRho average: 0 = the reference's sequential
+                            cell-order sum (bit-reproducible), 1 = parallel
+                            tree reduction (faster, order differs)          */
 } octlb_params;
 
 const char* octlb_last_error(void);

@@ -65,6 +65,7 @@ class Params(C.Structure):
         ("omega", C.c_double), ("precision", C.c_int32), ("handler", C.c_int32),
         ("const_rho", C.c_int32), ("stats_scope", C.c_int32), ("bouzidi", C.c_int32),
         ("snapshot", C.c_int32), ("num_specs", C.c_int32), ("specs", C.POINTER(BcSpec)),
+        ("fast_stats", C.c_int32),
     ]
 
 
@@ -126,7 +127,7 @@ class TopologyArrays:
 
 
 def make_params(omega, precision=64, handler=HANDLER_CONCRETE, const_rho=0, stats_scope=0,
-                bouzidi=0, snapshot=0, specs=()):
+                bouzidi=0, snapshot=0, specs=(), fast_stats=0):
     arr = (BcSpec * max(1, len(specs)))()
     for i, s in enumerate(specs):
         arr[i] = s
@@ -140,6 +141,7 @@ def make_params(omega, precision=64, handler=HANDLER_CONCRETE, const_rho=0, stat
     p.snapshot = snapshot
     p.num_specs = len(specs)
     p.specs = C.cast(arr, C.POINTER(BcSpec))
+    p.fast_stats = fast_stats
     p._keep = arr  # keep the spec array alive with the struct
     return p
 

@@ -128,6 +128,9 @@ struct octlb_dev {
   double* d_dblock_out = nullptr;
   int ndlinks = 0, ndblocks = 0;
   double* d_stats = nullptr;
+  double* d_stat_val = nullptr;
+  uint8_t* d_stat_use = nullptr;
+  bool fast_stats = false;
   double* d_avg = nullptr;
   int* d_step = nullptr;
   double* d_out = nullptr;
@@ -458,6 +461,11 @@ void octlb_dev::Build(const octlb_topology* t, const octlb_params* p) {
   d_level = Alloc<int>(nb);
   Upload(d_level, level, stream);
   d_stats = Alloc<double>(2 * tiles.size());
+  fast_stats = p->fast_stats != 0;
+  if (sc.const_rho && !fast_stats) {
+    d_stat_val = Alloc<double>(static_cast<size_t>(nb) * ncore);
+    d_stat_use = Alloc<uint8_t>(static_cast<size_t>(nb) * ncore);
+  }
   d_avg = Alloc<double>(1);
   const double one = 1.0;
   HIP_CHECK(hipMemcpyAsync(d_avg, &one, sizeof(double), hipMemcpyHostToDevice, stream));
@@ -677,9 +685,15 @@ void octlb_dev::Level(int l) {
     a.doff = d_doff;
     a.snap = static_cast<T*>(d_snap);
     a.stats = d_stats;
+    a.stat_val = d_stat_val;
+    a.stat_use = d_stat_use;
+    {
+      const int tpa = (NP + kTile - 1) / kTile;
+      a.tiles_per_block = padded ? tpa * tpa * tpa : 1;
+    }
     a.avg_rho = d_avg;
     a.step = d_step;
-    a.want_stats = (sc.const_rho && l >= max_level) ? 1 : 0;
+    a.want_stats = (sc.const_rho && l >= max_level) ? (fast_stats ? 1 : 2) : 0;
     a.sc = sc;
     const bool timed = timing && !capturing;
     if (timed) {
@@ -701,7 +715,8 @@ void octlb_dev::Level(int l) {
       tile_updates += static_cast<double>(cnt / tpb) * static_cast<double>(ncore);
     }
     par[l] ^= 1;
-    if (a.want_stats) LaunchStatsFinalize<T>(d_stats, cnt, d_avg, stream);
+    if (a.want_stats == 1) LaunchStatsFinalize<T>(d_stats, cnt, d_avg, stream);
+    if (a.want_stats == 2) LaunchStatsSequential(d_stat_val, d_stat_use, (cnt / a.tiles_per_block) * ncore, d_avg, stream);
   }
   if (fd_phase[3] > 0) {
     FdArgs<T> f{};

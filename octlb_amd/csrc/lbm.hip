@@ -226,7 +226,13 @@ __global__ __launch_bounds__(512) void KBlock(TileArgs<T> a) {
         if ((dm >> i) & 1u) sp[j++] = f[i];
     }
   }
-  if (a.want_stats) BlockReduceStats<T>(ssum, scnt, red_s, red_c, a.stats + 2 * ti);
+  if (a.want_stats == 1) {
+    BlockReduceStats<T>(ssum, scnt, red_s, red_c, a.stats + 2 * ti);
+  } else if (a.want_stats == 2) {
+    const size_t ord = static_cast<size_t>(ti) * ncore + tid;
+    a.stat_val[ord] = ssum;
+    a.stat_use[ord] = static_cast<uint8_t>(scnt);
+  }
 }
 
 // -------------------------------------------------------- padded fused kernel
@@ -360,7 +366,13 @@ __global__ __launch_bounds__(512) void KPadded(TileArgs<T> a) {
       }
     }
   }
-  if (a.want_stats) BlockReduceStats<T>(ssum, scnt, red_s, red_c, a.stats + 2 * ti);
+  if (a.want_stats == 1) {
+    BlockReduceStats<T>(ssum, scnt, red_s, red_c, a.stats + 2 * ti);
+  } else if (a.want_stats == 2 && core) {
+    const size_t ord = static_cast<size_t>(ti / a.tiles_per_block) * sc.ncore + (x * n + y) * n + z;
+    a.stat_val[ord] = ssum;
+    a.stat_use[ord] = static_cast<uint8_t>(scnt);
+  }
 }
 
 // ------------------------------------------------------------ small kernels
@@ -372,6 +384,21 @@ __global__ void KStatsFinalize(const double* partial, int ntiles, double* avg) {
     c += partial[2 * i + 1];
   }
   *avg = c > 0.0 ? s / c : 1.0;  // CollideRhoStats::average (block_lattice.h:31)
+}
+
+// CollideRhoStats accumulated in the reference's cell order
+// (blocks of the level in order, cells (ix,iy,iz) lexicographic).
+__global__ void KStatsSequential(const double* val, const uint8_t* use, int count, double* avg) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  double s = 0.0;
+  int c = 0;
+  for (int k = 0; k < count; ++k) {
+    if (use[k]) {
+      s += val[k];
+      ++c;
+    }
+  }
+  *avg = c > 0 ? s / static_cast<double>(c) : 1.0;
 }
 
 __global__ void KStepInc(int* step) {
@@ -431,9 +458,15 @@ struct FdView {
     const int n = sc->n;
     return x >= 0 && x < n && y >= 0 && y < n && z >= 0 && z < n;
   }
-  // IsPaddingOverlapCell (interpolated_velocity.h:238-240); h >= 1 and every
-  // stencil read lies within one cell of the block.
-  __device__ bool Padding(int x, int y, int z) const { return !InDomain(x, y, z); }
+  // IsPaddingOverlapCell (interpolated_velocity.h:238-240): outside the block
+  // but within min(h, 2) of it.  Farther reads (halo 1, mixed-sign edges of
+  // ApplyEdgeBoundary) take the mirror path (:254-279, 407-412).
+  __device__ bool Padding(int x, int y, int z) const {
+    const int n = sc->n, st = sc->h < 2 ? sc->h : 2;
+    return !InDomain(x, y, z) && x >= -st && x < n + st && y >= -st && y < n + st && z >= -st &&
+           z < n + st;
+  }
+  __device__ static int Mirror(int i, int n) { return i < 0 ? -i : (i >= n ? 2 * (n - 1) - i : i); }
   __device__ int Kind(int x, int y, int z) const { return kind[(x * sc->n + y) * sc->n + z]; }
   __device__ void Pops(int x, int y, int z, T* f) const {
     const int c = (x * sc->n + y) * sc->n + z;
@@ -454,23 +487,38 @@ struct FdView {
     u[1] = static_cast<T>(ud[1]);
     u[2] = static_cast<T>(ud[2]);
   }
-  // BoundaryLatticeView::ComputeU with ExtrapolatedU unrolled into a loop
+  // BoundaryLatticeView::ComputeU (interpolated_velocity.h:376-401) with
+  // MirrorExteriorVelocity (:266-279) and ExtrapolatedU (:297-316) unrolled
+  // into one loop (no device recursion).
   __device__ void ComputeU(int x, int y, int z, T* u) const {
+    const int n = sc->n;
+    bool fx = false, fy = false, fz = false;
+    bool check = true;
     for (;;) {
-      if (Padding(x, y, z)) {
-        u[0] = u[1] = u[2] = T(0);
-        return;
+      if (check) {
+        if (Padding(x, y, z)) {
+          u[0] = u[1] = u[2] = T(0);
+          break;
+        }
+        if (!InDomain(x, y, z)) {
+          fx = x < 0 || x >= n;
+          fy = y < 0 || y >= n;
+          fz = z < 0 || z >= n;
+          x = Mirror(x, n);
+          y = Mirror(y, n);
+          z = Mirror(z, n);
+        }
       }
       const int k = Kind(x, y, z);
       if (k == kVelocity) {
         Prescribed(x, y, z, u);
-        return;
+        break;
       }
       if (k == kPressure) {
         int dir, ori;
-        if (!FlatInfo(sc->n, x, y, z, dir, ori)) {
+        if (!FlatInfo(n, x, y, z, dir, ori)) {
           u[0] = u[1] = u[2] = T(0);
-          return;
+          break;
         }
         if (dir == 0) {
           x -= ori;
@@ -479,13 +527,17 @@ struct FdView {
         } else {
           z -= ori;
         }
+        check = true;
         continue;
       }
       T f[kQ], rho;
       Pops(x, y, z, f);
       ProxyRhoU(f, rho, u);
-      return;
+      break;
     }
+    if (fx) u[0] = -u[0];
+    if (fy) u[1] = -u[1];
+    if (fz) u[2] = -u[2];
   }
   __device__ T PrescribedRho(int x, int y, int z) const {
     int dir, ori;
@@ -494,6 +546,12 @@ struct FdView {
   }
   __device__ T ComputeRho(int x, int y, int z) const {
     if (Padding(x, y, z)) return T(1);
+    if (!InDomain(x, y, z)) {
+      const int n = sc->n;
+      x = Mirror(x, n);
+      y = Mirror(y, n);
+      z = Mirror(z, n);
+    }
     const int k = Kind(x, y, z);
     if (k == kPressure) return PrescribedRho(x, y, z);
     T f[kQ];
@@ -902,6 +960,10 @@ static inline dim3 Grid(long long n, int bs) { return dim3(static_cast<unsigned>
 template <typename T>
 void LaunchStatsFinalize(const double* partial, int ntiles, double* avg, hipStream_t s) {
   hipLaunchKernelGGL(KStatsFinalize, dim3(1), dim3(1), 0, s, partial, ntiles, avg);
+}
+
+void LaunchStatsSequential(const double* val, const uint8_t* use, int count, double* avg, hipStream_t s) {
+  hipLaunchKernelGGL(KStatsSequential, dim3(1), dim3(1), 0, s, val, use, count, avg);
 }
 
 void LaunchStepInc(int* step, hipStream_t s) { hipLaunchKernelGGL(KStepInc, dim3(1), dim3(1), 0, s, step); }

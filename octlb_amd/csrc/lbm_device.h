@@ -99,10 +99,13 @@ struct TileArgs {
   const uint32_t* dmask;
   const int* doff;
   T* snap;
-  double* stats;          // [ntiles*2]
+  double* stats;          // [ntiles*2] partial sums (want_stats == 1)
+  double* stat_val;       // per-cell statistic in reference order (want_stats == 2)
+  uint8_t* stat_use;
+  int tiles_per_block;
   const double* avg_rho;  // previous step's ConstRho average
   const int* step;        // coarse-step counter (time of the inlet ramps)
-  int want_stats;
+  int want_stats;         // 0 none, 1 parallel partial sums, 2 reference order
   SimConst sc;
 };
 
@@ -324,6 +327,7 @@ template <typename T>
 void LaunchTile(const TileArgs<T>& a, hipStream_t s);
 template <typename T>
 void LaunchStatsFinalize(const double* partial, int ntiles, double* avg, hipStream_t s);
+void LaunchStatsSequential(const double* val, const uint8_t* use, int count, double* avg, hipStream_t s);
 void LaunchStepInc(int* step, hipStream_t s);
 template <typename T>
 void LaunchInitEq(T* pops, const uint8_t* kind, int nb, const SimConst& sc, double rho0,

@@ -95,7 +95,7 @@ def _two_level():
 
 def test_two_level_cylinder():
     case = _two_level()
-    assert case.topo.max_level == 2 and case.topo.num_coarse_fine > 0
+    assert case.topo.max_level == 2 and case.topo.num_cf > 0
     o, g = _run_both(case, 12)
     a, b = g.download_core(), o.download_core()
     assert rel_err(a, b) <= 1e-12
