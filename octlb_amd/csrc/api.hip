@@ -146,6 +146,9 @@ struct octlb_dev {
   int* d_fd_index = nullptr;
   void* d_fd_post = nullptr;
   int fd_phase[4] = {0, 0, 0, 0};
+  int* d_fd_blocks = nullptr;  // blocked mode: boundary blocks with FD cells
+  short* d_fd_slot = nullptr;  // [fd block][512] FD slot of each cell or -1
+  int nfd_blocks = 0;
   std::vector<void*> owned;
   std::vector<void*> prep_owned;  // tables rebuilt by Prepare()
   // ---- state
@@ -638,6 +641,25 @@ void octlb_dev::Prepare() {
     d_fd_index = PAlloc<int>(nb);
     Upload(d_fd_index, fd_index, stream);
     d_fd_post = PAllocBytes(std::max<size_t>(all.size(), 1) * kQ * elem);
+    if (!padded) {
+      std::vector<int> fdb;
+      std::vector<short> slots;
+      for (int b : tb_blocks) {
+        const uint8_t* kb = &kinds[static_cast<size_t>(b) * ncore];
+        std::vector<short> sl(ncore, -1);
+        short k = 0;
+        for (int c = 0; c < ncore; ++c)
+          if (IsFd(kb[c])) sl[c] = k++;
+        if (k == 0) continue;
+        fdb.push_back(b);
+        slots.insert(slots.end(), sl.begin(), sl.end());
+      }
+      nfd_blocks = static_cast<int>(fdb.size());
+      d_fd_blocks = PAlloc<int>(fdb.size());
+      Upload(d_fd_blocks, fdb, stream);
+      d_fd_slot = PAlloc<short>(slots.size());
+      Upload(d_fd_slot, slots, stream);
+    }
   } else {
     fd_phase[0] = fd_phase[1] = fd_phase[2] = fd_phase[3] = 0;
   }
@@ -732,7 +754,11 @@ void octlb_dev::Level(int l) {
     f.step = d_step;
     f.par = Par();
     f.sc = sc;
-    LaunchFd<T>(f, fd_phase, stream);
+    if (padded) {
+      LaunchFd<T>(f, fd_phase, stream);
+    } else {
+      LaunchFdBlocks<T>(f, d_fd_blocks, d_fd_slot, nfd_blocks, stream);
+    }
   }
   if (l >= max_level) return;
   CoupleArgs<T> c{};

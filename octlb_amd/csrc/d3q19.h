@@ -180,19 +180,27 @@ __host__ __device__ inline double BouzidiPull(double f_bb_opp, double f_interior
   return 2.0 * q * f_bb_opp + (1.0 - 2.0 * q) * f_same;
 }
 
-// VelocityBoundaryRhoFromPop (interpolated_velocity.h:170-211); a zero
-// orientation takes the +1 instantiation like the reference's dispatch.
-template <typename T>
-__host__ __device__ inline T VelRhoFromPop(int direction, int orientation, const T* f, const T* u) {
-  const int o = orientation < 0 ? -1 : 1;
+// VelocityBoundaryRhoFromPop (interpolated_velocity.h:170-211) for a fixed
+// face direction / orientation (compile-time, like the reference's template).
+template <int D, int O, typename T>
+__host__ __device__ inline T VelRhoImpl(const T* f, const T* u) {
   T on_wall = T(0), normal = T(0);
 #pragma unroll
   for (int i = 0; i < kQ; ++i)
-    if (Ci(i, direction) == 0) on_wall += f[i];
+    if (Ci(i, D) == 0) on_wall += f[i];
 #pragma unroll
   for (int i = 0; i < kQ; ++i)
-    if (Ci(i, direction) == o) normal += f[i];
-  return (T(2) * normal + on_wall + T(1)) / (T(1) + T(o) * u[direction]);
+    if (Ci(i, D) == O) normal += f[i];
+  return (T(2) * normal + on_wall + T(1)) / (T(1) + T(O) * u[D]);
+}
+
+// Runtime dispatch; a zero orientation takes the +1 instantiation like the
+// reference's dispatch (interpolated_velocity.h:192-211).
+template <typename T>
+__host__ __device__ inline T VelRhoFromPop(int direction, int orientation, const T* f, const T* u) {
+  if (direction == 0) return orientation < 0 ? VelRhoImpl<0, -1>(f, u) : VelRhoImpl<0, 1>(f, u);
+  if (direction == 1) return orientation < 0 ? VelRhoImpl<1, -1>(f, u) : VelRhoImpl<1, 1>(f, u);
+  return orientation < 0 ? VelRhoImpl<2, -1>(f, u) : VelRhoImpl<2, 1>(f, u);
 }
 
 }  // namespace octlb

@@ -49,6 +49,7 @@ __device__ inline int XcdTile(int w, int nt) {
 template <typename T>
 __device__ inline void BlockReduceStats(double s, int c, double* red_s, int* red_c, double* out) {
   const int tid = threadIdx.x;
+  __syncthreads();  // red_s / red_c alias the streaming planes
   red_s[tid] = s;
   red_c[tid] = c;
   __syncthreads();
@@ -67,63 +68,48 @@ __device__ inline void BlockReduceStats(double s, int c, double* red_s, int* red
 
 // Pull-stream rule of one core cell (block_lattice.cpp:579-674), given the
 // LDS plane of population i at stream time.  `own` holds the cell's
-// post-collide populations.
-template <typename T, bool PERCELL, int I>
-__device__ inline T StreamPop(const T* pl, const uint8_t* kbox, int own_l, int k, const T* own,
-                              bool src_interior, uint32_t qm, const double* qv) {
-  constexpr int off = (Cx(I) * kBox + Cy(I)) * kBox + Cz(I);
-  if (!PERCELL) return pl[own_l - off];
-  if (k == kSolid) return own[I];
-  if (ReflectsOnPull(k) && !src_interior) return own[Opp(I)];
+// post-collide populations; i is a constant after the caller's unrolling.
+// own_i / own_opp are the cell's post-collide f_i and f_opp(i) as register
+// values (never addresses, so the population array stays in VGPRs).
+template <typename T, bool PERCELL>
+__device__ __forceinline__ T StreamPop(int i, const T* pl, const uint8_t* kbox, int own_l, int k,
+                                       T own_i, T own_opp, bool src_interior, uint32_t qm,
+                                       const double* qv) {
+  const int off = (Cx(i) * kBox + Cy(i)) * kBox + Cz(i);
+  const T pulled = pl[own_l - off];
+  if (!PERCELL) return pulled;
+  if (k == kSolid) return own_i;
+  if (ReflectsOnPull(k) && !src_interior) return own_opp;
   const int sk = kbox[own_l - off];
   if (sk == kSolid || sk == kBouzidi) {
     double q = 1.0;
-    if ((qm >> I) & 1u) q = qv[__popc(qm & ((1u << I) - 1u))];
+    if ((qm >> i) & 1u) q = qv[__popc(qm & ((1u << i) - 1u))];
     if (q > 1.0e-10 && q < 1.0 - 1.0e-10) {
       const int nl = own_l + off;
-      const double f_int = (kbox[nl] == kBulk) ? static_cast<double>(pl[nl]) : static_cast<double>(own[I]);
-      return static_cast<T>(BouzidiPull(static_cast<double>(own[Opp(I)]), f_int, static_cast<double>(own[I]), q));
+      const T nb_i = pl[nl];
+      const double f_int = (kbox[nl] == kBulk) ? static_cast<double>(nb_i) : static_cast<double>(own_i);
+      return static_cast<T>(BouzidiPull(static_cast<double>(own_opp), f_int, static_cast<double>(own_i), q));
     }
-    return own[Opp(I)];
+    return own_opp;
   }
-  return pl[own_l - off];
+  return pulled;
 }
 
-template <typename T, bool PERCELL>
-struct PopLoop {
-  template <int I>
-  __device__ static inline void Run(T (*plane)[kBoxCells], const uint8_t* kbox, int own_l, int ap_l,
-                                    int k, const T* f, const T* g, T* out, T* sout, bool live,
-                                    int ix, int iy, int iz, int n, uint32_t qm, const double* qv) {
-    if constexpr (I < kQ) {
-      T* pl = plane[I & 1];
-      pl[own_l] = f[I];
-      if (ap_l >= 0) pl[ap_l] = g[I];
-      __syncthreads();
-      const int sx = ix - Cx(I), sy = iy - Cy(I), sz = iz - Cz(I);
-      const bool src_int = sx >= 0 && sx < n && sy >= 0 && sy < n && sz >= 0 && sz < n;
-      out[I] = StreamPop<T, PERCELL, I>(pl, kbox, own_l, k, f, src_int, qm, qv);
-      if (live) {
-        // padded-block torus pull of a halo slot; in-halo sources and wrapped
-        // sources are both (d - off) mod NP^3 (block_lattice.cpp:595-611)
-        constexpr int off = (Cx(I) * kBox + Cy(I)) * kBox + Cz(I);
-        int s = ap_l - off;
-        if (s < 0) s += kBoxCells;
-        if (s >= kBoxCells) s -= kBoxCells;
-        sout[I] = pl[s];
-      }
-      Run<I + 1>(plane, kbox, own_l, ap_l, k, f, g, out, sout, live, ix, iy, iz, n, qm, qv);
-    }
-  }
+// ------------------------------------------------------- blocked fused kernel
+// Populations streamed per LDS round: FP32 4 planes (5 rounds), FP64 2 (10).
+template <typename T>
+struct Rounds {
+  static constexpr int P = sizeof(T) == 4 ? 4 : 2;
 };
 
-// ------------------------------------------------------- blocked fused kernel
 template <typename T, bool PERCELL>
 __global__ __launch_bounds__(512) void KBlock(TileArgs<T> a) {
-  __shared__ T plane[2][kBoxCells];
+  constexpr int P = Rounds<T>::P;
+  __shared__ T plane[2][P][kBoxCells];
   __shared__ uint8_t kbox[kBoxCells];
-  __shared__ double red_s[512];
-  __shared__ int red_c[512];
+  static_assert(sizeof(plane) >= 512 * (sizeof(double) + sizeof(int)), "stats alias");
+  double* red_s = reinterpret_cast<double*>(&plane[0][0][0]);
+  int* red_c = reinterpret_cast<int*>(red_s + 512);
 
   const int ti = XcdTile(blockIdx.x, a.ntiles);
   const int b = a.tiles[ti].block;
@@ -132,38 +118,26 @@ __global__ __launch_bounds__(512) void KBlock(TileArgs<T> a) {
   const int tid = threadIdx.x;
   const int ix = tid >> 6, iy = (tid >> 3) & 7, iz = tid & 7;
   const size_t bbase = static_cast<size_t>(b) * kQ * ncore;
-  const double t = static_cast<double>(*a.step);
-  const T avg = sc.const_rho ? static_cast<T>(*a.avg_rho) : T(1);
   const BlockTab& bt = a.bt[b];
   const int bflags = bt.flags;
   const int bshell = bt.shell;
-  const double borg[3] = {bt.origin[0], bt.origin[1], bt.origin[2]};
-  const double bcw = bt.cw;
 
+  // ---- issue every global load before any arithmetic: own cell, then the
+  // apron slot (the block's halo; apron index == shell index)
   T f[kQ];
 #pragma unroll
   for (int i = 0; i < kQ; ++i) f[i] = a.src[bbase + static_cast<size_t>(i) * ncore + tid];
   const int k = a.kind[static_cast<size_t>(b) * ncore + tid];
-  double ssum = 0.0;
-  int scnt = 0;
-  CollideCell<T>(f, k, sc, borg, bcw, ix, iy, iz, t, avg, a.want_stats ? &ssum : nullptr, &scnt);
-  const int own_l = ((ix + 1) * kBox + iy + 1) * kBox + iz + 1;
-  kbox[own_l] = static_cast<uint8_t>(k);
-
-  // apron = the block's halo shell; apron index == shell index
   T g[kQ];
-#pragma unroll
-  for (int i = 0; i < kQ; ++i) g[i] = T(0);
   int ap_l = -1;
+  int nb = -1, wx = 0, wy = 0, wz = 0, kk = kBulk;
   bool live = false;
   if (tid < kApron) {
     ap_l = kApronTab.idx[tid];
-    kbox[ap_l] = kBulk;  // kind_at_halo -> kBulk (block_lattice.cpp:510-518)
     const int px = ap_l / (kBox * kBox), py = (ap_l / kBox) % kBox, pz = ap_l % kBox;
     const int x = px - 1, y = py - 1, z = pz - 1;
     const int ox = (x < 0 || x >= n), oy = (y < 0 || y >= n), oz = (z < 0 || z >= n);
     const int nout = ox + oy + oz;
-    int nb = -1;
     if (nout == 1) {
       const int axis = ox ? 0 : (oy ? 1 : 2);
       const int v = axis == 0 ? x : (axis == 1 ? y : z);
@@ -176,25 +150,24 @@ __global__ __launch_bounds__(512) void KBlock(TileArgs<T> a) {
       nb = bt.nbr[EdgeSlot(a1, v1 >= n ? 1 : 0, a2, v2 >= n ? 1 : 0)];
     }
     if (nb >= 0) {
-      const int wx = x < 0 ? n - 1 : (x >= n ? 0 : x);
-      const int wy = y < 0 ? n - 1 : (y >= n ? 0 : y);
-      const int wz = z < 0 ? n - 1 : (z >= n ? 0 : z);
+      wx = x < 0 ? n - 1 : (x >= n ? 0 : x);
+      wy = y < 0 ? n - 1 : (y >= n ? 0 : y);
+      wz = z < 0 ? n - 1 : (z >= n ? 0 : z);
       const int cc = (wx * n + wy) * n + wz;
       const size_t nbase = static_cast<size_t>(nb) * kQ * ncore;
 #pragma unroll
       for (int i = 0; i < kQ; ++i) g[i] = a.src[nbase + static_cast<size_t>(i) * ncore + cc];
-      const int kk = a.kind[static_cast<size_t>(nb) * ncore + cc];
-      const BlockTab& nt = a.bt[nb];
-      const double norg[3] = {nt.origin[0], nt.origin[1], nt.origin[2]};
-      CollideCell<T>(g, kk, sc, norg, nt.cw, wx, wy, wz, t, avg, nullptr, nullptr);
+      kk = a.kind[static_cast<size_t>(nb) * ncore + cc];
     } else if (bshell >= 0) {
       const size_t sbase = static_cast<size_t>(bshell) * kQ * kApron;
 #pragma unroll
       for (int i = 0; i < kQ; ++i) g[i] = a.shell_src[sbase + static_cast<size_t>(i) * kApron + tid];
+    } else {
+#pragma unroll
+      for (int i = 0; i < kQ; ++i) g[i] = T(0);
     }
     live = (bflags & kFlagShellLive) != 0;
   }
-
   uint32_t qm = 0;
   const double* qv = a.qval;
   if (PERCELL && (bflags & kFlagLinks)) {
@@ -202,17 +175,63 @@ __global__ __launch_bounds__(512) void KBlock(TileArgs<T> a) {
     qv = a.qval + a.qoff[static_cast<size_t>(b) * ncore + tid];
   }
 
-  T out[kQ];
-  T sout[kQ];
-  PopLoop<T, PERCELL>::template Run<0>(plane, kbox, own_l, ap_l, k, f, g, out, sout, live, ix, iy,
-                                       iz, n, qm, qv);
+  // ---- collide own cell and apron cell
+  const double t = static_cast<double>(*a.step);
+  const T avg = sc.const_rho ? static_cast<T>(*a.avg_rho) : T(1);
+  double ssum = 0.0;
+  int scnt = 0;
+  {
+    const Origin og{bt.origin[0], bt.origin[1], bt.origin[2], bt.cw};
+    CollideCell<T>(f, k, sc, og, ix, iy, iz, t, avg, a.want_stats != 0, ssum, scnt);
+  }
+  const int own_l = ((ix + 1) * kBox + iy + 1) * kBox + iz + 1;
+  kbox[own_l] = static_cast<uint8_t>(k);
+  if (ap_l >= 0) {
+    kbox[ap_l] = kBulk;  // kind_at_halo -> kBulk (block_lattice.cpp:510-518)
+    if (nb >= 0) {
+      const BlockTab& nt = a.bt[nb];
+      const Origin og{nt.origin[0], nt.origin[1], nt.origin[2], nt.cw};
+      double d0 = 0.0;
+      int d1 = 0;
+      CollideCell<T>(g, kk, sc, og, wx, wy, wz, t, avg, false, d0, d1);
+    }
+  }
 
+  // ---- stream P population planes per LDS round; results go straight out
+  T* dst = a.dst + bbase + tid;
+  T* sdst = (live && bshell >= 0) ? a.shell_dst + static_cast<size_t>(bshell) * kQ * kApron + tid : nullptr;
 #pragma unroll
-  for (int i = 0; i < kQ; ++i) a.dst[bbase + static_cast<size_t>(i) * ncore + tid] = out[i];
-  if (live) {
-    const size_t sbase = static_cast<size_t>(bshell) * kQ * kApron;
+  for (int i0 = 0; i0 < kQ; i0 += P) {
+    T (*pl)[kBoxCells] = plane[(i0 / P) & 1];
 #pragma unroll
-    for (int i = 0; i < kQ; ++i) a.shell_dst[sbase + static_cast<size_t>(i) * kApron + tid] = sout[i];
+    for (int j = 0; j < P; ++j) {
+      if (i0 + j < kQ) {
+        pl[j][own_l] = f[i0 + j];
+        if (ap_l >= 0) pl[j][ap_l] = g[i0 + j];
+      }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < P; ++j) {
+      const int i = i0 + j;
+      if (i < kQ) {
+        const int sx = ix - Cx(i), sy = iy - Cy(i), sz = iz - Cz(i);
+        const bool src_int = sx >= 0 && sx < n && sy >= 0 && sy < n && sz >= 0 && sz < n;
+        T oi = f[i], oo = f[Opp(i)];
+        asm volatile("" : "+v"(oi));
+        asm volatile("" : "+v"(oo));
+        dst[static_cast<size_t>(i) * ncore] = StreamPop<T, PERCELL>(i, pl[j], kbox, own_l, k, oi, oo, src_int, qm, qv);
+        if (sdst) {
+          // padded-block torus pull of a halo slot; in-halo and wrapped
+          // sources are both (d - off) mod NP^3 (block_lattice.cpp:595-611)
+          const int off = (Cx(i) * kBox + Cy(i)) * kBox + Cz(i);
+          int sidx = ap_l - off;
+          if (sidx < 0) sidx += kBoxCells;
+          if (sidx >= kBoxCells) sidx -= kBoxCells;
+          sdst[static_cast<size_t>(i) * kApron] = pl[j][sidx];
+        }
+      }
+    }
   }
   if (bflags & kFlagDrag) {
     // post-collide snapshot of the outgoing populations on wall links
@@ -244,8 +263,9 @@ template <typename T, bool PERCELL>
 __global__ __launch_bounds__(512) void KPadded(TileArgs<T> a) {
   __shared__ T plane[2][kBoxCells];
   __shared__ uint8_t kbox[kBoxCells];  // 0..7 core kinds, 254 padding, 255 outside
-  __shared__ double red_s[512];
-  __shared__ int red_c[512];
+  static_assert(sizeof(plane) >= 512 * (sizeof(double) + sizeof(int)), "stats alias");
+  double* red_s = reinterpret_cast<double*>(&plane[0][0]);
+  int* red_c = reinterpret_cast<int*>(red_s + 512);
 
   const int ti = XcdTile(blockIdx.x, a.ntiles);
   const Tile tl = a.tiles[ti];
@@ -257,8 +277,7 @@ __global__ __launch_bounds__(512) void KPadded(TileArgs<T> a) {
   const double t = static_cast<double>(*a.step);
   const T avg = sc.const_rho ? static_cast<T>(*a.avg_rho) : T(1);
   const BlockTab& bt = a.bt[b];
-  const double borg[3] = {bt.origin[0], bt.origin[1], bt.origin[2]};
-  const double bcw = bt.cw;
+  const Origin borg{bt.origin[0], bt.origin[1], bt.origin[2], bt.cw};
   double ssum = 0.0;
   int scnt = 0;
 
@@ -279,7 +298,7 @@ __global__ __launch_bounds__(512) void KPadded(TileArgs<T> a) {
     for (int i = 0; i < kQ; ++i) f[i] = a.src[bbase + static_cast<size_t>(i) * NS + d];
     if (core) {
       k = a.kind[static_cast<size_t>(b) * sc.ncore + (x * n + y) * n + z];
-      CollideCell<T>(f, k, sc, borg, bcw, x, y, z, t, avg, a.want_stats ? &ssum : nullptr, &scnt);
+      CollideCell<T>(f, k, sc, borg, x, y, z, t, avg, a.want_stats != 0, ssum, scnt);
       kbox[own_l] = static_cast<uint8_t>(k);
     } else {
       kbox[own_l] = 254;
@@ -303,7 +322,9 @@ __global__ __launch_bounds__(512) void KPadded(TileArgs<T> a) {
       const int cx = gx - h, cy = gy - h, cz = gz - h;
       if (cx >= 0 && cx < n && cy >= 0 && cy < n && cz >= 0 && cz < n) {
         const int kk = a.kind[static_cast<size_t>(b) * sc.ncore + (cx * n + cy) * n + cz];
-        CollideCell<T>(g, kk, sc, borg, bcw, cx, cy, cz, t, avg, nullptr, nullptr);
+        double d0 = 0.0;
+        int d1 = 0;
+        CollideCell<T>(g, kk, sc, borg, cx, cy, cz, t, avg, false, d0, d1);
         kbox[ap_l] = static_cast<uint8_t>(kk);
       } else {
         kbox[ap_l] = 254;
@@ -446,7 +467,10 @@ template <typename T>
 struct FdView {
   const T* cur;
   const T* post;
-  const int* smap;  // this block's FD cell -> entry (or -1)
+  const int* smap;      // this block's FD cell -> entry (or -1)
+  const T* lpre;        // LDS copy of the block ([19][512]) or nullptr
+  const T* lpost;       // LDS new values per FD slot ([slot][19])
+  const short* lsmap;   // LDS FD cell -> slot (or -1)
   const uint8_t* kind;
   const SimConst* sc;
   const double* org;
@@ -470,6 +494,17 @@ struct FdView {
   __device__ int Kind(int x, int y, int z) const { return kind[(x * sc->n + y) * sc->n + z]; }
   __device__ void Pops(int x, int y, int z, T* f) const {
     const int c = (x * sc->n + y) * sc->n + z;
+    if (lpre) {
+      const int e = lsmap[c];
+      if (e >= 0 && c < self) {
+#pragma unroll
+        for (int i = 0; i < kQ; ++i) f[i] = lpost[e * kQ + i];
+      } else {
+#pragma unroll
+        for (int i = 0; i < kQ; ++i) f[i] = lpre[i * kTileCells + c];
+      }
+      return;
+    }
     const int e = smap[c];
     if (e >= 0 && c < self) {
 #pragma unroll
@@ -609,6 +644,7 @@ struct FdView {
 template <typename T>
 __device__ inline void WriteFromPi(T* f, T rho, const T* u, const T* pi) {
   const T usqr = u[0] * u[0] + u[1] * u[1] + u[2] * u[2];
+#pragma unroll
   for (int i = 0; i < kQ; ++i) f[i] = Feq(i, rho, u, usqr) + PiToFneq(i, pi);
 }
 
@@ -623,31 +659,15 @@ __device__ inline void PiFromGrad(T rho, T omega, const T* dx, const T* dy, cons
   pi[4] = (dy[2] + dz[1]) * s;
 }
 
-template <typename T>
-__global__ void KFd(FdArgs<T> a, int begin, int count) {
-  const int e = begin + static_cast<int>(blockIdx.x * blockDim.x + threadIdx.x);
-  if (e >= begin + count) return;
-  const int b = a.cells[e].x, c = a.cells[e].y;
-  const SimConst& sc = a.sc;
+// One FD boundary cell (ApplyInterpolatedVelocityBoundaryCells body).
+template <typename T, typename V>
+__device__ inline void FdCell(const V& v, int ix, int iy, int iz, T* fnew) {
+  const SimConst& sc = *v.sc;
   const int n = sc.n;
-  const int ix = c / (n * n), iy = (c / n) % n, iz = c % n;
-  const int lvl = a.level[b];
-  const T* cur = (a.par.v[lvl] ? a.pops1 : a.pops0) + static_cast<size_t>(b) * kQ * sc.plane;
-  FdView<T> v;
-  v.cur = cur;
-  v.post = a.post;
-  v.smap = a.smap + static_cast<size_t>(a.fd_index[b]) * sc.ncore;
-  v.kind = a.kind + static_cast<size_t>(b) * sc.ncore;
-  v.sc = &a.sc;
-  v.org = a.bt[b].origin;
-  v.cw = a.bt[b].cw;
-  v.t = static_cast<double>(*a.step);
-  v.self = c;
   const T omega = static_cast<T>(sc.omega);
   const int xm = ix == 0, xp = ix == n - 1, ym = iy == 0, yp = iy == n - 1, zm = iz == 0, zp = iz == n - 1;
   const int fc = xm + xp + ym + yp + zm + zp;
   const int xn = xm ? -1 : (xp ? 1 : 0), yn = ym ? -1 : (yp ? 1 : 0), zn = zm ? -1 : (zp ? 1 : 0);
-  T fnew[kQ];
   T pi[6];
   if (fc == 3) {
     // ApplyCornerBoundary (:638-685)
@@ -741,8 +761,211 @@ __global__ void KFd(FdArgs<T> a, int begin, int count) {
     PiFromGrad(rho, omega, dx, dy, dz, pi);
     WriteFromPi(fnew, rho, u, pi);
   }
+}
+
+template <typename T>
+__global__ void KFd(FdArgs<T> a, int begin, int count) {
+  const int e = begin + static_cast<int>(blockIdx.x * blockDim.x + threadIdx.x);
+  if (e >= begin + count) return;
+  const int b = a.cells[e].x, c = a.cells[e].y;
+  const SimConst& sc = a.sc;
+  const int n = sc.n;
+  const int ix = c / (n * n), iy = (c / n) % n, iz = c % n;
+  const int lvl = a.level[b];
+  const T* cur = (a.par.v[lvl] ? a.pops1 : a.pops0) + static_cast<size_t>(b) * kQ * sc.plane;
+  FdView<T> v;
+  v.cur = cur;
+  v.post = a.post;
+  v.smap = a.smap + static_cast<size_t>(a.fd_index[b]) * sc.ncore;
+  v.lpre = nullptr;
+  v.lpost = nullptr;
+  v.lsmap = nullptr;
+  v.kind = a.kind + static_cast<size_t>(b) * sc.ncore;
+  v.sc = &a.sc;
+  v.org = a.bt[b].origin;
+  v.cw = a.bt[b].cw;
+  v.t = static_cast<double>(*a.step);
+  v.self = c;
+  T fnew[kQ];
+  FdCell<T>(v, ix, iy, iz, fnew);
   T* po = a.post + static_cast<size_t>(e) * kQ;
   for (int i = 0; i < kQ; ++i) po[i] = fnew[i];
+}
+
+// Blocked mode: one workgroup per boundary block.  Every cell's density and
+// velocity as the FD stencils see them (BoundaryLatticeView::ComputeRho /
+// ComputeU, interpolated_velocity.h:352-434) are computed once into LDS, the
+// pressure-extrapolation chains are resolved once, and the plane / edge /
+// corner phases read only those arrays.  A phase stores the density of each
+// cell it rewrote, which later phases read for cells that precede them in
+// (ix,iy,iz) order.
+template <typename T>
+struct FdLds {
+  const T* rho_pre;
+  const T* rho_post;
+  const T* ux;
+  const T* uy;
+  const T* uz;
+  const short* slot;
+  const SimConst* sc;
+  int self;
+
+  __device__ bool InDomain(int x, int y, int z) const {
+    const int n = sc->n;
+    return x >= 0 && x < n && y >= 0 && y < n && z >= 0 && z < n;
+  }
+  __device__ bool Padding(int x, int y, int z) const {
+    const int n = sc->n, st = sc->h < 2 ? sc->h : 2;
+    return !InDomain(x, y, z) && x >= -st && x < n + st && y >= -st && y < n + st && z >= -st &&
+           z < n + st;
+  }
+  __device__ static int Mirror(int i, int n) { return i < 0 ? -i : (i >= n ? 2 * (n - 1) - i : i); }
+  __device__ void ComputeU(int x, int y, int z, T* u) const {
+    const int n = sc->n;
+    if (Padding(x, y, z)) {
+      u[0] = u[1] = u[2] = T(0);
+      return;
+    }
+    const bool fx = x < 0 || x >= n, fy = y < 0 || y >= n, fz = z < 0 || z >= n;
+    const int c = (Mirror(x, n) * n + Mirror(y, n)) * n + Mirror(z, n);
+    u[0] = fx ? -ux[c] : ux[c];
+    u[1] = fy ? -uy[c] : uy[c];
+    u[2] = fz ? -uz[c] : uz[c];
+  }
+  __device__ T ComputeRho(int x, int y, int z) const {
+    const int n = sc->n;
+    if (Padding(x, y, z)) return T(1);
+    const int c = (Mirror(x, n) * n + Mirror(y, n)) * n + Mirror(z, n);
+    return (slot[c] >= 0 && c < self) ? rho_post[c] : rho_pre[c];
+  }
+  __device__ void ComputeRhoU(int x, int y, int z, T& rho, T* u) const {
+    const int c = (x * sc->n + y) * sc->n + z;
+    rho = ComputeRho(x, y, z);
+    u[0] = ux[c];
+    u[1] = uy[c];
+    u[2] = uz[c];
+  }
+  __device__ void NormalGrad(int x, int y, int z, int axis, int normal, T* g) const {
+    const int ox = axis == 0 ? -normal : 0, oy = axis == 1 ? -normal : 0, oz = axis == 2 ? -normal : 0;
+    T u0[3], u1[3], u2[3];
+    ComputeU(x, y, z, u0);
+    ComputeU(x + ox, y + oy, z + oz, u1);
+    ComputeU(x + 2 * ox, y + 2 * oy, z + 2 * oz, u2);
+    for (int d = 0; d < 3; ++d)
+      g[d] = T(-normal) * ((-T(3) * u0[d] + T(4) * u1[d] - T(1) * u2[d]) / T(2));
+  }
+  __device__ void TangGrad(int x, int y, int z, int tx, int ty, int tz, T* g) const {
+    T up[3], um[3];
+    ComputeU(x + tx, y + ty, z + tz, up);
+    ComputeU(x - tx, y - ty, z - tz, um);
+    for (int d = 0; d < 3; ++d) g[d] = (up[d] - um[d]) / T(2);
+  }
+};
+
+// ComputeRho of an in-domain cell from its populations (:403-434).
+template <typename T>
+__device__ inline T FdRhoOf(int k, const T* f, const T* pu, const SimConst& sc, int ix, int iy, int iz) {
+  int dir, ori;
+  const bool flat = FlatInfo(sc.n, ix, iy, iz, dir, ori);
+  if (k == kPressure) return flat ? static_cast<T>(sc.spec[dir * 2 + (ori < 0 ? 0 : 1)].rho_target) : T(1);
+  if (k == kVelocity && flat) return VelRhoFromPop(dir, ori, f, pu);
+  T rho, u[3];
+  ProxyRhoU(f, rho, u);
+  return rho;
+}
+
+template <typename T>
+__global__ __launch_bounds__(512) void KFdBlock(FdArgs<T> a, const int* fd_blocks, const short* fd_slot) {
+  __shared__ T rho_pre[kTileCells], rho_post[kTileCells];
+  __shared__ T ux[kTileCells], uy[kTileCells], uz[kTileCells];
+  __shared__ short slot[kTileCells];
+  __shared__ uint8_t kind[kTileCells];
+  const int fb = blockIdx.x;
+  const int b = fd_blocks[fb];
+  const int tid = threadIdx.x;
+  const SimConst& sc = a.sc;
+  T* cur = (a.par.v[a.level[b]] ? a.pops1 : a.pops0) + static_cast<size_t>(b) * kQ * kTileCells;
+  const int ix = tid >> 6, iy = (tid >> 3) & 7, iz = tid & 7;
+  const int k = a.kind[static_cast<size_t>(b) * kTileCells + tid];
+  const int sl = fd_slot[static_cast<size_t>(fb) * kTileCells + tid];
+  kind[tid] = static_cast<uint8_t>(k);
+  slot[tid] = static_cast<short>(sl);
+  const double t = static_cast<double>(*a.step);
+  const Origin og{a.bt[b].origin[0], a.bt[b].origin[1], a.bt[b].origin[2], a.bt[b].cw};
+  T f[kQ];
+#pragma unroll
+  for (int i = 0; i < kQ; ++i) f[i] = cur[static_cast<size_t>(i) * kTileCells + tid];
+  T pu[3] = {T(0), T(0), T(0)};
+  if (k == kVelocity) {
+    const double org[3] = {og.x, og.y, og.z};
+    double ud[3];
+    PrescribedU(sc, org, og.cw, ix, iy, iz, t, ud);
+    pu[0] = static_cast<T>(ud[0]);
+    pu[1] = static_cast<T>(ud[1]);
+    pu[2] = static_cast<T>(ud[2]);
+  }
+  rho_pre[tid] = FdRhoOf(k, f, pu, sc, ix, iy, iz);
+  if (k == kVelocity) {
+    ux[tid] = pu[0];
+    uy[tid] = pu[1];
+    uz[tid] = pu[2];
+  } else {
+    T rho, u[3];
+    ProxyRhoU(f, rho, u);
+    ux[tid] = u[0];
+    uy[tid] = u[1];
+    uz[tid] = u[2];
+  }
+  __syncthreads();
+  // ExtrapolatedU (:297-316): a pressure cell takes the velocity of the first
+  // non-pressure cell inward along its face normal; only pressure entries are
+  // rewritten, and walks only read non-pressure entries.
+  T pux = T(0), puy = T(0), puz = T(0);
+  if (k == kPressure) {
+    int x = ix, y = iy, z = iz;
+    for (;;) {
+      const int kk = kind[(x * kTile + y) * kTile + z];
+      if (kk != kPressure) {
+        const int c = (x * kTile + y) * kTile + z;
+        pux = ux[c];
+        puy = uy[c];
+        puz = uz[c];
+        break;
+      }
+      int dir, ori;
+      if (!FlatInfo(kTile, x, y, z, dir, ori)) break;
+      if (dir == 0) {
+        x -= ori;
+      } else if (dir == 1) {
+        y -= ori;
+      } else {
+        z -= ori;
+      }
+      if (x < 0 || x >= kTile || y < 0 || y >= kTile || z < 0 || z >= kTile) break;
+    }
+  }
+  __syncthreads();
+  if (k == kPressure) {
+    ux[tid] = pux;
+    uy[tid] = puy;
+    uz[tid] = puz;
+  }
+  __syncthreads();
+  FdLds<T> v{rho_pre, rho_post, ux, uy, uz, slot, &a.sc, tid};
+  const int fc = (ix == 0) + (ix == kTile - 1) + (iy == 0) + (iy == kTile - 1) + (iz == 0) + (iz == kTile - 1);
+  const int phase = fc == 3 ? 2 : (fc == 2 ? 1 : 0);
+  T fnew[kQ];
+  for (int ph = 0; ph < 3; ++ph) {
+    if (sl >= 0 && phase == ph) {
+      FdCell<T>(v, ix, iy, iz, fnew);
+      rho_post[tid] = FdRhoOf(k, fnew, pu, sc, ix, iy, iz);
+    }
+    __syncthreads();
+  }
+  if (sl >= 0) {
+#pragma unroll
+    for (int i = 0; i < kQ; ++i) cur[static_cast<size_t>(i) * kTileCells + tid] = fnew[i];
+  }
 }
 
 template <typename T>
@@ -984,6 +1207,12 @@ void LaunchInitField(T* pops, int nb, const SimConst& sc, const double* rho, con
 }
 
 template <typename T>
+void LaunchFdBlocks(const FdArgs<T>& a, const int* fd_blocks, const short* fd_slot, int nfd, hipStream_t s) {
+  if (nfd <= 0) return;
+  hipLaunchKernelGGL(KFdBlock<T>, dim3(nfd), dim3(kTileCells), 0, s, a, fd_blocks, fd_slot);
+}
+
+template <typename T>
 void LaunchFd(const FdArgs<T>& a, const int* phase_begin, hipStream_t s) {
   for (int ph = 0; ph < 3; ++ph) {
     const int cnt = phase_begin[ph + 1] - phase_begin[ph];
@@ -1033,6 +1262,7 @@ void LaunchMass(const T* p0, const T* p1, const ParArray& par, const BlockTab* b
   template void LaunchInitField<T>(T*, int, const SimConst&, const double*, const double*,        \
                                    hipStream_t);                                                  \
   template void LaunchFd<T>(const FdArgs<T>&, const int*, hipStream_t);                           \
+  template void LaunchFdBlocks<T>(const FdArgs<T>&, const int*, const short*, int, hipStream_t);  \
   template void LaunchCouplerInit<T>(const CoupleArgs<T>&, int, hipStream_t);                     \
   template void LaunchProlong<T>(const CoupleArgs<T>&, const int*, int, int, hipStream_t);        \
   template void LaunchRestrict<T>(const CoupleArgs<T>&, const int*, int, hipStream_t);            \

@@ -266,18 +266,23 @@ __device__ inline bool FlatInfo(int n, int ix, int iy, int iz, int& dir, int& or
 // BcDispatcher::collide (bc_dispatcher.h:32-81) for the concrete handler;
 // BlockLattice::collide / collide_const_rho (block_lattice.cpp:376-499) for
 // the NoOp handler.  Statistics follow CollideRhoStats (block_lattice.h:22-32).
+struct Origin {
+  double x, y, z, cw;
+};
+
 template <typename T>
-__device__ inline void CollideCell(T* f, int k, const SimConst& sc, const double* org, double cw,
-                                   int ix, int iy, int iz, double t, T avg, double* ssum,
-                                   int* scnt) {
+__device__ __forceinline__ void CollideCell(T* f, int k, const SimConst& sc, const Origin& og, int ix, int iy,
+                                   int iz, double t, T avg, bool want, double& ssum, int& scnt) {
+  const double org[3] = {og.x, og.y, og.z};
+  const double cw = og.cw;
   const T omega = static_cast<T>(sc.omega);
   if (k == kBulk) {
     if (sc.const_rho) {
       T rho, u[3];
       ProxyRhoU(f, rho, u);
-      if (ssum) {
-        *ssum += static_cast<double>(rho + (T(1) - avg));
-        *scnt += 1;
+      if (want) {
+        ssum += static_cast<double>(rho + (T(1) - avg));
+        scnt += 1;
       }
       ConstRhoBgk(f, omega, avg, rho, u);
     } else if (sc.handler == OCTLB_HANDLER_CONCRETE) {
@@ -315,9 +320,9 @@ __device__ inline void CollideCell(T* f, int k, const SimConst& sc, const double
       rho = VelRhoFromPop(dir, ori, f, u);
     }
     OlbBgk(f, rho, u, omega);
-    if (ssum) {
-      *ssum += static_cast<double>(rho);
-      *scnt += 1;
+    if (want) {
+      ssum += static_cast<double>(rho);
+      scnt += 1;
     }
   }
 }
@@ -337,6 +342,8 @@ void LaunchInitField(T* pops, int nb, const SimConst& sc, const double* rho, con
                      hipStream_t s);
 template <typename T>
 void LaunchFd(const FdArgs<T>& a, const int* phase_begin, hipStream_t s);
+template <typename T>
+void LaunchFdBlocks(const FdArgs<T>& a, const int* fd_blocks, const short* fd_slot, int nfd, hipStream_t s);
 template <typename T>
 void LaunchCouplerInit(const CoupleArgs<T>& a, int npts, hipStream_t s);
 template <typename T>
