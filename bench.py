@@ -46,6 +46,20 @@ def build_case(name):
     raise SystemExit(f"unknown workload {name}")
 
 
+def traffic_bytes(workload, precision):
+    """HBM bytes per launch of the dominant kernel, from the committed
+    rocprofv3 FETCH_SIZE / WRITE_SIZE passes (profiles/kblock_traffic.json,
+    which records the commands and the copy-kernel scale factors), or None."""
+    path = ROOT / "profiles" / "kblock_traffic.json"
+    if not path.exists():
+        return None
+    rec = json.loads(path.read_text()).get(f"{workload}/fp{precision}")
+    if not rec:
+        return None
+    kib = rec["fetch_kib"] * rec["fetch_scale"] + rec["write_kib"] * rec["write_scale"]
+    return round(kib * 1024.0)
+
+
 def run_gpu(case, precision, steps, warmup, timing_steps):
     from octlb_amd.solver import GpuSolver
     from oracle.binding import setup  # applies kinds / links / init (shared with tests)
@@ -59,22 +73,20 @@ def run_gpu(case, precision, steps, warmup, timing_steps):
     sim.advance_graph(steps)
     sim.synchronize()
     dt = time.perf_counter() - t0
-    # per-launch duration of the fused kernel: eager launches, HIP events on
-    # the solver's stream
-    sim.enable_timing(True)
-    sim.advance(timing_steps)
-    ms, launches, updates = sim.kernel_timing()
-    sim.enable_timing(False)
     drag = sim.drag(case.drag_bbox) if case.drag_bbox is not None else np.zeros(3)
     mass = sim.mass()
+    # per-launch duration of the fused kernel of every level: back-to-back
+    # replays between HIP events on the solver's stream (state unchanged)
+    levels = {}
+    for lv in sorted(set(int(x) for x in case.topo.level)):
+        ms, cells = sim.time_kernel(lv, timing_steps)
+        levels[lv] = {"avg_launch_us": 1e3 * ms, "cells_per_launch": cells}
     res = {
         "seconds": dt,
         "ms_per_step": 1e3 * dt / steps,
         "mlups_feq": feq * steps / dt / 1e6,
         "mlups_actual": actual * steps / dt / 1e6,
-        "kernel_ms": ms,
-        "kernel_launches": launches,
-        "kernel_updates": updates,
+        "kernel_levels": levels,
         "drag": drag.tolist(),
         "mass": mass,
         "actual_per_step": actual,
@@ -115,10 +127,14 @@ def main():
     ap.add_argument("--steps", type=int, default=400)
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--workload", default="cylinder")
-    ap.add_argument("--timing-steps", type=int, default=50)
+    ap.add_argument("--timing-steps", type=int, default=50,
+                    help="back-to-back replays of each level's fused kernel for its launch time")
     ap.add_argument("--cpu-budget", type=float, default=12.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--precision", default="both", choices=["both", "32", "64"])
+    ap.add_argument("--calibrate-copy-mb", type=int, default=0,
+                    help="run one device-to-device copy of this many MiB first (profiler "
+                         "counter scale: a plain copy reads and writes exactly that many bytes)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; multi-GPU sharding is a design "
@@ -126,6 +142,17 @@ def main():
                           "n_gpus": args.gpus}))
         sys.exit(2)
 
+    if args.calibrate_copy_mb:
+        import torch
+
+        nbytes = args.calibrate_copy_mb << 20
+        src = torch.empty(nbytes, dtype=torch.uint8, device="cuda").fill_(1)
+        dst = torch.empty_like(src)
+        torch.cuda.synchronize()
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        log(f"calibration copy: {nbytes} bytes")
+        del src, dst
     log("building case")
     case = build_case(args.workload)
     res = {}
@@ -136,8 +163,12 @@ def main():
         log(f"  fp{p}: {res[p]['mlups_feq']:.1f} MLUPS feq, {res[p]['ms_per_step']:.4f} ms/step")
     head = 32 if 32 in res else precs[0]
     r = res[head]
-    kb = BYTES_PER_UPDATE[head] * r["kernel_updates"]
-    achieved = kb / (r["kernel_ms"] * 1e-3) / 1e9 if r["kernel_ms"] > 0 else 0.0
+    # dominant kernel: the fused collide+stream launch of the level with the
+    # most cells (the finest level of this workload)
+    dom_level = max(r["kernel_levels"], key=lambda lv: r["kernel_levels"][lv]["cells_per_launch"])
+    dom = r["kernel_levels"][dom_level]
+    kb = BYTES_PER_UPDATE[head] * dom["cells_per_launch"]
+    achieved = kb / (dom["avg_launch_us"] * 1e-6) / 1e9
     cpu = None
     if not args.no_cpu:
         log("cpu baseline")
@@ -178,11 +209,13 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4),
-            "traffic": None,
-            "kernel": "fused collide+stream (KBlock)",
-            "launches": r["kernel_launches"],
-            "avg_launch_us": round(1e3 * r["kernel_ms"] / max(r["kernel_launches"], 1), 3),
+            "traffic": traffic_bytes(args.workload, head),
+            "kernel": f"fused collide+stream (KBlock), level {dom_level}",
+            "cells_per_launch": dom["cells_per_launch"],
+            "avg_launch_us": round(dom["avg_launch_us"], 3),
             "bytes_per_update": BYTES_PER_UPDATE[head],
+            "per_level_avg_launch_us": {str(k): round(v["avg_launch_us"], 3)
+                                        for k, v in r["kernel_levels"].items()},
         },
         "cpu_baseline": cpu,
         "drag_fp32": res[32]["drag"] if 32 in res else None,

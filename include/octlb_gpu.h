@@ -229,6 +229,14 @@ int octlb_dev_update_counts(octlb_dev* dev, double* actual, double* fine_equival
 int octlb_dev_kernel_timing(octlb_dev* dev, double* ms, int64_t* launches,
                             double* cell_updates);
 int octlb_dev_enable_timing(octlb_dev* dev, int32_t on);
+/* Mean duration of the fused collide+stream kernel of one level, measured by
+ * replaying it reps times back to back between two HIP events on the
+ * handle's stream.  The replays read the current state and write only the
+ * idle ping-pong buffer (no snapshot, no ConstRho statistics), so the
+ * simulation state is unchanged.  cell_updates = cells updated per launch.
+ * (No reference counterpart: the measurement hook of the roofline figure.) */
+int octlb_dev_time_kernel(octlb_dev* dev, int32_t level, int32_t reps, double* avg_ms,
+                          double* cell_updates);
 
 #ifdef __cplusplus
 }

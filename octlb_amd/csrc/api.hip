@@ -55,6 +55,30 @@ std::vector<int> ShellSlots() {
   return s;
 }
 
+// Block that supplies halo cell (x,y,z) (core coordinates, one of them
+// outside [0,8)) through a face or edge exchange, or -1; *cc = its cell.
+int ApronNeighbour(const BlockTab& r, int x, int y, int z, int* cc) {
+  constexpr int n = kTile;
+  const int ox = (x < 0 || x >= n), oy = (y < 0 || y >= n), oz = (z < 0 || z >= n);
+  const int nout = ox + oy + oz;
+  int nbk = -1;
+  if (nout == 1) {
+    const int axis = ox ? 0 : (oy ? 1 : 2);
+    const int v = axis == 0 ? x : (axis == 1 ? y : z);
+    nbk = r.nbr[axis * 2 + (v >= n ? 1 : 0)];
+  } else if (nout == 2) {
+    const int a1 = ox ? 0 : 1, a2 = oz ? 2 : 1;
+    const int v1 = a1 == 0 ? x : y, v2 = a2 == 2 ? z : y;
+    const int pair = (a1 == 0) ? (a2 == 1 ? 0 : 1) : 2;
+    nbk = r.nbr[6 + pair * 4 + (v1 >= n ? 1 : 0) * 2 + (v2 >= n ? 1 : 0)];
+  }
+  const int wx = x < 0 ? n - 1 : (x >= n ? 0 : x);
+  const int wy = y < 0 ? n - 1 : (y >= n ? 0 : y);
+  const int wz = z < 0 ? n - 1 : (z >= n ? 0 : z);
+  *cc = (wx * n + wy) * n + wz;
+  return nbk;
+}
+
 template <typename P>
 P* DevAlloc(size_t count) {
   if (count == 0) count = 1;
@@ -222,6 +246,10 @@ struct octlb_dev {
 
   void Build(const octlb_topology* t, const octlb_params* p);
   void Prepare();
+  template <typename T>
+  TileArgs<T> MakeTileArgs(int l);
+  template <typename T>
+  double TimeKernel(int l, int reps);
   template <typename T>
   void Level(int l);
   template <typename T>
@@ -688,35 +716,66 @@ void octlb_dev::Prepare() {
 }
 
 template <typename T>
+TileArgs<T> octlb_dev::MakeTileArgs(int l) {
+  TileArgs<T> a{};
+  a.tiles = d_tiles + tile_begin[l];
+  a.ntiles = tile_count[l];
+  a.bt = d_bt;
+  a.kind = d_kind;
+  a.src = static_cast<const T*>(pops[par[l]]);
+  a.dst = static_cast<T*>(pops[par[l] ^ 1]);
+  a.shell_src = static_cast<const T*>(shells[par[l]]);
+  a.shell_dst = static_cast<T*>(shells[par[l] ^ 1]);
+  a.qmask = d_qmask;
+  a.qoff = d_qoff;
+  a.qval = d_qval;
+  a.dmask = d_dmask;
+  a.doff = d_doff;
+  a.snap = static_cast<T*>(d_snap);
+  a.stats = d_stats;
+  a.stat_val = d_stat_val;
+  a.stat_use = d_stat_use;
+  {
+    const int tpa = (NP + kTile - 1) / kTile;
+    a.tiles_per_block = padded ? tpa * tpa * tpa : 1;
+  }
+  a.avg_rho = d_avg;
+  a.step = d_step;
+  a.want_stats = (sc.const_rho && l >= max_level) ? (fast_stats ? 1 : 2) : 0;
+  a.snap_on = 1;
+  a.sc = sc;
+  return a;
+}
+
+// Replays the fused collide+stream kernel of level l `reps` times on the
+// current state: the input buffer is only read, the output goes to the idle
+// buffer (overwritten by the next step anyway), and the drag snapshot and
+// ConstRho statistics are left alone.  Returns the mean launch duration (ms).
+template <typename T>
+double octlb_dev::TimeKernel(int l, int reps) {
+  TileArgs<T> a = MakeTileArgs<T>(l);
+  a.want_stats = 0;
+  a.snap_on = 0;
+  hipEvent_t e0, e1;
+  HIP_CHECK(hipEventCreate(&e0));
+  HIP_CHECK(hipEventCreate(&e1));
+  LaunchTile<T>(a, stream);  // warm the code object and caches
+  HIP_CHECK(hipEventRecord(e0, stream));
+  for (int r = 0; r < reps; ++r) LaunchTile<T>(a, stream);
+  HIP_CHECK(hipEventRecord(e1, stream));
+  HIP_CHECK(hipEventSynchronize(e1));
+  float ms = 0.0f;
+  HIP_CHECK(hipEventElapsedTime(&ms, e0, e1));
+  HIP_CHECK(hipEventDestroy(e0));
+  HIP_CHECK(hipEventDestroy(e1));
+  return static_cast<double>(ms) / reps;
+}
+
+template <typename T>
 void octlb_dev::Level(int l) {
   const int cnt = tile_count[l];
   if (cnt > 0) {
-    TileArgs<T> a{};
-    a.tiles = d_tiles + tile_begin[l];
-    a.ntiles = cnt;
-    a.bt = d_bt;
-    a.kind = d_kind;
-    a.src = static_cast<const T*>(pops[par[l]]);
-    a.dst = static_cast<T*>(pops[par[l] ^ 1]);
-    a.shell_src = static_cast<const T*>(shells[par[l]]);
-    a.shell_dst = static_cast<T*>(shells[par[l] ^ 1]);
-    a.qmask = d_qmask;
-    a.qoff = d_qoff;
-    a.qval = d_qval;
-    a.dmask = d_dmask;
-    a.doff = d_doff;
-    a.snap = static_cast<T*>(d_snap);
-    a.stats = d_stats;
-    a.stat_val = d_stat_val;
-    a.stat_use = d_stat_use;
-    {
-      const int tpa = (NP + kTile - 1) / kTile;
-      a.tiles_per_block = padded ? tpa * tpa * tpa : 1;
-    }
-    a.avg_rho = d_avg;
-    a.step = d_step;
-    a.want_stats = (sc.const_rho && l >= max_level) ? (fast_stats ? 1 : 2) : 0;
-    a.sc = sc;
+    TileArgs<T> a = MakeTileArgs<T>(l);
     const bool timed = timing && !capturing;
     if (timed) {
       while (ev.size() < ev_used + 2) {
@@ -859,14 +918,31 @@ void ToHostBlock(octlb_dev* d, int b, double* aos) {
     const size_t s = static_cast<size_t>(((x + 1) * kBox + y + 1) * kBox + z + 1);
     for (int i = 0; i < kQ; ++i) aos[s * kQ + i] = static_cast<double>(tmp[static_cast<size_t>(i) * d->ncore + c]);
   }
+  const std::vector<int> slots = ShellSlots();
   const int sh = d->bt[b].shell;
   if (sh >= 0) {
     std::vector<T> st(static_cast<size_t>(kQ) * kApron);
     HIP_CHECK(hipMemcpy(st.data(), static_cast<char*>(d->shells[buf]) + static_cast<size_t>(sh) * kQ * kApron * sizeof(T),
                         st.size() * sizeof(T), hipMemcpyDeviceToHost));
-    const std::vector<int> slots = ShellSlots();
     for (int j = 0; j < kApron; ++j)
       for (int i = 0; i < kQ; ++i) aos[static_cast<size_t>(slots[j]) * kQ + i] = static_cast<double>(st[static_cast<size_t>(i) * kApron + j]);
+  }
+  // Exchanged halo slots: the values the next ghost exchange writes there
+  // (the neighbour's boundary layer).  The step never reads these slots
+  // from the shell, so they are not kept on the device.
+  std::vector<T> nbuf(static_cast<size_t>(kQ) * d->ncore);
+  int loaded = -1;
+  for (int j = 0; j < kApron; ++j) {
+    const int s = slots[j];
+    const int x = s / (kBox * kBox) - 1, y = (s / kBox) % kBox - 1, z = s % kBox - 1;
+    int cc = 0;
+    const int nbk = ApronNeighbour(d->bt[b], x, y, z, &cc);
+    if (nbk < 0) continue;
+    if (nbk != loaded) {
+      HIP_CHECK(hipMemcpy(nbuf.data(), d->Core(d->par[d->level[nbk]], nbk), nbuf.size() * sizeof(T), hipMemcpyDeviceToHost));
+      loaded = nbk;
+    }
+    for (int i = 0; i < kQ; ++i) aos[static_cast<size_t>(s) * kQ + i] = static_cast<double>(nbuf[static_cast<size_t>(i) * d->ncore + cc]);
   }
 }
 
@@ -1210,6 +1286,24 @@ int octlb_dev_kernel_timing(octlb_dev* d, double* ms, int64_t* launches, double*
     if (ms) *ms = d->tile_ms;
     if (launches) *launches = d->tile_launches;
     if (updates) *updates = d->tile_updates;
+  });
+}
+
+int octlb_dev_time_kernel(octlb_dev* d, int32_t level, int32_t reps, double* avg_ms,
+                          double* cell_updates) {
+  return Guard([&] {
+    CheckDev(d);
+    if (reps <= 0) throw ArgError("reps must be positive");
+    if (level < 0 || level > d->max_level || d->tile_count[level] == 0) throw ArgError("level has no blocks");
+    EnsureReady(d);
+    d->Sync();
+    const double ms = d->prec == 64 ? d->TimeKernel<double>(level, reps) : d->TimeKernel<float>(level, reps);
+    if (avg_ms) *avg_ms = ms;
+    if (cell_updates) {
+      const int tpa = (d->NP + kTile - 1) / kTile;
+      const int tpb = d->padded ? tpa * tpa * tpa : 1;
+      *cell_updates = static_cast<double>(d->tile_count[level] / tpb) * static_cast<double>(d->ncore);
+    }
   });
 }
 

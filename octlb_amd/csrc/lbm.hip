@@ -199,7 +199,9 @@ __global__ __launch_bounds__(512) void KBlock(TileArgs<T> a) {
 
   // ---- stream P population planes per LDS round; results go straight out
   T* dst = a.dst + bbase + tid;
-  T* sdst = (live && bshell >= 0) ? a.shell_dst + static_cast<size_t>(bshell) * kQ * kApron + tid : nullptr;
+  // only slots that no exchange refills keep their torus-streamed value
+  // (GhostSchedule::exchange overwrites the others before they are read)
+  T* sdst = (live && bshell >= 0 && nb < 0) ? a.shell_dst + static_cast<size_t>(bshell) * kQ * kApron + tid : nullptr;
 #pragma unroll
   for (int i0 = 0; i0 < kQ; i0 += P) {
     T (*pl)[kBoxCells] = plane[(i0 / P) & 1];
@@ -233,7 +235,7 @@ __global__ __launch_bounds__(512) void KBlock(TileArgs<T> a) {
       }
     }
   }
-  if (bflags & kFlagDrag) {
+  if ((bflags & kFlagDrag) && a.snap_on) {
     // post-collide snapshot of the outgoing populations on wall links
     // (take_post_collide_snapshot, block_lattice.cpp:309-316)
     const uint32_t dm = a.dmask[static_cast<size_t>(b) * ncore + tid];
@@ -375,7 +377,7 @@ __global__ __launch_bounds__(512) void KPadded(TileArgs<T> a) {
   if (active) {
 #pragma unroll
     for (int i = 0; i < kQ; ++i) a.dst[bbase + static_cast<size_t>(i) * NS + d] = out[i];
-    if (core && (bt.flags & kFlagDrag)) {
+    if (core && (bt.flags & kFlagDrag) && a.snap_on) {
       const int c = (x * n + y) * n + z;
       const uint32_t dm = a.dmask[static_cast<size_t>(b) * sc.ncore + c];
       if (dm) {

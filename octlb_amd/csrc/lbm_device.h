@@ -106,6 +106,7 @@ struct TileArgs {
   const double* avg_rho;  // previous step's ConstRho average
   const int* step;        // coarse-step counter (time of the inlet ramps)
   int want_stats;         // 0 none, 1 parallel partial sums, 2 reference order
+  int snap_on;            // write the post-collide drag snapshot
   SimConst sc;
 };
 

@@ -114,6 +114,14 @@ class GpuSolver:
         abi.check(self._lib.octlb_dev_update_counts(self._h, abi.f64p(a), abi.f64p(f)), "counts")
         return float(a[0]), float(f[0])
 
+    def time_kernel(self, level, reps=50):
+        """Mean fused-kernel launch duration (ms) of one level and the cells
+        one launch updates; leaves the simulation state unchanged."""
+        ms, cells = C.c_double(), C.c_double()
+        abi.check(self._lib.octlb_dev_time_kernel(self._h, int(level), int(reps), C.byref(ms),
+                                                  C.byref(cells)), "time_kernel")
+        return ms.value, cells.value
+
     def enable_timing(self, on=True):
         abi.check(self._lib.octlb_dev_enable_timing(self._h, 1 if on else 0), "timing")
 

@@ -58,6 +58,9 @@ def lib():
             "ol_block_collide": ([vp, C.c_int32, C.c_double], C.c_int),
             "ol_block_stream": ([vp, C.c_int32], C.c_int),
             "ol_block_post_stream": ([vp, C.c_int32, C.c_double], C.c_int),
+            "ol_coupler_init": ([vp], C.c_int),
+            "ol_apply_prolongation": ([vp, C.c_int32, C.c_int32], C.c_int),
+            "ol_apply_restriction": ([vp, C.c_int32], C.c_int),
         }
         for name, (a, r) in sig.items():
             fn = getattr(h, name)
@@ -177,6 +180,15 @@ class OracleSim:
 
     def block_post_stream(self, b, t=0.0):
         _ok(lib().ol_block_post_stream(self._h, int(b), float(t)), "post_stream")
+
+    def coupler_init(self):
+        _ok(lib().ol_coupler_init(self._h), "coupler_init")
+
+    def apply_prolongation(self, level, half_time):
+        _ok(lib().ol_apply_prolongation(self._h, int(level), 1 if half_time else 0), "prolongation")
+
+    def apply_restriction(self, fine_level):
+        _ok(lib().ol_apply_restriction(self._h, int(fine_level)), "restriction")
 
 
 def setup(sim, case, precision_note=None):

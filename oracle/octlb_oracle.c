@@ -1538,3 +1538,21 @@ int ol_block_post_stream(ol_sim* s, int32_t b, double t) {
   post_stream_block(s, b, t);
   return 0;
 }
+
+int ol_coupler_init(ol_sim* s) {
+  if (!s) return fail("null sim");
+  coupler_init(s);
+  return 0;
+}
+
+int ol_apply_prolongation(ol_sim* s, int32_t level, int32_t half_time) {
+  if (!s) return fail("null sim");
+  apply_prolongation(s, range_begin(s, level), range_end(s, level), half_time != 0);
+  return 0;
+}
+
+int ol_apply_restriction(ol_sim* s, int32_t fine_level) {
+  if (!s) return fail("null sim");
+  apply_restriction(s, fine_level);
+  return 0;
+}

@@ -63,6 +63,15 @@ int ol_block_collide(ol_sim* sim, int32_t block, double t);
 int ol_block_stream(ol_sim* sim, int32_t block);
 int ol_block_post_stream(ol_sim* sim, int32_t block, double t);
 
+/* Coupler phases on the current state, for the unit checks of the Lagrava
+ * coupling: capture prev from the coarse cells (LevelCoupler ctor), one
+ * prolongation of the plan points whose coarse level is `level` (half_time
+ * 1 = apply_half_time, 0 = apply_full_time), one restriction onto the
+ * coarse cells of the points whose fine level is `fine_level`. */
+int ol_coupler_init(ol_sim* sim);
+int ol_apply_prolongation(ol_sim* sim, int32_t level, int32_t half_time);
+int ol_apply_restriction(ol_sim* sim, int32_t fine_level);
+
 #ifdef __cplusplus
 }
 #endif
