@@ -156,7 +156,15 @@ struct octlb_dev {
   uint8_t* d_stat_use = nullptr;
   bool fast_stats = false;
   double* d_avg = nullptr;
-  int* d_step = nullptr;
+  int* d_step = nullptr;   // [2]: double-buffered coarse-step counter
+  int step_slot = 0;       // slot holding the current step's time
+  int* inc_next = nullptr; // next-step slot, handed to the step's first kernel
+  int* TakeInc(bool launching) {
+    if (!launching) return nullptr;
+    int* p = inc_next;
+    inc_next = nullptr;
+    return p;
+  }
   double* d_out = nullptr;
   double* d_mass_partial = nullptr;
   int* d_plan = nullptr;
@@ -171,6 +179,8 @@ struct octlb_dev {
   void* d_fd_post = nullptr;
   int fd_phase[4] = {0, 0, 0, 0};
   int* d_fd_blocks = nullptr;  // blocked mode: boundary blocks with FD cells
+  short* d_fd_list = nullptr;  // blocked mode: FD cells per FD block, phase order
+  int4* d_fd_count = nullptr;  // blocked mode: phase end offsets per FD block
   short* d_fd_slot = nullptr;  // [fd block][512] FD slot of each cell or -1
   int nfd_blocks = 0;
   std::vector<void*> owned;
@@ -215,18 +225,20 @@ struct octlb_dev {
     prep_owned.push_back(p);
     return p;
   }
+  // Teardown: failures cannot be reported from here and must not stop the
+  // remaining releases, so their status is dropped on purpose.
   void FreePrepared() {
-    for (void* p : prep_owned) hipFree(p);
+    for (void* p : prep_owned) (void)hipFree(p);
     prep_owned.clear();
   }
   void FreeAll() {
     FreePrepared();
     for (auto& g : gexec)
-      if (g) hipGraphExecDestroy(g);
-    for (hipEvent_t e : ev) hipEventDestroy(e);
-    for (void* p : owned) hipFree(p);
+      if (g) (void)hipGraphExecDestroy(g);
+    for (hipEvent_t e : ev) (void)hipEventDestroy(e);
+    for (void* p : owned) (void)hipFree(p);
     owned.clear();
-    if (stream) hipStreamDestroy(stream);
+    if (stream) (void)hipStreamDestroy(stream);
     stream = nullptr;
   }
   ParArray Par() const {
@@ -239,7 +251,7 @@ struct octlb_dev {
   }
   void InvalidateGraphs() {
     for (auto& g : gexec) {
-      if (g) hipGraphExecDestroy(g);
+      if (g) (void)hipGraphExecDestroy(g);
       g = nullptr;
     }
   }
@@ -500,8 +512,8 @@ void octlb_dev::Build(const octlb_topology* t, const octlb_params* p) {
   d_avg = Alloc<double>(1);
   const double one = 1.0;
   HIP_CHECK(hipMemcpyAsync(d_avg, &one, sizeof(double), hipMemcpyHostToDevice, stream));
-  d_step = Alloc<int>(1);
-  HIP_CHECK(hipMemsetAsync(d_step, 0, sizeof(int), stream));
+  d_step = Alloc<int>(2);
+  HIP_CHECK(hipMemsetAsync(d_step, 0, 2 * sizeof(int), stream));
   d_out = Alloc<double>(8);
   d_mass_partial = Alloc<double>(nb);
   d_plan = Alloc<int>(plan.size());
@@ -532,6 +544,11 @@ void octlb_dev::Build(const octlb_topology* t, const octlb_params* p) {
 void octlb_dev::Prepare() {
   Sync();
   FreePrepared();
+  nfd_blocks = 0;
+  d_fd_blocks = nullptr;
+  d_fd_slot = nullptr;
+  d_fd_list = nullptr;
+  d_fd_count = nullptr;
   if (!links.empty() && !sc.bouzidi) throw StateError("Bouzidi links set on a solver built without them");
   if (!links.empty() && padded) throw StateError("Bouzidi links need 8^3 blocks with halo 1");
   for (BlockTab& r : bt) r.flags = 0;
@@ -671,22 +688,41 @@ void octlb_dev::Prepare() {
     d_fd_post = PAllocBytes(std::max<size_t>(all.size(), 1) * kQ * elem);
     if (!padded) {
       std::vector<int> fdb;
-      std::vector<short> slots;
+      std::vector<short> slots, lists;
+      std::vector<int4> counts;
       for (int b : tb_blocks) {
         const uint8_t* kb = &kinds[static_cast<size_t>(b) * ncore];
         std::vector<short> sl(ncore, -1);
+        std::vector<short> byph[3];
         short k = 0;
-        for (int c = 0; c < ncore; ++c)
-          if (IsFd(kb[c])) sl[c] = k++;
+        for (int c = 0; c < ncore; ++c) {
+          if (!IsFd(kb[c])) continue;
+          sl[c] = k++;
+          const int ix = c / (n * n), iy = (c / n) % n, iz = c % n;
+          const int fc = (ix == 0) + (ix == n - 1) + (iy == 0) + (iy == n - 1) + (iz == 0) + (iz == n - 1);
+          byph[fc == 3 ? 2 : (fc == 2 ? 1 : 0)].push_back(static_cast<short>(c));
+        }
         if (k == 0) continue;
         fdb.push_back(b);
         slots.insert(slots.end(), sl.begin(), sl.end());
+        std::vector<short> ls;
+        for (int ph = 0; ph < 3; ++ph) ls.insert(ls.end(), byph[ph].begin(), byph[ph].end());
+        if (ls.size() > static_cast<size_t>(6 * 36 + 12 * 6 + 8))
+          throw ArgError("FD cells must lie on block faces");
+        const int e0 = static_cast<int>(byph[0].size()), e1 = e0 + static_cast<int>(byph[1].size());
+        counts.push_back(make_int4(e0, e1, static_cast<int>(ls.size()), 0));
+        ls.resize(ncore, -1);
+        lists.insert(lists.end(), ls.begin(), ls.end());
       }
       nfd_blocks = static_cast<int>(fdb.size());
       d_fd_blocks = PAlloc<int>(fdb.size());
       Upload(d_fd_blocks, fdb, stream);
       d_fd_slot = PAlloc<short>(slots.size());
       Upload(d_fd_slot, slots, stream);
+      d_fd_list = PAlloc<short>(lists.size());
+      Upload(d_fd_list, lists, stream);
+      d_fd_count = PAlloc<int4>(counts.size());
+      Upload(d_fd_count, counts, stream);
     }
   } else {
     fd_phase[0] = fd_phase[1] = fd_phase[2] = fd_phase[3] = 0;
@@ -740,7 +776,7 @@ TileArgs<T> octlb_dev::MakeTileArgs(int l) {
     a.tiles_per_block = padded ? tpa * tpa * tpa : 1;
   }
   a.avg_rho = d_avg;
-  a.step = d_step;
+  a.step = d_step + step_slot;
   a.want_stats = (sc.const_rho && l >= max_level) ? (fast_stats ? 1 : 2) : 0;
   a.snap_on = 1;
   a.sc = sc;
@@ -776,6 +812,7 @@ void octlb_dev::Level(int l) {
   const int cnt = tile_count[l];
   if (cnt > 0) {
     TileArgs<T> a = MakeTileArgs<T>(l);
+    a.step_next = TakeInc(true);
     const bool timed = timing && !capturing;
     if (timed) {
       while (ev.size() < ev_used + 2) {
@@ -810,7 +847,10 @@ void octlb_dev::Level(int l) {
     f.pops0 = static_cast<T*>(pops[0]);
     f.pops1 = static_cast<T*>(pops[1]);
     f.post = static_cast<T*>(d_fd_post);
-    f.step = d_step;
+    f.step = d_step + step_slot;
+    f.fd_list = d_fd_list;
+    f.fd_count = d_fd_count;
+    f.step_next = TakeInc(padded ? fd_phase[3] > 0 : nfd_blocks > 0);
     f.par = Par();
     f.sc = sc;
     if (padded) {
@@ -828,20 +868,37 @@ void octlb_dev::Level(int l) {
   const double tau = 1.0 / sc.omega;  // level_coupler.cpp:226-228
   c.prolong_scale = static_cast<T>((tau - 0.25) / tau);
   c.restrict_scale = static_cast<T>(tau / (tau - 0.25));
+  c.step = d_step + step_slot;
+  const int npro = static_cast<int>(prolong_active[l].size());
+  const int nres = static_cast<int>(restrict_active[l + 1].size());
   c.par = Par();
-  LaunchProlong<T>(c, d_prolong + prolong_off[l], static_cast<int>(prolong_active[l].size()), 1, stream);
+  c.step_next = TakeInc(npro > 0);
+  LaunchProlong<T>(c, d_prolong + prolong_off[l], npro, 1, stream);
   Level<T>(l + 1);
   c.par = Par();
-  LaunchProlong<T>(c, d_prolong + prolong_off[l], static_cast<int>(prolong_active[l].size()), 0, stream);
+  c.step_next = TakeInc(npro > 0);
+  LaunchProlong<T>(c, d_prolong + prolong_off[l], npro, 0, stream);
   Level<T>(l + 1);
   c.par = Par();
-  LaunchRestrict<T>(c, d_restrict + restrict_off[l + 1], static_cast<int>(restrict_active[l + 1].size()), stream);
+  c.step_next = TakeInc(nres > 0);
+  LaunchRestrict<T>(c, d_restrict + restrict_off[l + 1], nres, stream);
 }
 
+// One coarse step.  The step counter lives in two slots: this step's kernels
+// read step_slot, the first kernel launched also writes step_slot ^ 1.
+// Level 0's buffer parity flips every coarse step, even when level 0 has no
+// blocks, so (par[0], step_slot) is the same for every replay of a graph.
 template <typename T>
 void octlb_dev::CoarseStep() {
+  step_slot = par[0];
+  inc_next = d_step + (step_slot ^ 1);
   Level<T>(0);
-  LaunchStepInc(d_step, stream);
+  if (inc_next) {
+    LaunchStepInc(d_step + step_slot, inc_next, stream);
+    inc_next = nullptr;
+  }
+  par[0] = step_slot ^ 1;
+  step_slot = par[0];
 }
 
 void octlb_dev::Step() {
@@ -1042,7 +1099,7 @@ int octlb_dev_create(const octlb_topology* topo, const octlb_params* params, oct
 
 int octlb_dev_destroy(octlb_dev* dev) {
   if (!dev) return OCTLB_OK;
-  if (dev->stream) hipStreamSynchronize(dev->stream);
+  if (dev->stream) (void)hipStreamSynchronize(dev->stream);  // teardown: nothing to report to
   dev->FreeAll();
   delete dev;
   return OCTLB_OK;
@@ -1111,8 +1168,8 @@ int octlb_dev_init_field(octlb_dev* d, const double* rho, const double* u) {
     }
     HIP_CHECK(hipGetLastError());
     d->Sync();
-    hipFree(dr);
-    hipFree(du);
+    HIP_CHECK(hipFree(dr));
+    HIP_CHECK(hipFree(du));
     d->coupler_ready = false;
   });
 }
@@ -1191,6 +1248,7 @@ int octlb_dev_advance_graph(octlb_dev* d, int32_t steps) {
       if (!d->gexec[p0]) {
         int saved[kMaxLevels];
         std::memcpy(saved, d->par, sizeof(saved));
+        const int saved_slot = d->step_slot;
         hipGraph_t g = nullptr;
         d->capturing = true;
         HIP_CHECK(hipStreamBeginCapture(d->stream, hipStreamCaptureModeThreadLocal));
@@ -1202,11 +1260,13 @@ int octlb_dev_advance_graph(octlb_dev* d, int32_t steps) {
         HIP_CHECK(hipStreamEndCapture(d->stream, &g));
         d->capturing = false;
         std::memcpy(d->par, saved, sizeof(saved));
+        d->step_slot = saved_slot;
         HIP_CHECK(hipGraphInstantiate(&d->gexec[p0], g, nullptr, nullptr, 0));
         HIP_CHECK(hipGraphDestroy(g));
       }
       HIP_CHECK(hipGraphLaunch(d->gexec[p0], d->stream));
       d->par[0] ^= 1;  // finer levels step an even number of times per coarse step
+      d->step_slot = d->par[0];
       ++d->step;
     }
     d->Sync();
@@ -1224,7 +1284,7 @@ int octlb_dev_set_step(octlb_dev* d, int32_t step) {
   return Guard([&] {
     CheckDev(d);
     d->step = step;
-    HIP_CHECK(hipMemcpyAsync(d->d_step, &d->step, sizeof(int), hipMemcpyHostToDevice, d->stream));
+    HIP_CHECK(hipMemcpyAsync(d->d_step + d->step_slot, &d->step, sizeof(int), hipMemcpyHostToDevice, d->stream));
     d->Sync();
   });
 }

@@ -46,6 +46,13 @@ __device__ inline int XcdTile(int w, int nt) {
   return (w < (q8 << 3)) ? (w & 7) * q8 + (w >> 3) : w;
 }
 
+// The coarse-step counter is double-buffered: a step's kernels read *step;
+// the first kernel of the step also writes the next step's slot, which
+// nothing in this step reads.
+__device__ inline void BumpStep(const int* step, int* step_next) {
+  if (step_next != nullptr && blockIdx.x == 0 && threadIdx.x == 0) *step_next = *step + 1;
+}
+
 template <typename T>
 __device__ inline void BlockReduceStats(double s, int c, double* red_s, int* red_c, double* out) {
   const int tid = threadIdx.x;
@@ -66,28 +73,27 @@ __device__ inline void BlockReduceStats(double s, int c, double* red_s, int* red
   }
 }
 
-// Pull-stream rule of one core cell (block_lattice.cpp:579-674), given the
-// LDS plane of population i at stream time.  `own` holds the cell's
-// post-collide populations; i is a constant after the caller's unrolling.
-// own_i / own_opp are the cell's post-collide f_i and f_opp(i) as register
-// values (never addresses, so the population array stays in VGPRs).
+// Pull-stream rule of one core cell (block_lattice.cpp:579-674) for
+// population i (a constant after the caller's unrolling), reading the LDS
+// box of post-collide values: pl = plane of population i, po = plane of its
+// opposite, own_l = the cell's box index.
 template <typename T, bool PERCELL>
-__device__ __forceinline__ T StreamPop(int i, const T* pl, const uint8_t* kbox, int own_l, int k,
-                                       T own_i, T own_opp, bool src_interior, uint32_t qm,
-                                       const double* qv) {
+__device__ __forceinline__ T StreamPop(int i, const T* pl, const T* po, const uint8_t* kbox, int own_l, int k,
+                                       bool src_interior, uint32_t qm, const double* qv) {
   const int off = (Cx(i) * kBox + Cy(i)) * kBox + Cz(i);
   const T pulled = pl[own_l - off];
   if (!PERCELL) return pulled;
-  if (k == kSolid) return own_i;
-  if (ReflectsOnPull(k) && !src_interior) return own_opp;
+  if (k == kSolid) return pl[own_l];
+  if (ReflectsOnPull(k) && !src_interior) return po[own_l];
   const int sk = kbox[own_l - off];
   if (sk == kSolid || sk == kBouzidi) {
+    const T own_opp = po[own_l];
     double q = 1.0;
     if ((qm >> i) & 1u) q = qv[__popc(qm & ((1u << i) - 1u))];
     if (q > 1.0e-10 && q < 1.0 - 1.0e-10) {
+      const T own_i = pl[own_l];
       const int nl = own_l + off;
-      const T nb_i = pl[nl];
-      const double f_int = (kbox[nl] == kBulk) ? static_cast<double>(nb_i) : static_cast<double>(own_i);
+      const double f_int = (kbox[nl] == kBulk) ? static_cast<double>(pl[nl]) : static_cast<double>(own_i);
       return static_cast<T>(BouzidiPull(static_cast<double>(own_opp), f_int, static_cast<double>(own_i), q));
     }
     return own_opp;
@@ -96,45 +102,56 @@ __device__ __forceinline__ T StreamPop(int i, const T* pl, const uint8_t* kbox, 
 }
 
 // ------------------------------------------------------- blocked fused kernel
-// Populations streamed per LDS round: FP32 4 planes (5 rounds), FP64 2 (10).
+// One 1024-thread workgroup per 8^3 block.  Threads 0..511 own the block's
+// cells; threads 512..999 own its 488 apron slots (the halo of the 10^3 box,
+// in shell order).  Every thread loads its 19 pre-collide populations -- own
+// cell, neighbour block's boundary cell (the ghost exchange, done in place)
+// or, for slots no exchange fills, the block's shell -- collides real cells,
+// and writes all 19 post-collide values into a 19 x 10^3 LDS box (76 KB FP32,
+// 152 KB FP64; gfx950 gives a workgroup up to 160 KB).  After one barrier the
+// cell threads pull-stream and store, and apron threads of unexchanged slots
+// do the halo-shell torus pull (block_lattice.cpp:595-611).
+// FP32: two workgroups per CU (2 x 77 KB LDS) need <= 64 VGPRs (8 waves per
+// SIMD); FP64: one workgroup per CU (153 KB LDS), 4 waves per SIMD.
 template <typename T>
-struct Rounds {
-  static constexpr int P = sizeof(T) == 4 ? 4 : 2;
+struct BlockOccupancy {
+  static constexpr int kWavesPerSimd = sizeof(T) == 4 ? 8 : 4;
 };
 
 template <typename T, bool PERCELL>
-__global__ __launch_bounds__(512) void KBlock(TileArgs<T> a) {
-  constexpr int P = Rounds<T>::P;
-  __shared__ T plane[2][P][kBoxCells];
+__global__ __launch_bounds__(1024, BlockOccupancy<T>::kWavesPerSimd) void KBlock(TileArgs<T> a) {
+  __shared__ T plane[kQ][kBoxCells];
   __shared__ uint8_t kbox[kBoxCells];
-  static_assert(sizeof(plane) >= 512 * (sizeof(double) + sizeof(int)), "stats alias");
-  double* red_s = reinterpret_cast<double*>(&plane[0][0][0]);
-  int* red_c = reinterpret_cast<int*>(red_s + 512);
+  static_assert(sizeof(plane) >= 1024 * (sizeof(double) + sizeof(int)), "stats alias");
+  double* red_s = reinterpret_cast<double*>(&plane[0][0]);
+  int* red_c = reinterpret_cast<int*>(red_s + 1024);
 
+  BumpStep(a.step, a.step_next);
   const int ti = XcdTile(blockIdx.x, a.ntiles);
   const int b = a.tiles[ti].block;
   const SimConst& sc = a.sc;
   constexpr int n = kTile, ncore = kTileCells;
   const int tid = threadIdx.x;
-  const int ix = tid >> 6, iy = (tid >> 3) & 7, iz = tid & 7;
-  const size_t bbase = static_cast<size_t>(b) * kQ * ncore;
+  const bool own = tid < ncore;
+  const int ap = tid - ncore;  // apron slot (shell index) when !own
   const BlockTab& bt = a.bt[b];
   const int bflags = bt.flags;
   const int bshell = bt.shell;
 
-  // ---- issue every global load before any arithmetic: own cell, then the
-  // apron slot (the block's halo; apron index == shell index)
-  T f[kQ];
-#pragma unroll
-  for (int i = 0; i < kQ; ++i) f[i] = a.src[bbase + static_cast<size_t>(i) * ncore + tid];
-  const int k = a.kind[static_cast<size_t>(b) * ncore + tid];
-  T g[kQ];
-  int ap_l = -1;
-  int nb = -1, wx = 0, wy = 0, wz = 0, kk = kBulk;
-  bool live = false;
-  if (tid < kApron) {
-    ap_l = kApronTab.idx[tid];
-    const int px = ap_l / (kBox * kBox), py = (ap_l / kBox) % kBox, pz = ap_l % kBox;
+  // ---- where this thread's cell lives
+  int l = -1;       // 10^3 box index
+  int src_b = -1;   // block whose core holds the cell (-1: shell or nothing)
+  int cx = 0, cy = 0, cz = 0;
+  int nb = -1;
+  if (own) {
+    cx = tid >> 6;
+    cy = (tid >> 3) & 7;
+    cz = tid & 7;
+    l = ((cx + 1) * kBox + cy + 1) * kBox + cz + 1;
+    src_b = b;
+  } else if (ap < kApron) {
+    l = kApronTab.idx[ap];
+    const int px = l / (kBox * kBox), py = (l / kBox) % kBox, pz = l % kBox;
     const int x = px - 1, y = py - 1, z = pz - 1;
     const int ox = (x < 0 || x >= n), oy = (y < 0 || y >= n), oz = (z < 0 || z >= n);
     const int nout = ox + oy + oz;
@@ -150,92 +167,54 @@ __global__ __launch_bounds__(512) void KBlock(TileArgs<T> a) {
       nb = bt.nbr[EdgeSlot(a1, v1 >= n ? 1 : 0, a2, v2 >= n ? 1 : 0)];
     }
     if (nb >= 0) {
-      wx = x < 0 ? n - 1 : (x >= n ? 0 : x);
-      wy = y < 0 ? n - 1 : (y >= n ? 0 : y);
-      wz = z < 0 ? n - 1 : (z >= n ? 0 : z);
-      const int cc = (wx * n + wy) * n + wz;
-      const size_t nbase = static_cast<size_t>(nb) * kQ * ncore;
-#pragma unroll
-      for (int i = 0; i < kQ; ++i) g[i] = a.src[nbase + static_cast<size_t>(i) * ncore + cc];
-      kk = a.kind[static_cast<size_t>(nb) * ncore + cc];
-    } else if (bshell >= 0) {
-      const size_t sbase = static_cast<size_t>(bshell) * kQ * kApron;
-#pragma unroll
-      for (int i = 0; i < kQ; ++i) g[i] = a.shell_src[sbase + static_cast<size_t>(i) * kApron + tid];
-    } else {
-#pragma unroll
-      for (int i = 0; i < kQ; ++i) g[i] = T(0);
+      cx = x < 0 ? n - 1 : (x >= n ? 0 : x);
+      cy = y < 0 ? n - 1 : (y >= n ? 0 : y);
+      cz = z < 0 ? n - 1 : (z >= n ? 0 : z);
+      src_b = nb;
     }
-    live = (bflags & kFlagShellLive) != 0;
+  }
+  const int cc = (cx * n + cy) * n + cz;
+
+  // ---- load: every global read is issued before any arithmetic
+  T f[kQ];
+  int k = kBulk;
+  if (src_b >= 0) {
+    const size_t base = static_cast<size_t>(src_b) * kQ * ncore + cc;
+#pragma unroll
+    for (int i = 0; i < kQ; ++i) f[i] = a.src[base + static_cast<size_t>(i) * ncore];
+    k = a.kind[static_cast<size_t>(src_b) * ncore + cc];
+  } else if (l >= 0 && bshell >= 0) {
+    const size_t base = static_cast<size_t>(bshell) * kQ * kApron + ap;
+#pragma unroll
+    for (int i = 0; i < kQ; ++i) f[i] = a.shell_src[base + static_cast<size_t>(i) * kApron];
+  } else {
+#pragma unroll
+    for (int i = 0; i < kQ; ++i) f[i] = T(0);
   }
   uint32_t qm = 0;
   const double* qv = a.qval;
-  if (PERCELL && (bflags & kFlagLinks)) {
+  if (PERCELL && own && (bflags & kFlagLinks)) {
     qm = a.qmask[static_cast<size_t>(b) * ncore + tid];
     qv = a.qval + a.qoff[static_cast<size_t>(b) * ncore + tid];
   }
 
-  // ---- collide own cell and apron cell
-  const double t = static_cast<double>(*a.step);
-  const T avg = sc.const_rho ? static_cast<T>(*a.avg_rho) : T(1);
+  // ---- collide real cells (own and exchanged apron); halo-shell slots are
+  // streamed but never collided
   double ssum = 0.0;
   int scnt = 0;
-  {
-    const Origin og{bt.origin[0], bt.origin[1], bt.origin[2], bt.cw};
-    CollideCell<T>(f, k, sc, og, ix, iy, iz, t, avg, a.want_stats != 0, ssum, scnt);
+  if (src_b >= 0) {
+    const double t = static_cast<double>(*a.step);
+    const T avg = sc.const_rho ? static_cast<T>(*a.avg_rho) : T(1);
+    const BlockTab& st = a.bt[src_b];
+    const Origin og{st.origin[0], st.origin[1], st.origin[2], st.cw};
+    CollideCell<T>(f, k, sc, og, cx, cy, cz, t, avg, own && a.want_stats != 0, ssum, scnt);
   }
-  const int own_l = ((ix + 1) * kBox + iy + 1) * kBox + iz + 1;
-  kbox[own_l] = static_cast<uint8_t>(k);
-  if (ap_l >= 0) {
-    kbox[ap_l] = kBulk;  // kind_at_halo -> kBulk (block_lattice.cpp:510-518)
-    if (nb >= 0) {
-      const BlockTab& nt = a.bt[nb];
-      const Origin og{nt.origin[0], nt.origin[1], nt.origin[2], nt.cw};
-      double d0 = 0.0;
-      int d1 = 0;
-      CollideCell<T>(g, kk, sc, og, wx, wy, wz, t, avg, false, d0, d1);
-    }
+  if (l >= 0) {
+#pragma unroll
+    for (int i = 0; i < kQ; ++i) plane[i][l] = f[i];
+    kbox[l] = static_cast<uint8_t>(own ? k : kBulk);  // kind_at_halo -> kBulk (block_lattice.cpp:510-518)
   }
-
-  // ---- stream P population planes per LDS round; results go straight out
-  T* dst = a.dst + bbase + tid;
-  // only slots that no exchange refills keep their torus-streamed value
-  // (GhostSchedule::exchange overwrites the others before they are read)
-  T* sdst = (live && bshell >= 0 && nb < 0) ? a.shell_dst + static_cast<size_t>(bshell) * kQ * kApron + tid : nullptr;
-#pragma unroll
-  for (int i0 = 0; i0 < kQ; i0 += P) {
-    T (*pl)[kBoxCells] = plane[(i0 / P) & 1];
-#pragma unroll
-    for (int j = 0; j < P; ++j) {
-      if (i0 + j < kQ) {
-        pl[j][own_l] = f[i0 + j];
-        if (ap_l >= 0) pl[j][ap_l] = g[i0 + j];
-      }
-    }
-    __syncthreads();
-#pragma unroll
-    for (int j = 0; j < P; ++j) {
-      const int i = i0 + j;
-      if (i < kQ) {
-        const int sx = ix - Cx(i), sy = iy - Cy(i), sz = iz - Cz(i);
-        const bool src_int = sx >= 0 && sx < n && sy >= 0 && sy < n && sz >= 0 && sz < n;
-        T oi = f[i], oo = f[Opp(i)];
-        asm volatile("" : "+v"(oi));
-        asm volatile("" : "+v"(oo));
-        dst[static_cast<size_t>(i) * ncore] = StreamPop<T, PERCELL>(i, pl[j], kbox, own_l, k, oi, oo, src_int, qm, qv);
-        if (sdst) {
-          // padded-block torus pull of a halo slot; in-halo and wrapped
-          // sources are both (d - off) mod NP^3 (block_lattice.cpp:595-611)
-          const int off = (Cx(i) * kBox + Cy(i)) * kBox + Cz(i);
-          int sidx = ap_l - off;
-          if (sidx < 0) sidx += kBoxCells;
-          if (sidx >= kBoxCells) sidx -= kBoxCells;
-          sdst[static_cast<size_t>(i) * kApron] = pl[j][sidx];
-        }
-      }
-    }
-  }
-  if ((bflags & kFlagDrag) && a.snap_on) {
+  if (own && (bflags & kFlagDrag) && a.snap_on) {
     // post-collide snapshot of the outgoing populations on wall links
     // (take_post_collide_snapshot, block_lattice.cpp:309-316)
     const uint32_t dm = a.dmask[static_cast<size_t>(b) * ncore + tid];
@@ -247,9 +226,33 @@ __global__ __launch_bounds__(512) void KBlock(TileArgs<T> a) {
         if ((dm >> i) & 1u) sp[j++] = f[i];
     }
   }
+  __syncthreads();
+
+  if (own) {
+    T* dst = a.dst + static_cast<size_t>(b) * kQ * ncore + tid;
+#pragma unroll
+    for (int i = 0; i < kQ; ++i) {
+      const int sx = cx - Cx(i), sy = cy - Cy(i), sz = cz - Cz(i);
+      const bool src_int = sx >= 0 && sx < n && sy >= 0 && sy < n && sz >= 0 && sz < n;
+      dst[static_cast<size_t>(i) * ncore] =
+          StreamPop<T, PERCELL>(i, plane[i], plane[Opp(i)], kbox, l, k, src_int, qm, qv);
+    }
+  } else if (l >= 0 && nb < 0 && bshell >= 0 && (bflags & kFlagShellLive)) {
+    // padded-block torus pull of a halo slot that no exchange refills;
+    // in-halo and wrapped sources are both (d - off) mod NP^3
+    T* sdst = a.shell_dst + static_cast<size_t>(bshell) * kQ * kApron + ap;
+#pragma unroll
+    for (int i = 0; i < kQ; ++i) {
+      const int off = (Cx(i) * kBox + Cy(i)) * kBox + Cz(i);
+      int sidx = l - off;
+      if (sidx < 0) sidx += kBoxCells;
+      if (sidx >= kBoxCells) sidx -= kBoxCells;
+      sdst[static_cast<size_t>(i) * kApron] = plane[i][sidx];
+    }
+  }
   if (a.want_stats == 1) {
     BlockReduceStats<T>(ssum, scnt, red_s, red_c, a.stats + 2 * ti);
-  } else if (a.want_stats == 2) {
+  } else if (a.want_stats == 2 && own) {
     const size_t ord = static_cast<size_t>(ti) * ncore + tid;
     a.stat_val[ord] = ssum;
     a.stat_use[ord] = static_cast<uint8_t>(scnt);
@@ -269,6 +272,7 @@ __global__ __launch_bounds__(512) void KPadded(TileArgs<T> a) {
   double* red_s = reinterpret_cast<double*>(&plane[0][0]);
   int* red_c = reinterpret_cast<int*>(red_s + 512);
 
+  BumpStep(a.step, a.step_next);
   const int ti = XcdTile(blockIdx.x, a.ntiles);
   const Tile tl = a.tiles[ti];
   const int b = tl.block;
@@ -424,8 +428,8 @@ __global__ void KStatsSequential(const double* val, const uint8_t* use, int coun
   *avg = c > 0 ? s / static_cast<double>(c) : 1.0;
 }
 
-__global__ void KStepInc(int* step) {
-  if (threadIdx.x == 0 && blockIdx.x == 0) *step += 1;
+__global__ void KStepInc(const int* step, int* step_next) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) *step_next = *step + 1;
 }
 
 template <typename T>
@@ -767,6 +771,7 @@ __device__ inline void FdCell(const V& v, int ix, int iy, int iz, T* fnew) {
 
 template <typename T>
 __global__ void KFd(FdArgs<T> a, int begin, int count) {
+  BumpStep(a.step, a.step_next);
   const int e = begin + static_cast<int>(blockIdx.x * blockDim.x + threadIdx.x);
   if (e >= begin + count) return;
   const int b = a.cells[e].x, c = a.cells[e].y;
@@ -876,97 +881,289 @@ __device__ inline T FdRhoOf(int k, const T* f, const T* pu, const SimConst& sc, 
   return rho;
 }
 
+// Runtime-index forms of Feq / PiToFneq for the (cell, population) task
+// loops; same operations in the same order as the compile-time versions.
+__constant__ int kCxT[kQ] = {0, -1, 0, 0, -1, -1, -1, -1, 0, 0, 1, 0, 0, 1, 1, 1, 1, 0, 0};
+__constant__ int kCyT[kQ] = {0, 0, -1, 0, -1, 1, 0, 0, -1, -1, 0, 1, 0, 1, -1, 0, 0, 1, 1};
+__constant__ int kCzT[kQ] = {0, 0, 0, -1, 0, 0, -1, 1, -1, 1, 0, 0, 1, 0, 0, 1, -1, 1, -1};
+__constant__ int kWDenT[kQ] = {3, 18, 18, 18, 36, 36, 36, 36, 36, 36, 18, 18, 18, 36, 36, 36, 36, 36, 36};
+
+template <typename T>
+__device__ inline T FdPopulation(int i, T rho, const T* u, T usqr, const T* pi) {
+  const int c[3] = {kCxT[i], kCyT[i], kCzT[i]};
+  const T t = T(1) / T(kWDenT[i]);
+  T cu = T(0);
+  cu += T(c[0]) * u[0];
+  cu += T(c[1]) * u[1];
+  cu += T(c[2]) * u[2];
+  const T feq = rho * t * (T(1) + T(3) * cu + T(3) * T(3) * T(0.5) * cu * cu - T(3) * T(0.5) * usqr) - t;
+  T fneq = T(0);
+  int ipi = 0;
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+#pragma unroll
+    for (int b = a; b < 3; ++b) {
+      T to_add = T(c[a] * c[b]);
+      if (a == b) {
+        to_add -= T(1) / T(3);
+      } else {
+        to_add *= T(2);
+      }
+      to_add *= pi[ipi++];
+      fneq += to_add;
+    }
+  }
+  fneq *= t * T(3) * T(3) / T(2);
+  return feq + fneq;
+}
+
+constexpr int kMaxFdCells = 6 * 36 + 12 * 6 + 8;  // face, edge and corner cells of an 8^3 block
+
+// Blocked mode: one workgroup per boundary block.  Every cell's density and
+// velocity as the FD stencils see them (BoundaryLatticeView::ComputeRho /
+// ComputeU, interpolated_velocity.h:352-434) are computed once into LDS and
+// the pressure-extrapolation chains resolved once.  FD cells are then spread
+// over the threads (one per cell, plane|edge|corner order): all gradients are
+// taken at once; per phase the cells of that phase get their density (edges
+// and corners extrapolate densities that earlier phases rewrote, as the
+// reference's in-place (ix,iy,iz) sweep does), and the 19 populations of every
+// cell of the phase are written by all 512 threads as (cell, population) tasks.
 template <typename T>
 __global__ __launch_bounds__(512) void KFdBlock(FdArgs<T> a, const int* fd_blocks, const short* fd_slot) {
   __shared__ T rho_pre[kTileCells], rho_post[kTileCells];
   __shared__ T ux[kTileCells], uy[kTileCells], uz[kTileCells];
   __shared__ short slot[kTileCells];
   __shared__ uint8_t kind[kTileCells];
+  __shared__ T crho[kMaxFdCells], cusq[kMaxFdCells];
+  __shared__ T cu[kMaxFdCells][3], cpi[kMaxFdCells][6];
+  __shared__ T fbuf[kMaxFdCells][kQ];
+  BumpStep(a.step, a.step_next);
   const int fb = blockIdx.x;
   const int b = fd_blocks[fb];
   const int tid = threadIdx.x;
   const SimConst& sc = a.sc;
   T* cur = (a.par.v[a.level[b]] ? a.pops1 : a.pops0) + static_cast<size_t>(b) * kQ * kTileCells;
-  const int ix = tid >> 6, iy = (tid >> 3) & 7, iz = tid & 7;
-  const int k = a.kind[static_cast<size_t>(b) * kTileCells + tid];
-  const int sl = fd_slot[static_cast<size_t>(fb) * kTileCells + tid];
-  kind[tid] = static_cast<uint8_t>(k);
-  slot[tid] = static_cast<short>(sl);
-  const double t = static_cast<double>(*a.step);
-  const Origin og{a.bt[b].origin[0], a.bt[b].origin[1], a.bt[b].origin[2], a.bt[b].cw};
-  T f[kQ];
+  {
+    const int ix = tid >> 6, iy = (tid >> 3) & 7, iz = tid & 7;
+    const int k = a.kind[static_cast<size_t>(b) * kTileCells + tid];
+    kind[tid] = static_cast<uint8_t>(k);
+    slot[tid] = fd_slot[static_cast<size_t>(fb) * kTileCells + tid];
+    const double t = static_cast<double>(*a.step);
+    const Origin og{a.bt[b].origin[0], a.bt[b].origin[1], a.bt[b].origin[2], a.bt[b].cw};
+    T f[kQ];
 #pragma unroll
-  for (int i = 0; i < kQ; ++i) f[i] = cur[static_cast<size_t>(i) * kTileCells + tid];
-  T pu[3] = {T(0), T(0), T(0)};
-  if (k == kVelocity) {
-    const double org[3] = {og.x, og.y, og.z};
-    double ud[3];
-    PrescribedU(sc, org, og.cw, ix, iy, iz, t, ud);
-    pu[0] = static_cast<T>(ud[0]);
-    pu[1] = static_cast<T>(ud[1]);
-    pu[2] = static_cast<T>(ud[2]);
-  }
-  rho_pre[tid] = FdRhoOf(k, f, pu, sc, ix, iy, iz);
-  if (k == kVelocity) {
-    ux[tid] = pu[0];
-    uy[tid] = pu[1];
-    uz[tid] = pu[2];
-  } else {
-    T rho, u[3];
-    ProxyRhoU(f, rho, u);
-    ux[tid] = u[0];
-    uy[tid] = u[1];
-    uz[tid] = u[2];
-  }
-  __syncthreads();
-  // ExtrapolatedU (:297-316): a pressure cell takes the velocity of the first
-  // non-pressure cell inward along its face normal; only pressure entries are
-  // rewritten, and walks only read non-pressure entries.
-  T pux = T(0), puy = T(0), puz = T(0);
-  if (k == kPressure) {
-    int x = ix, y = iy, z = iz;
-    for (;;) {
-      const int kk = kind[(x * kTile + y) * kTile + z];
-      if (kk != kPressure) {
-        const int c = (x * kTile + y) * kTile + z;
-        pux = ux[c];
-        puy = uy[c];
-        puz = uz[c];
-        break;
-      }
-      int dir, ori;
-      if (!FlatInfo(kTile, x, y, z, dir, ori)) break;
-      if (dir == 0) {
-        x -= ori;
-      } else if (dir == 1) {
-        y -= ori;
-      } else {
-        z -= ori;
-      }
-      if (x < 0 || x >= kTile || y < 0 || y >= kTile || z < 0 || z >= kTile) break;
+    for (int i = 0; i < kQ; ++i) f[i] = cur[static_cast<size_t>(i) * kTileCells + tid];
+    T pu[3] = {T(0), T(0), T(0)};
+    if (k == kVelocity) {
+      const double org[3] = {og.x, og.y, og.z};
+      double ud[3];
+      PrescribedU(sc, org, og.cw, ix, iy, iz, t, ud);
+      pu[0] = static_cast<T>(ud[0]);
+      pu[1] = static_cast<T>(ud[1]);
+      pu[2] = static_cast<T>(ud[2]);
+    }
+    rho_pre[tid] = FdRhoOf(k, f, pu, sc, ix, iy, iz);
+    if (k == kVelocity) {
+      ux[tid] = pu[0];
+      uy[tid] = pu[1];
+      uz[tid] = pu[2];
+    } else {
+      T rho, u[3];
+      ProxyRhoU(f, rho, u);
+      ux[tid] = u[0];
+      uy[tid] = u[1];
+      uz[tid] = u[2];
     }
   }
   __syncthreads();
-  if (k == kPressure) {
-    ux[tid] = pux;
-    uy[tid] = puy;
-    uz[tid] = puz;
-  }
-  __syncthreads();
-  FdLds<T> v{rho_pre, rho_post, ux, uy, uz, slot, &a.sc, tid};
-  const int fc = (ix == 0) + (ix == kTile - 1) + (iy == 0) + (iy == kTile - 1) + (iz == 0) + (iz == kTile - 1);
-  const int phase = fc == 3 ? 2 : (fc == 2 ? 1 : 0);
-  T fnew[kQ];
-  for (int ph = 0; ph < 3; ++ph) {
-    if (sl >= 0 && phase == ph) {
-      FdCell<T>(v, ix, iy, iz, fnew);
-      rho_post[tid] = FdRhoOf(k, fnew, pu, sc, ix, iy, iz);
+  {
+    // ExtrapolatedU (:297-316): a pressure cell takes the velocity of the
+    // first non-pressure cell inward along its face normal; only pressure
+    // entries are rewritten, and walks only read non-pressure entries.
+    const int k = kind[tid];
+    T pux = T(0), puy = T(0), puz = T(0);
+    if (k == kPressure) {
+      int x = tid >> 6, y = (tid >> 3) & 7, z = tid & 7;
+      for (;;) {
+        const int c = (x * kTile + y) * kTile + z;
+        if (kind[c] != kPressure) {
+          pux = ux[c];
+          puy = uy[c];
+          puz = uz[c];
+          break;
+        }
+        int dir, ori;
+        if (!FlatInfo(kTile, x, y, z, dir, ori)) break;
+        if (dir == 0) {
+          x -= ori;
+        } else if (dir == 1) {
+          y -= ori;
+        } else {
+          z -= ori;
+        }
+        if (x < 0 || x >= kTile || y < 0 || y >= kTile || z < 0 || z >= kTile) break;
+      }
     }
     __syncthreads();
+    if (k == kPressure) {
+      ux[tid] = pux;
+      uy[tid] = puy;
+      uz[tid] = puz;
+    }
   }
-  if (sl >= 0) {
+  __syncthreads();
+
+  // ---- gradients of the FD cell this thread owns
+  const int4 cnt = a.fd_count[fb];
+  const int ph_end[3] = {cnt.x, cnt.y, cnt.z};
+  const int nfd = cnt.z;
+  int my = -1, myph = 3;
+  T G[3][3];
+  T myu[3] = {T(0), T(0), T(0)};
+  int fc = 0, xn = 0, yn = 0, zn = 0, ix = 0, iy = 0, iz = 0;
+  if (tid < nfd) {
+    my = a.fd_list[static_cast<size_t>(fb) * kTileCells + tid];
+    myph = tid < cnt.x ? 0 : (tid < cnt.y ? 1 : 2);
+    ix = my >> 6;
+    iy = (my >> 3) & 7;
+    iz = my & 7;
+    FdLds<T> v{rho_pre, rho_post, ux, uy, uz, slot, &a.sc, my};
+    const int xm = ix == 0, xp = ix == kTile - 1, ym = iy == 0, yp = iy == kTile - 1, zm = iz == 0,
+              zp = iz == kTile - 1;
+    fc = xm + xp + ym + yp + zm + zp;
+    xn = xm ? -1 : (xp ? 1 : 0);
+    yn = ym ? -1 : (yp ? 1 : 0);
+    zn = zm ? -1 : (zp ? 1 : 0);
 #pragma unroll
-    for (int i = 0; i < kQ; ++i) cur[static_cast<size_t>(i) * kTileCells + tid] = fnew[i];
+    for (int p = 0; p < 3; ++p)
+#pragma unroll
+      for (int q = 0; q < 3; ++q) G[p][q] = T(0);
+    if (fc == 3) {
+      // ApplyCornerBoundary (:638-685)
+      v.NormalGrad(ix, iy, iz, 0, xn, G[0]);
+      v.NormalGrad(ix, iy, iz, 1, yn, G[1]);
+      v.NormalGrad(ix, iy, iz, 2, zn, G[2]);
+    } else if (fc == 2) {
+      // ApplyEdgeBoundary (:687-752)
+      int plane, n1, n2;
+      if (xn != 0 && yn != 0) {
+        plane = 2;
+        n1 = xn;
+        n2 = yn;
+      } else if (xn != 0 && zn != 0) {
+        plane = 1;
+        n1 = xn;
+        n2 = zn;
+      } else {
+        plane = 0;
+        n1 = yn;
+        n2 = zn;
+      }
+      const int d1 = (plane + 1) % 3, d2 = (plane + 2) % 3;
+      v.TangGrad(ix, iy, iz, plane == 0, plane == 1, plane == 2, G[plane]);
+      v.NormalGrad(ix, iy, iz, d1, n1, G[d1]);
+      v.NormalGrad(ix, iy, iz, d2, n2, G[d2]);
+    } else {
+      // ApplyPlaneFdBoundary (:570-636)
+      int face = OCTLB_XMIN;
+      if (xp) {
+        face = OCTLB_XMAX;
+      } else if (yp) {
+        face = OCTLB_YMAX;
+      } else if (ym) {
+        face = OCTLB_YMIN;
+      } else if (zp) {
+        face = OCTLB_ZMAX;
+      } else if (zm) {
+        face = OCTLB_ZMIN;
+      }
+      const int direction = face / 2, orientation = (face & 1) ? 1 : -1;
+      if (direction == 0) {
+        v.NormalGrad(ix, iy, iz, 0, orientation, G[0]);
+        v.TangGrad(ix, iy, iz, 0, 1, 0, G[1]);
+        v.TangGrad(ix, iy, iz, 0, 0, 1, G[2]);
+      } else if (direction == 1) {
+        v.TangGrad(ix, iy, iz, 1, 0, 0, G[0]);
+        v.NormalGrad(ix, iy, iz, 1, orientation, G[1]);
+        v.TangGrad(ix, iy, iz, 0, 0, 1, G[2]);
+      } else {
+        v.TangGrad(ix, iy, iz, 1, 0, 0, G[0]);
+        v.TangGrad(ix, iy, iz, 0, 1, 0, G[1]);
+        v.NormalGrad(ix, iy, iz, 2, orientation, G[2]);
+      }
+    }
+    v.ComputeU(ix, iy, iz, myu);
+  }
+
+  const T omega = static_cast<T>(sc.omega);
+  for (int ph = 0; ph < 3; ++ph) {
+    const int begin = ph == 0 ? 0 : ph_end[ph - 1], end = ph_end[ph];
+    if (begin == end) continue;  // uniform: the counts are per block
+    if (myph == ph) {
+      FdLds<T> v{rho_pre, rho_post, ux, uy, uz, slot, &a.sc, my};
+      T rho;
+      if (fc == 3) {
+        const T r100 = v.ComputeRho(ix - xn, iy, iz);
+        const T r010 = v.ComputeRho(ix, iy - yn, iz);
+        const T r001 = v.ComputeRho(ix, iy, iz - zn);
+        const T r200 = v.ComputeRho(ix - 2 * xn, iy, iz);
+        const T r020 = v.ComputeRho(ix, iy - 2 * yn, iz);
+        const T r002 = v.ComputeRho(ix, iy, iz - 2 * zn);
+        rho = T(4) / T(9) * (r001 + r010 + r100) - T(1) / T(9) * (r002 + r020 + r200);
+      } else if (fc == 2) {
+        int plane, n1, n2;
+        if (xn != 0 && yn != 0) {
+          plane = 2;
+          n1 = xn;
+          n2 = yn;
+        } else if (xn != 0 && zn != 0) {
+          plane = 1;
+          n1 = xn;
+          n2 = zn;
+        } else {
+          plane = 0;
+          n1 = yn;
+          n2 = zn;
+        }
+        const int d1 = (plane + 1) % 3, d2 = (plane + 2) % 3;
+        T rs[4];
+        const int st[4][2] = {{1, 0}, {0, 1}, {2, 0}, {0, 2}};
+        for (int q = 0; q < 4; ++q) {
+          int o[3] = {0, 0, 0};
+          o[d1] = -n1 * st[q][0];
+          o[d2] = -n2 * st[q][1];
+          rs[q] = v.ComputeRho(ix + o[0], iy + o[1], iz + o[2]);
+        }
+        rho = T(2) / T(3) * (rs[1] + rs[0]) - T(1) / T(6) * (rs[3] + rs[2]);
+      } else {
+        rho = v.ComputeRho(ix, iy, iz);
+      }
+      T pi[6];
+      PiFromGrad(rho, omega, G[0], G[1], G[2], pi);
+      crho[tid] = rho;
+      cusq[tid] = myu[0] * myu[0] + myu[1] * myu[1] + myu[2] * myu[2];
+      cu[tid][0] = myu[0];
+      cu[tid][1] = myu[1];
+      cu[tid][2] = myu[2];
+#pragma unroll
+      for (int q = 0; q < 6; ++q) cpi[tid][q] = pi[q];
+    }
+    __syncthreads();
+    // WriteFromPi (:446-453) as (cell, population) tasks
+    const int tasks = (end - begin) * kQ;
+    for (int w = tid; w < tasks; w += kTileCells) {
+      const int e = begin + w / kQ, i = w % kQ;
+      const T f = FdPopulation<T>(i, crho[e], cu[e], cusq[e], cpi[e]);
+      fbuf[e][i] = f;
+      cur[static_cast<size_t>(i) * kTileCells + a.fd_list[static_cast<size_t>(fb) * kTileCells + e]] = f;
+    }
+    __syncthreads();
+    if (myph == ph) {
+      const int k = kind[my];
+      const T pu[3] = {ux[my], uy[my], uz[my]};
+      rho_post[my] = FdRhoOf(k, fbuf[tid], pu, sc, ix, iy, iz);
+    }
+    __syncthreads();
   }
 }
 
@@ -1023,6 +1220,7 @@ __global__ void KCouplerInit(CoupleArgs<T> a, int npts) {
 // of one coarse level (a later point writing the same fine cell wins).
 template <typename T>
 __global__ void KProlong(CoupleArgs<T> a, const int* list, int count, int half) {
+  BumpStep(a.step, a.step_next);
   const int j = static_cast<int>(blockIdx.x * blockDim.x + threadIdx.x);
   if (j >= count) return;
   const int p = list[j];
@@ -1045,31 +1243,60 @@ __global__ void KProlong(CoupleArgs<T> a, const int* list, int count, int half) 
 
 // LevelCoupler::restrict (level_coupler.cpp:613-654) over the deduplicated
 // points of one fine level (a later point writing the same coarse cell wins).
+// Restriction (level_coupler.cpp:613-654).  A 32-lane group per coupling
+// point: lane j < 19 reads the macroscopic state of the point's fine cell
+// (j = 0) or of its D3Q19 neighbour j when that lies inside the fine block;
+// after a barrier lane q < 19 averages f_neq population q over those cells in
+// the reference's neighbour order and writes population q of the coarse cell.
+constexpr int kRestrictGroups = 8;
+
 template <typename T>
-__global__ void KRestrict(CoupleArgs<T> a, const int* list, int count) {
-  const int j = static_cast<int>(blockIdx.x * blockDim.x + threadIdx.x);
-  if (j >= count) return;
-  const int p = list[j];
-  const int* r = a.plan + 12 * p;
-  const int cl = r[9], fl = r[10];
-  T* cb = (a.par.v[cl] ? a.pops1 : a.pops0) + static_cast<size_t>(r[0]) * kQ * kTileCells;
-  const T* fb = (a.par.v[fl] ? a.pops1 : a.pops0) + static_cast<size_t>(r[4]) * kQ * kTileCells;
-  const int fi = r[5], fj = r[6], fk = r[7];
-  T fm[kMacro];
-  ReadMacro(fb, kTileCells, (fi * kTile + fj) * kTile + fk, fm);
-  T sum[kQ];
-  for (int q = 0; q < kQ; ++q) sum[q] = T(0);
+__global__ __launch_bounds__(256) void KRestrict(CoupleArgs<T> a, const int* list, int count) {
+  __shared__ T fneq[kRestrictGroups][kQ][kQ + 1];
+  __shared__ T mac[kRestrictGroups][4];
+  BumpStep(a.step, a.step_next);
+  const int g = threadIdx.x >> 5, lane = threadIdx.x & 31;
+  const int j = static_cast<int>(blockIdx.x) * kRestrictGroups + g;
+  const bool active = j < count && lane < kQ;
+  int fi = 0, fj = 0, fk = 0;
+  const int* r = nullptr;
+  if (active) {
+    r = a.plan + 12 * list[j];
+    fi = r[5];
+    fj = r[6];
+    fk = r[7];
+    const int x = fi + Cx(lane), y = fj + Cy(lane), z = fk + Cz(lane);
+    if (x >= 0 && x < kTile && y >= 0 && y < kTile && z >= 0 && z < kTile) {
+      const T* fb = (a.par.v[r[10]] ? a.pops1 : a.pops0) + static_cast<size_t>(r[4]) * kQ * kTileCells;
+      T m[kMacro];
+      ReadMacro(fb, kTileCells, (x * kTile + y) * kTile + z, m);
+#pragma unroll
+      for (int q = 0; q < kQ; ++q) fneq[g][lane][q] = m[4 + q];
+      if (lane == 0) {
+        mac[g][0] = m[0];
+        mac[g][1] = m[1];
+        mac[g][2] = m[2];
+        mac[g][3] = m[3];
+      }
+    }
+  }
+  __syncthreads();
+  if (!active) return;
+  const int q = lane;
+  T sum = T(0);
   int cnt = 0;
   for (int jp = 1; jp < kQ; ++jp) {
     const int x = fi + Cx(jp), y = fj + Cy(jp), z = fk + Cz(jp);
     if (x < 0 || x >= kTile || y < 0 || y >= kTile || z < 0 || z >= kTile) continue;
-    T nm[kMacro];
-    ReadMacro(fb, kTileCells, (x * kTile + y) * kTile + z, nm);
-    for (int q = 0; q < kQ; ++q) sum[q] += nm[4 + q];
+    sum += fneq[g][jp][q];
     ++cnt;
   }
-  for (int q = 0; q < kQ; ++q) fm[4 + q] = (fm[4 + q] + sum[q]) / static_cast<T>(cnt + 1);
-  WriteMacro(cb, kTileCells, (r[1] * kTile + r[2]) * kTile + r[3], fm, a.restrict_scale);
+  const T avg = (fneq[g][0][q] + sum) / static_cast<T>(cnt + 1);
+  const T u[3] = {mac[g][1], mac[g][2], mac[g][3]};
+  const T usqr = NormSqr(u);
+  T* cb = (a.par.v[r[9]] ? a.pops1 : a.pops0) + static_cast<size_t>(r[0]) * kQ * kTileCells;
+  cb[static_cast<size_t>(q) * kTileCells + (r[1] * kTile + r[2]) * kTile + r[3]] =
+      Feq(q, mac[g][0], u, usqr) + a.restrict_scale * avg;
 }
 
 // ------------------------------------------------------ drag and mass
@@ -1174,9 +1401,9 @@ void LaunchTile(const TileArgs<T>& a, hipStream_t s) {
       hipLaunchKernelGGL((KPadded<T, false>), dim3(a.ntiles), dim3(512), 0, s, a);
     }
   } else if (a.sc.bouzidi) {
-    hipLaunchKernelGGL((KBlock<T, true>), dim3(a.ntiles), dim3(512), 0, s, a);
+    hipLaunchKernelGGL((KBlock<T, true>), dim3(a.ntiles), dim3(1024), 0, s, a);
   } else {
-    hipLaunchKernelGGL((KBlock<T, false>), dim3(a.ntiles), dim3(512), 0, s, a);
+    hipLaunchKernelGGL((KBlock<T, false>), dim3(a.ntiles), dim3(1024), 0, s, a);
   }
 }
 
@@ -1191,7 +1418,9 @@ void LaunchStatsSequential(const double* val, const uint8_t* use, int count, dou
   hipLaunchKernelGGL(KStatsSequential, dim3(1), dim3(1), 0, s, val, use, count, avg);
 }
 
-void LaunchStepInc(int* step, hipStream_t s) { hipLaunchKernelGGL(KStepInc, dim3(1), dim3(1), 0, s, step); }
+void LaunchStepInc(const int* step, int* step_next, hipStream_t s) {
+  hipLaunchKernelGGL(KStepInc, dim3(1), dim3(1), 0, s, step, step_next);
+}
 
 template <typename T>
 void LaunchInitEq(T* pops, const uint8_t* kind, int nb, const SimConst& sc, double rho0,
@@ -1236,7 +1465,9 @@ void LaunchProlong(const CoupleArgs<T>& a, const int* list, int count, int half,
 
 template <typename T>
 void LaunchRestrict(const CoupleArgs<T>& a, const int* list, int count, hipStream_t s) {
-  if (count > 0) hipLaunchKernelGGL(KRestrict<T>, Grid(count, 64), dim3(64), 0, s, a, list, count);
+  if (count > 0)
+    hipLaunchKernelGGL(KRestrict<T>, dim3((count + kRestrictGroups - 1) / kRestrictGroups), dim3(32 * kRestrictGroups),
+                       0, s, a, list, count);
 }
 
 template <typename T>

@@ -105,6 +105,7 @@ struct TileArgs {
   int tiles_per_block;
   const double* avg_rho;  // previous step's ConstRho average
   const int* step;        // coarse-step counter (time of the inlet ramps)
+  int* step_next;         // non-null in the first kernel of a step: *step_next = *step + 1
   int want_stats;         // 0 none, 1 parallel partial sums, 2 reference order
   int snap_on;            // write the post-collide drag snapshot
   SimConst sc;
@@ -122,6 +123,9 @@ struct FdArgs {
   T* pops1;
   T* post;               // [entries][19]
   const int* step;
+  int* step_next;
+  const short* fd_list;  // blocked mode: [fd block][512] FD cells, plane|edge|corner order
+  const int4* fd_count;  // blocked mode: [fd block] end offsets of the three phases, total
   ParArray par;
   SimConst sc;
 };
@@ -135,6 +139,8 @@ struct CoupleArgs {
   ParArray par;
   T prolong_scale;
   T restrict_scale;
+  const int* step;
+  int* step_next;
 };
 
 template <typename T>
@@ -334,7 +340,7 @@ void LaunchTile(const TileArgs<T>& a, hipStream_t s);
 template <typename T>
 void LaunchStatsFinalize(const double* partial, int ntiles, double* avg, hipStream_t s);
 void LaunchStatsSequential(const double* val, const uint8_t* use, int count, double* avg, hipStream_t s);
-void LaunchStepInc(int* step, hipStream_t s);
+void LaunchStepInc(const int* step, int* step_next, hipStream_t s);
 template <typename T>
 void LaunchInitEq(T* pops, const uint8_t* kind, int nb, const SimConst& sc, double rho0,
                   const double* u0, uint32_t mask, hipStream_t s);
