@@ -136,6 +136,7 @@ struct octlb_dev {
   void* pops[2] = {nullptr, nullptr};
   void* shells[2] = {nullptr, nullptr};
   uint8_t* d_kind = nullptr;
+  uint8_t* d_rule = nullptr;  // per core cell: needs the per-cell stream rules
   BlockTab* d_bt = nullptr;
   Tile* d_tiles = nullptr;
   int* d_level = nullptr;
@@ -569,6 +570,32 @@ void octlb_dev::Prepare() {
       }
     }
   }
+  // ---- per-cell stream rules (block_lattice.cpp:579-674): a cell needs more
+  // than the plain pull when it is not kBulk, carries Bouzidi links, or pulls
+  // from an in-block kSolid / kBouzidi cell (halo slots count as kBulk)
+  std::vector<uint8_t> rule(static_cast<size_t>(nb) * ncore, 0);
+  if (!padded) {
+    for (int b = 0; b < nb; ++b) {
+      const uint8_t* kb = &kinds[static_cast<size_t>(b) * ncore];
+      bool plain = true;
+      for (int c = 0; c < ncore; ++c) {
+        const size_t cg = static_cast<size_t>(b) * ncore + c;
+        bool need = kb[c] != kBulk || qmask[cg] != 0;
+        const int ix = c / (n * n), iy = (c / n) % n, iz = c % n;
+        for (int i = 1; i < kQ && !need; ++i) {
+          const int x = ix - Cx(i), y = iy - Cy(i), z = iz - Cz(i);
+          if (x < 0 || x >= n || y < 0 || y >= n || z < 0 || z >= n) continue;
+          const int sk = kb[(x * n + y) * n + z];
+          need = sk == kSolid || sk == kBouzidi;
+        }
+        rule[cg] = need ? 1 : 0;
+        plain = plain && !need;
+      }
+      if (plain) bt[b].flags |= kFlagPlain;
+    }
+  }
+  d_rule = PAlloc<uint8_t>(rule.size());
+  Upload(d_rule, rule, stream);
   // ---- drag links: kBulk cells with an in-block kSolid/kBouzidi neighbour at
   // x + c_i, in the order of drag_postprocessor.h:77-110 (iz, iy, ix, iPop)
   std::vector<uint32_t> dmask(static_cast<size_t>(nb) * ncore, 0);
@@ -758,6 +785,7 @@ TileArgs<T> octlb_dev::MakeTileArgs(int l) {
   a.ntiles = tile_count[l];
   a.bt = d_bt;
   a.kind = d_kind;
+  a.rule = d_rule;
   a.src = static_cast<const T*>(pops[par[l]]);
   a.dst = static_cast<T*>(pops[par[l] ^ 1]);
   a.shell_src = static_cast<const T*>(shells[par[l]]);

@@ -191,6 +191,7 @@ __global__ __launch_bounds__(1024, BlockOccupancy<T>::kWavesPerSimd) void KBlock
 #pragma unroll
     for (int i = 0; i < kQ; ++i) f[i] = T(0);
   }
+  const bool rules = PERCELL && own && !(bflags & kFlagPlain) && a.rule[static_cast<size_t>(b) * ncore + tid] != 0;
   uint32_t qm = 0;
   const double* qv = a.qval;
   if (PERCELL && own && (bflags & kFlagLinks)) {
@@ -230,12 +231,19 @@ __global__ __launch_bounds__(1024, BlockOccupancy<T>::kWavesPerSimd) void KBlock
 
   if (own) {
     T* dst = a.dst + static_cast<size_t>(b) * kQ * ncore + tid;
+    if (!PERCELL || !rules) {
+      // plain pull: no rule of the per-cell path changes any population
 #pragma unroll
-    for (int i = 0; i < kQ; ++i) {
-      const int sx = cx - Cx(i), sy = cy - Cy(i), sz = cz - Cz(i);
-      const bool src_int = sx >= 0 && sx < n && sy >= 0 && sy < n && sz >= 0 && sz < n;
-      dst[static_cast<size_t>(i) * ncore] =
-          StreamPop<T, PERCELL>(i, plane[i], plane[Opp(i)], kbox, l, k, src_int, qm, qv);
+      for (int i = 0; i < kQ; ++i)
+        dst[static_cast<size_t>(i) * ncore] = plane[i][l - ((Cx(i) * kBox + Cy(i)) * kBox + Cz(i))];
+    } else {
+#pragma unroll
+      for (int i = 0; i < kQ; ++i) {
+        const int sx = cx - Cx(i), sy = cy - Cy(i), sz = cz - Cz(i);
+        const bool src_int = sx >= 0 && sx < n && sy >= 0 && sy < n && sz >= 0 && sz < n;
+        dst[static_cast<size_t>(i) * ncore] =
+            StreamPop<T, true>(i, plane[i], plane[Opp(i)], kbox, l, k, src_int, qm, qv);
+      }
     }
   } else if (l >= 0 && nb < 0 && bshell >= 0 && (bflags & kFlagShellLive)) {
     // padded-block torus pull of a halo slot that no exchange refills;
@@ -814,21 +822,20 @@ struct FdLds {
   const T* uy;
   const T* uz;
   const short* slot;
-  const SimConst* sc;
   int self;
 
   __device__ bool InDomain(int x, int y, int z) const {
-    const int n = sc->n;
+    constexpr int n = kTile;
     return x >= 0 && x < n && y >= 0 && y < n && z >= 0 && z < n;
   }
   __device__ bool Padding(int x, int y, int z) const {
-    const int n = sc->n, st = sc->h < 2 ? sc->h : 2;
+    constexpr int n = kTile, st = 1;  // blocked mode: 8^3 cells, halo 1 -> min(h, 2) = 1
     return !InDomain(x, y, z) && x >= -st && x < n + st && y >= -st && y < n + st && z >= -st &&
            z < n + st;
   }
   __device__ static int Mirror(int i, int n) { return i < 0 ? -i : (i >= n ? 2 * (n - 1) - i : i); }
   __device__ void ComputeU(int x, int y, int z, T* u) const {
-    const int n = sc->n;
+    constexpr int n = kTile;
     if (Padding(x, y, z)) {
       u[0] = u[1] = u[2] = T(0);
       return;
@@ -840,13 +847,13 @@ struct FdLds {
     u[2] = fz ? -uz[c] : uz[c];
   }
   __device__ T ComputeRho(int x, int y, int z) const {
-    const int n = sc->n;
+    constexpr int n = kTile;
     if (Padding(x, y, z)) return T(1);
     const int c = (Mirror(x, n) * n + Mirror(y, n)) * n + Mirror(z, n);
     return (slot[c] >= 0 && c < self) ? rho_post[c] : rho_pre[c];
   }
   __device__ void ComputeRhoU(int x, int y, int z, T& rho, T* u) const {
-    const int c = (x * sc->n + y) * sc->n + z;
+    const int c = (x * kTile + y) * kTile + z;
     rho = ComputeRho(x, y, z);
     u[0] = ux[c];
     u[1] = uy[c];
@@ -889,9 +896,7 @@ __constant__ int kCzT[kQ] = {0, 0, 0, -1, 0, 0, -1, 1, -1, 1, 0, 0, 1, 0, 0, 1, 
 __constant__ int kWDenT[kQ] = {3, 18, 18, 18, 36, 36, 36, 36, 36, 36, 18, 18, 18, 36, 36, 36, 36, 36, 36};
 
 template <typename T>
-__device__ inline T FdPopulation(int i, T rho, const T* u, T usqr, const T* pi) {
-  const int c[3] = {kCxT[i], kCyT[i], kCzT[i]};
-  const T t = T(1) / T(kWDenT[i]);
+__device__ inline T FdPopulation(const int* c, T t, T rho, const T* u, T usqr, const T* pi) {
   T cu = T(0);
   cu += T(c[0]) * u[0];
   cu += T(c[1]) * u[1];
@@ -937,7 +942,18 @@ __global__ __launch_bounds__(512) void KFdBlock(FdArgs<T> a, const int* fd_block
   __shared__ T crho[kMaxFdCells], cusq[kMaxFdCells];
   __shared__ T cu[kMaxFdCells][3], cpi[kMaxFdCells][6];
   __shared__ T fbuf[kMaxFdCells][kQ];
+  __shared__ short lst[kMaxFdCells];
+  __shared__ int ctab[kQ][3];
+  __shared__ T wtab[kQ];
   BumpStep(a.step, a.step_next);
+  if (threadIdx.x < kQ) {
+    // velocity set and weights for the (cell, population) task loop
+    const int i = threadIdx.x;
+    ctab[i][0] = kCxT[i];
+    ctab[i][1] = kCyT[i];
+    ctab[i][2] = kCzT[i];
+    wtab[i] = T(1) / T(kWDenT[i]);
+  }
   const int fb = blockIdx.x;
   const int b = fd_blocks[fb];
   const int tid = threadIdx.x;
@@ -1023,11 +1039,12 @@ __global__ __launch_bounds__(512) void KFdBlock(FdArgs<T> a, const int* fd_block
   int fc = 0, xn = 0, yn = 0, zn = 0, ix = 0, iy = 0, iz = 0;
   if (tid < nfd) {
     my = a.fd_list[static_cast<size_t>(fb) * kTileCells + tid];
+    lst[tid] = static_cast<short>(my);
     myph = tid < cnt.x ? 0 : (tid < cnt.y ? 1 : 2);
     ix = my >> 6;
     iy = (my >> 3) & 7;
     iz = my & 7;
-    FdLds<T> v{rho_pre, rho_post, ux, uy, uz, slot, &a.sc, my};
+    FdLds<T> v{rho_pre, rho_post, ux, uy, uz, slot, my};
     const int xm = ix == 0, xp = ix == kTile - 1, ym = iy == 0, yp = iy == kTile - 1, zm = iz == 0,
               zp = iz == kTile - 1;
     fc = xm + xp + ym + yp + zm + zp;
@@ -1100,7 +1117,7 @@ __global__ __launch_bounds__(512) void KFdBlock(FdArgs<T> a, const int* fd_block
     const int begin = ph == 0 ? 0 : ph_end[ph - 1], end = ph_end[ph];
     if (begin == end) continue;  // uniform: the counts are per block
     if (myph == ph) {
-      FdLds<T> v{rho_pre, rho_post, ux, uy, uz, slot, &a.sc, my};
+      FdLds<T> v{rho_pre, rho_post, ux, uy, uz, slot, my};
       T rho;
       if (fc == 3) {
         const T r100 = v.ComputeRho(ix - xn, iy, iz);
@@ -1153,9 +1170,9 @@ __global__ __launch_bounds__(512) void KFdBlock(FdArgs<T> a, const int* fd_block
     const int tasks = (end - begin) * kQ;
     for (int w = tid; w < tasks; w += kTileCells) {
       const int e = begin + w / kQ, i = w % kQ;
-      const T f = FdPopulation<T>(i, crho[e], cu[e], cusq[e], cpi[e]);
+      const T f = FdPopulation<T>(ctab[i], wtab[i], crho[e], cu[e], cusq[e], cpi[e]);
       fbuf[e][i] = f;
-      cur[static_cast<size_t>(i) * kTileCells + a.fd_list[static_cast<size_t>(fb) * kTileCells + e]] = f;
+      cur[static_cast<size_t>(i) * kTileCells + lst[e]] = f;
     }
     __syncthreads();
     if (myph == ph) {

@@ -62,6 +62,7 @@ enum : int {
   kFlagLinks = 1,      // block has Bouzidi links
   kFlagShellLive = 2,  // block's shell is read by its core: evolve and persist it
   kFlagDrag = 4,       // block has drag-snapshot links
+  kFlagPlain = 8,      // every cell streams by the plain pull (no per-cell rule applies)
 };
 
 struct BlockTab {
@@ -89,6 +90,7 @@ struct TileArgs {
   int ntiles;
   const BlockTab* bt;
   const uint8_t* kind;
+  const uint8_t* rule;    // per core cell: 1 = some pull-stream rule beyond the plain pull applies
   const T* src;
   T* dst;
   const T* shell_src;
