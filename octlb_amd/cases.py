@@ -161,11 +161,12 @@ def cell_centres(topo: abi.TopologyArrays):
 def cylinder_channel(bricks_x: int = 4, base_level: int = 2, fine: bool = True, n: int = 8,
                      bricks_yz: int = 1,
                      xc: float = 1.0, radius: float = 0.125, u_in: float = 0.04, tau: float = 0.55,
-                     refine_lo: float = None, refine_hi: float = None, ramp_end_t: float = 0.0) -> Case:
+                     refine_lo: float = None, refine_hi: float = None, ramp_end_t: float = 0.0,
+                     refine_y: tuple = (-1.0, 2.0)) -> Case:
     """Channel with a z-spanning cylinder on a 2-level brick forest.
 
     Coarse blocks at ``base_level`` fill [0,bricks_x]x[0,1]x[0,1]; blocks
-    overlapping x in [xc-2D, xc+6D] are refined once.  Inlet x-min =
+    overlapping x in [xc-2D, xc+6D] (and y in ``refine_y``) are refined once.  Inlet x-min =
     interpolated velocity (uniform u_in), outlet x-max = interpolated pressure
     (rho 1), lateral faces have no spec (default no-slip -> kBounceBack,
     bc_installer.h:44-58).  Cylinder cells are solid, its surface cells
@@ -179,7 +180,7 @@ def cylinder_channel(bricks_x: int = 4, base_level: int = 2, fine: bool = True, 
     hi = xc + 6.0 * d if refine_hi is None else refine_hi
     if fine:
         eps = 1e-9
-        f.refine_box([lo + eps, -1.0, -1.0, hi - eps, 2.0, 2.0], base_level + 1)
+        f.refine_box([lo + eps, refine_y[0] + eps, -1.0, hi - eps, refine_y[1] - eps, 2.0], base_level + 1)
         f.balance()
     topo = f.topology(n, 1)
     X, Y, Z, cw = cell_centres(topo)

@@ -266,6 +266,14 @@ struct octlb_dev {
   template <typename T>
   void Level(int l);
   template <typename T>
+  FdArgs<T> MakeFdArgs();
+  template <typename T>
+  void FlushFd();
+  template <typename T>
+  void Couple(CoupleArgs<T> c, const int* list, int count, int mode);
+  bool fd_pending = false;   // blocked mode: FD pass of the last streamed level not launched yet
+  bool fd_merge_ok = false;  // FD blocks and coupled blocks are disjoint
+  template <typename T>
   void CoarseStep();
   void Step();
   void Sync() { HIP_CHECK(hipStreamSynchronize(stream)); }
@@ -546,6 +554,8 @@ void octlb_dev::Prepare() {
   Sync();
   FreePrepared();
   nfd_blocks = 0;
+  fd_merge_ok = false;
+  fd_pending = false;
   d_fd_blocks = nullptr;
   d_fd_slot = nullptr;
   d_fd_list = nullptr;
@@ -571,8 +581,11 @@ void octlb_dev::Prepare() {
     }
   }
   // ---- per-cell stream rules (block_lattice.cpp:579-674): a cell needs more
-  // than the plain pull when it is not kBulk, carries Bouzidi links, or pulls
-  // from an in-block kSolid / kBouzidi cell (halo slots count as kBulk)
+  // than the plain pull when it is kSolid (not streamed), reflects on pull
+  // (velocity, pressure, Bouzidi kinds), carries Bouzidi links, or pulls from
+  // an in-block kSolid / kBouzidi cell (halo slots count as kBulk).  Wall
+  // kinds that act in the collide (bounce-back, moving wall, outflow) stream
+  // like kBulk.
   std::vector<uint8_t> rule(static_cast<size_t>(nb) * ncore, 0);
   if (!padded) {
     for (int b = 0; b < nb; ++b) {
@@ -580,7 +593,7 @@ void octlb_dev::Prepare() {
       bool plain = true;
       for (int c = 0; c < ncore; ++c) {
         const size_t cg = static_cast<size_t>(b) * ncore + c;
-        bool need = kb[c] != kBulk || qmask[cg] != 0;
+        bool need = kb[c] == kSolid || ReflectsOnPull(kb[c]) || qmask[cg] != 0;
         const int ix = c / (n * n), iy = (c / n) % n, iz = c % n;
         for (int i = 1; i < kQ && !need; ++i) {
           const int x = ix - Cx(i), y = iy - Cy(i), z = iz - Cz(i);
@@ -750,6 +763,13 @@ void octlb_dev::Prepare() {
       Upload(d_fd_list, lists, stream);
       d_fd_count = PAlloc<int4>(counts.size());
       Upload(d_fd_count, counts, stream);
+      // the FD pass may share a launch with a coupling step only if no FD
+      // block is read or written by a coupling point
+      std::vector<char> is_fd(nb, 0);
+      for (int b : fdb) is_fd[b] = 1;
+      fd_merge_ok = true;
+      for (int i = 0; i < npts && fd_merge_ok; ++i)
+        fd_merge_ok = !is_fd[plan[12 * i]] && !is_fd[plan[12 * i + 4]];
     }
   } else {
     fd_phase[0] = fd_phase[1] = fd_phase[2] = fd_phase[3] = 0;
@@ -865,26 +885,12 @@ void octlb_dev::Level(int l) {
     if (a.want_stats == 2) LaunchStatsSequential(d_stat_val, d_stat_use, (cnt / a.tiles_per_block) * ncore, d_avg, stream);
   }
   if (fd_phase[3] > 0) {
-    FdArgs<T> f{};
-    f.cells = d_fd_cells;
-    f.smap = d_fd_smap;
-    f.fd_index = d_fd_index;
-    f.level = d_level;
-    f.bt = d_bt;
-    f.kind = d_kind;
-    f.pops0 = static_cast<T*>(pops[0]);
-    f.pops1 = static_cast<T*>(pops[1]);
-    f.post = static_cast<T*>(d_fd_post);
-    f.step = d_step + step_slot;
-    f.fd_list = d_fd_list;
-    f.fd_count = d_fd_count;
-    f.step_next = TakeInc(padded ? fd_phase[3] > 0 : nfd_blocks > 0);
-    f.par = Par();
-    f.sc = sc;
     if (padded) {
+      FdArgs<T> f = MakeFdArgs<T>();
+      f.step_next = TakeInc(true);
       LaunchFd<T>(f, fd_phase, stream);
     } else {
-      LaunchFdBlocks<T>(f, d_fd_blocks, d_fd_slot, nfd_blocks, stream);
+      fd_pending = true;  // launched with the coupling step that follows, or alone
     }
   }
   if (l >= max_level) return;
@@ -899,17 +905,63 @@ void octlb_dev::Level(int l) {
   c.step = d_step + step_slot;
   const int npro = static_cast<int>(prolong_active[l].size());
   const int nres = static_cast<int>(restrict_active[l + 1].size());
-  c.par = Par();
-  c.step_next = TakeInc(npro > 0);
-  LaunchProlong<T>(c, d_prolong + prolong_off[l], npro, 1, stream);
+  Couple<T>(c, d_prolong + prolong_off[l], npro, 1);
   Level<T>(l + 1);
-  c.par = Par();
-  c.step_next = TakeInc(npro > 0);
-  LaunchProlong<T>(c, d_prolong + prolong_off[l], npro, 0, stream);
+  Couple<T>(c, d_prolong + prolong_off[l], npro, 0);
   Level<T>(l + 1);
+  Couple<T>(c, d_restrict + restrict_off[l + 1], nres, 2);
+}
+
+template <typename T>
+FdArgs<T> octlb_dev::MakeFdArgs() {
+  FdArgs<T> f{};
+  f.cells = d_fd_cells;
+  f.smap = d_fd_smap;
+  f.fd_index = d_fd_index;
+  f.level = d_level;
+  f.bt = d_bt;
+  f.kind = d_kind;
+  f.pops0 = static_cast<T*>(pops[0]);
+  f.pops1 = static_cast<T*>(pops[1]);
+  f.post = static_cast<T*>(d_fd_post);
+  f.step = d_step + step_slot;
+  f.fd_list = d_fd_list;
+  f.fd_count = d_fd_count;
+  f.par = Par();
+  f.sc = sc;
+  return f;
+}
+
+// Launches a deferred FD pass on its own.
+template <typename T>
+void octlb_dev::FlushFd() {
+  if (!fd_pending) return;
+  fd_pending = false;
+  FdArgs<T> f = MakeFdArgs<T>();
+  f.step_next = TakeInc(nfd_blocks > 0);
+  LaunchFdBlocks<T>(f, d_fd_blocks, d_fd_slot, nfd_blocks, stream);
+}
+
+// One coupling step (mode 1 = prolongation at half time, 0 = at full time,
+// 2 = restriction); a deferred FD pass rides in the same launch when its
+// blocks and the coupled blocks are disjoint.
+template <typename T>
+void octlb_dev::Couple(CoupleArgs<T> c, const int* list, int count, int mode) {
   c.par = Par();
-  c.step_next = TakeInc(nres > 0);
-  LaunchRestrict<T>(c, d_restrict + restrict_off[l + 1], nres, stream);
+  if (fd_pending && fd_merge_ok) {
+    fd_pending = false;
+    FdArgs<T> f = MakeFdArgs<T>();
+    f.step_next = TakeInc(true);
+    LaunchFdCouple<T>(f, d_fd_blocks, d_fd_slot, nfd_blocks, c, list, count, mode, stream);
+    return;
+  }
+  FlushFd<T>();
+  c.step_next = TakeInc(count > 0);
+  if (mode == 2) {
+    LaunchRestrict<T>(c, list, count, stream);
+  } else {
+    LaunchProlong<T>(c, list, count, mode, stream);
+  }
 }
 
 // One coarse step.  The step counter lives in two slots: this step's kernels
@@ -921,6 +973,7 @@ void octlb_dev::CoarseStep() {
   step_slot = par[0];
   inc_next = d_step + (step_slot ^ 1);
   Level<T>(0);
+  FlushFd<T>();
   if (inc_next) {
     LaunchStepInc(d_step + step_slot, inc_next, stream);
     inc_next = nullptr;

@@ -934,7 +934,7 @@ constexpr int kMaxFdCells = 6 * 36 + 12 * 6 + 8;  // face, edge and corner cells
 // reference's in-place (ix,iy,iz) sweep does), and the 19 populations of every
 // cell of the phase are written by all 512 threads as (cell, population) tasks.
 template <typename T>
-__global__ __launch_bounds__(512) void KFdBlock(FdArgs<T> a, const int* fd_blocks, const short* fd_slot) {
+__device__ inline void FdBlockBody(const FdArgs<T>& a, const int* fd_blocks, const short* fd_slot, int fb) {
   __shared__ T rho_pre[kTileCells], rho_post[kTileCells];
   __shared__ T ux[kTileCells], uy[kTileCells], uz[kTileCells];
   __shared__ short slot[kTileCells];
@@ -945,7 +945,6 @@ __global__ __launch_bounds__(512) void KFdBlock(FdArgs<T> a, const int* fd_block
   __shared__ short lst[kMaxFdCells];
   __shared__ int ctab[kQ][3];
   __shared__ T wtab[kQ];
-  BumpStep(a.step, a.step_next);
   if (threadIdx.x < kQ) {
     // velocity set and weights for the (cell, population) task loop
     const int i = threadIdx.x;
@@ -954,7 +953,6 @@ __global__ __launch_bounds__(512) void KFdBlock(FdArgs<T> a, const int* fd_block
     ctab[i][2] = kCzT[i];
     wtab[i] = T(1) / T(kWDenT[i]);
   }
-  const int fb = blockIdx.x;
   const int b = fd_blocks[fb];
   const int tid = threadIdx.x;
   const SimConst& sc = a.sc;
@@ -1185,6 +1183,12 @@ __global__ __launch_bounds__(512) void KFdBlock(FdArgs<T> a, const int* fd_block
 }
 
 template <typename T>
+__global__ __launch_bounds__(512) void KFdBlock(FdArgs<T> a, const int* fd_blocks, const short* fd_slot) {
+  BumpStep(a.step, a.step_next);
+  FdBlockBody<T>(a, fd_blocks, fd_slot, static_cast<int>(blockIdx.x));
+}
+
+template <typename T>
 __global__ void KFdCommit(FdArgs<T> a, int count) {
   const int e = static_cast<int>(blockIdx.x * blockDim.x + threadIdx.x);
   if (e >= count) return;
@@ -1236,9 +1240,7 @@ __global__ void KCouplerInit(CoupleArgs<T> a, int npts) {
 // ApplyProlongation (level_coupler.cpp:438-478) over the deduplicated points
 // of one coarse level (a later point writing the same fine cell wins).
 template <typename T>
-__global__ void KProlong(CoupleArgs<T> a, const int* list, int count, int half) {
-  BumpStep(a.step, a.step_next);
-  const int j = static_cast<int>(blockIdx.x * blockDim.x + threadIdx.x);
+__device__ inline void ProlongPoint(const CoupleArgs<T>& a, const int* list, int count, int half, int j) {
   if (j >= count) return;
   const int p = list[j];
   const int* r = a.plan + 12 * p;
@@ -1258,6 +1260,12 @@ __global__ void KProlong(CoupleArgs<T> a, const int* list, int count, int half) 
   }
 }
 
+template <typename T>
+__global__ void KProlong(CoupleArgs<T> a, const int* list, int count, int half) {
+  BumpStep(a.step, a.step_next);
+  ProlongPoint<T>(a, list, count, half, static_cast<int>(blockIdx.x * blockDim.x + threadIdx.x));
+}
+
 // LevelCoupler::restrict (level_coupler.cpp:613-654) over the deduplicated
 // points of one fine level (a later point writing the same coarse cell wins).
 // Restriction (level_coupler.cpp:613-654).  A 32-lane group per coupling
@@ -1267,13 +1275,14 @@ __global__ void KProlong(CoupleArgs<T> a, const int* list, int count, int half) 
 // the reference's neighbour order and writes population q of the coarse cell.
 constexpr int kRestrictGroups = 8;
 
-template <typename T>
-__global__ __launch_bounds__(256) void KRestrict(CoupleArgs<T> a, const int* list, int count) {
-  __shared__ T fneq[kRestrictGroups][kQ][kQ + 1];
-  __shared__ T mac[kRestrictGroups][4];
-  BumpStep(a.step, a.step_next);
+// Groups = threads per workgroup / 32; wg = this workgroup's index among the
+// restriction workgroups.
+template <typename T, int Groups>
+__device__ inline void RestrictBody(const CoupleArgs<T>& a, const int* list, int count, int wg) {
+  __shared__ T fneq[Groups][kQ][kQ + 1];
+  __shared__ T mac[Groups][4];
   const int g = threadIdx.x >> 5, lane = threadIdx.x & 31;
-  const int j = static_cast<int>(blockIdx.x) * kRestrictGroups + g;
+  const int j = wg * Groups + g;
   const bool active = j < count && lane < kQ;
   int fi = 0, fj = 0, fk = 0;
   const int* r = nullptr;
@@ -1314,6 +1323,32 @@ __global__ __launch_bounds__(256) void KRestrict(CoupleArgs<T> a, const int* lis
   T* cb = (a.par.v[r[9]] ? a.pops1 : a.pops0) + static_cast<size_t>(r[0]) * kQ * kTileCells;
   cb[static_cast<size_t>(q) * kTileCells + (r[1] * kTile + r[2]) * kTile + r[3]] =
       Feq(q, mac[g][0], u, usqr) + a.restrict_scale * avg;
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void KRestrict(CoupleArgs<T> a, const int* list, int count) {
+  BumpStep(a.step, a.step_next);
+  RestrictBody<T, kRestrictGroups>(a, list, count, static_cast<int>(blockIdx.x));
+}
+
+// FD boundary pass of the level just streamed, merged into one launch with
+// the coupling step that follows it (prolongation or restriction) when the
+// host has checked that the FD blocks and the coupled blocks are disjoint:
+// workgroups [0, nfd) run the FD pass, the rest the coupling points.
+constexpr int kMergedGroups = 16;  // 512 threads / 32-lane restriction groups
+
+template <typename T>
+__global__ __launch_bounds__(512) void KFdCouple(FdArgs<T> fa, const int* fd_blocks, const short* fd_slot, int nfd,
+                                                 CoupleArgs<T> ca, const int* list, int count, int mode) {
+  BumpStep(fa.step, fa.step_next);
+  const int w = static_cast<int>(blockIdx.x);
+  if (w < nfd) {
+    FdBlockBody<T>(fa, fd_blocks, fd_slot, w);
+  } else if (mode == 2) {
+    RestrictBody<T, kMergedGroups>(ca, list, count, w - nfd);
+  } else {
+    ProlongPoint<T>(ca, list, count, mode, (w - nfd) * 512 + static_cast<int>(threadIdx.x));
+  }
 }
 
 // ------------------------------------------------------ drag and mass
@@ -1481,6 +1516,16 @@ void LaunchProlong(const CoupleArgs<T>& a, const int* list, int count, int half,
 }
 
 template <typename T>
+void LaunchFdCouple(const FdArgs<T>& fa, const int* fd_blocks, const short* fd_slot, int nfd, const CoupleArgs<T>& ca,
+                    const int* list, int count, int mode, hipStream_t s) {
+  const int per = mode == 2 ? kMergedGroups : 512;
+  const int ncw = (count + per - 1) / per;
+  if (nfd + ncw <= 0) return;
+  hipLaunchKernelGGL(KFdCouple<T>, dim3(nfd + ncw), dim3(512), 0, s, fa, fd_blocks, fd_slot, nfd, ca, list, count,
+                     mode);
+}
+
+template <typename T>
 void LaunchRestrict(const CoupleArgs<T>& a, const int* list, int count, hipStream_t s) {
   if (count > 0)
     hipLaunchKernelGGL(KRestrict<T>, dim3((count + kRestrictGroups - 1) / kRestrictGroups), dim3(32 * kRestrictGroups),
@@ -1516,6 +1561,8 @@ void LaunchMass(const T* p0, const T* p1, const ParArray& par, const BlockTab* b
   template void LaunchCouplerInit<T>(const CoupleArgs<T>&, int, hipStream_t);                     \
   template void LaunchProlong<T>(const CoupleArgs<T>&, const int*, int, int, hipStream_t);        \
   template void LaunchRestrict<T>(const CoupleArgs<T>&, const int*, int, hipStream_t);            \
+  template void LaunchFdCouple<T>(const FdArgs<T>&, const int*, const short*, int, const CoupleArgs<T>&, \
+                                  const int*, int, int, hipStream_t);                              \
   template void LaunchDrag<T>(const DragArgs<T>&, int, const int*, int, double*, double*,         \
                               hipStream_t);                                                       \
   template void LaunchMass<T>(const T*, const T*, const ParArray&, const BlockTab*,               \

@@ -353,6 +353,11 @@ template <typename T>
 void LaunchFd(const FdArgs<T>& a, const int* phase_begin, hipStream_t s);
 template <typename T>
 void LaunchFdBlocks(const FdArgs<T>& a, const int* fd_blocks, const short* fd_slot, int nfd, hipStream_t s);
+// FD pass + the coupling step after it in one launch; mode 0/1 = prolongation
+// (full / half time), 2 = restriction.
+template <typename T>
+void LaunchFdCouple(const FdArgs<T>& fa, const int* fd_blocks, const short* fd_slot, int nfd, const CoupleArgs<T>& ca,
+                    const int* list, int count, int mode, hipStream_t s);
 template <typename T>
 void LaunchCouplerInit(const CoupleArgs<T>& a, int npts, hipStream_t s);
 template <typename T>

@@ -114,3 +114,16 @@ def test_graph_replay_matches_eager():
     g1.advance(2)
     g2.advance_graph(2)
     np.testing.assert_array_equal(g1.download_core(), g2.download_core())
+
+
+def test_fd_blocks_on_coarse_fine_interface():
+    """Fine region covering part of the inlet: FD blocks are also coupling
+    blocks, so the FD pass and the coupling steps run as separate launches."""
+    case = cases.cylinder_channel(bricks_x=2, base_level=1, fine=True, xc=0.5, radius=0.1,
+                                  refine_lo=0.0, refine_hi=0.75, refine_y=(0.0, 0.5))
+    assert case.topo.num_cf > 0
+    o, g = _run_both(case, 8)
+    a, b = g.download_core(), o.download_core()
+    assert rel_err(a, b) <= 1e-12
+    fd, fo = g.drag(case.drag_bbox), o.drag(case.drag_bbox)
+    assert np.abs(fd - fo).max() <= 1e-12 * max(np.abs(fo).max(), 1e-30)
