@@ -232,3 +232,86 @@ def cylinder_channel(bricks_x: int = 4, base_level: int = 2, fine: bool = True, 
                 const_rho=0, bouzidi=1, snapshot=1, specs=specs, kinds=kinds, link=link, q=q,
                 init_mask=1 << abi.KIND_BULK, drag_bbox=bbox, area=area, u_ref=u_in,
                 info={"fine_dx": fine_dx, "D": d, "material_fluid": fluid})
+
+
+# ------------------------------------------------------ synthetic AMR (C3)
+def gaussian_blobs(seed: int, count: int = 8):
+    """Centres U[0,1]^3 and widths sigma U[0.05, 0.15] of the blob field."""
+    rng = np.random.default_rng(seed)
+    return rng.random((count, 3)), rng.uniform(0.05, 0.15, count)
+
+
+def blob_field(points, centres, sigmas):
+    """phi(x) = sum_b exp(-|x - c_b|^2 / (2 sigma_b^2))."""
+    d2 = ((points[:, None, :] - centres[None, :, :]) ** 2).sum(-1)
+    return np.exp(-d2 / (2.0 * sigmas[None, :] ** 2)).sum(-1)
+
+
+def fourier_spectrum(seed: int, kmax: int = 4):
+    """Random complex amplitudes a_k on the integer wave vectors 0 < |k| <=
+    kmax, each projected onto the plane normal to k (Helmholtz projection:
+    every mode, hence the field, is divergence-free).  Returned dense on the
+    (2 kmax + 1)^3 lattice of k, zero outside the ball."""
+    rng = np.random.default_rng(seed)
+    r = np.arange(-kmax, kmax + 1)
+    KX, KY, KZ = np.meshgrid(r, r, r, indexing="ij")
+    K = np.stack([KX, KY, KZ], -1).astype(np.float64)
+    k2 = (K ** 2).sum(-1)
+    A = rng.normal(size=K.shape) + 1j * rng.normal(size=K.shape)
+    kk = np.where(k2[..., None] > 0, K / np.sqrt(np.maximum(k2, 1))[..., None], 0.0)
+    A = A - (A * kk).sum(-1, keepdims=True) * kk
+    A[(k2 == 0) | (k2 > kmax * kmax)] = 0.0
+    return A
+
+
+def fourier_velocity(topo: abi.TopologyArrays, A, kmax: int = 4):
+    """u(x) = Re sum_k a_k exp(2 pi i k.x) at every cell centre, evaluated by
+    contracting one axis at a time over each block's 8 x 8 x 8 centres."""
+    X, Y, Z, _ = cell_centres(topo)
+    r = np.arange(-kmax, kmax + 1)
+    xs, ys, zs = X[:, :, 0, 0], Y[:, 0, :, 0], Z[:, 0, 0, :]
+    Ex = np.exp(2j * np.pi * xs[..., None] * r)  # (nb, n, 2kmax+1)
+    Ey = np.exp(2j * np.pi * ys[..., None] * r)
+    Ez = np.exp(2j * np.pi * zs[..., None] * r)
+    t1 = np.einsum("bia,aecd->biecd", Ex, A, optimize=True)
+    t2 = np.einsum("bje,biecd->bijcd", Ey, t1, optimize=True)
+    u = np.einsum("bkc,bijcd->bijkd", Ez, t2, optimize=True).real
+    return u
+
+
+def synthetic_amr(bricks: int = 8, max_level: int = 2, frac: tuple = (0.5, 0.25), seed: int = 7,
+                  amp: float = 0.01, kmax: int = 4, tau: float = 0.6) -> Case:
+    """Unit cube of bricks^3 level-0 blocks refined where a seeded sum of 8
+    Gaussian blobs is largest: level-0 octants whose centre value lies in the
+    top frac[0] of the domain volume are refined, then level-1 octants
+    likewise until frac[1] of the volume is at level 2; 2:1 balanced.  The
+    initial velocity is a divergence-free random Fourier field (|k| <= kmax)
+    scaled to max |u| = amp, loaded as feq(1, u).  Walls on all six faces
+    (no spec: no-slip -> kBounceBack, bc_installer.h:44-58)."""
+    centres, sigmas = gaussian_blobs(seed)
+    f = Forest([0.0, 0.0, 0.0, 1.0, 1.0, 1.0], (bricks, bricks, bricks))
+    for lv in range(max_level):
+        t = f.topology(8, 1)
+        b = t.bounds
+        mid = 0.5 * (b[:, :3] + b[:, 3:])
+        phi = blob_field(mid, centres, sigmas)
+        vol = np.prod(b[:, 3:] - b[:, :3], axis=1)
+        at = t.level == lv
+        order = np.argsort(-phi[at])
+        target = frac[lv] if lv < len(frac) else frac[-1]
+        cum = np.cumsum(vol[at][order])
+        take = order[: int(np.searchsorted(cum, target)) + 1] if cum.size else order[:0]
+        mark = np.zeros(t.num_blocks, np.uint8)
+        mark[np.flatnonzero(at)[take]] = 1
+        f.refine_marked(mark)
+        f.balance()
+    topo = f.topology(8, 1)
+    kinds = np.zeros((topo.num_blocks, 8, 8, 8), np.uint8)
+    stamp_tree_boundary(kinds, topo.tree_boundary, [])
+    u = fourier_velocity(topo, fourier_spectrum(seed + 1, kmax), kmax)
+    u *= amp / np.sqrt((u ** 2).sum(-1)).max()
+    rho = np.ones(kinds.shape)
+    return Case(name="synthetic_amr", topo=topo, omega=1.0 / tau, handler=abi.HANDLER_CONCRETE,
+                const_rho=0, bouzidi=0, snapshot=0, specs=[], kinds=kinds,
+                init_field=(rho.reshape(-1), u.reshape(-1)),
+                info={"seed": seed, "bricks": bricks, "max_level": max_level, "frac": frac})

@@ -127,3 +127,14 @@ def test_fd_blocks_on_coarse_fine_interface():
     assert rel_err(a, b) <= 1e-12
     fd, fo = g.drag(case.drag_bbox), o.drag(case.drag_bbox)
     assert np.abs(fd - fo).max() <= 1e-12 * max(np.abs(fo).max(), 1e-30)
+
+
+def test_synthetic_three_level():
+    """Synthetic 3-level octree (Gaussian-blob refinement, Fourier initial
+    velocity, bounce-back walls), FP64 against the oracle."""
+    case = cases.synthetic_amr(bricks=2, max_level=2, frac=(0.5, 0.25), seed=7)
+    assert case.topo.max_level == 2
+    o, g = _run_both(case, 4)
+    a, b = g.download_core(), o.download_core()
+    assert rel_err(a, b) <= 1e-12
+    assert abs(g.mass() - o.mass()) <= 1e-12 * o.mass()
