@@ -1030,15 +1030,25 @@ __device__ inline void FdBlockBody(const FdArgs<T>& a, const int* fd_blocks, con
   // ---- gradients of the FD cell this thread owns
   const int4 cnt = a.fd_count[fb];
   const int ph_end[3] = {cnt.x, cnt.y, cnt.z};
-  const int nfd = cnt.z;
+  // plane, edge and corner cells start on separate waves, so the three
+  // stencil shapes run side by side instead of one after another in a wave
+  const int t1 = (cnt.x + 63) & ~63, t2 = (t1 + (cnt.y - cnt.x) + 63) & ~63;
+  int e = -1;
+  if (tid < cnt.x) {
+    e = tid;
+  } else if (tid >= t1 && tid < t1 + (cnt.y - cnt.x)) {
+    e = cnt.x + (tid - t1);
+  } else if (tid >= t2 && tid < t2 + (cnt.z - cnt.y)) {
+    e = cnt.y + (tid - t2);
+  }
   int my = -1, myph = 3;
   T G[3][3];
   T myu[3] = {T(0), T(0), T(0)};
   int fc = 0, xn = 0, yn = 0, zn = 0, ix = 0, iy = 0, iz = 0;
-  if (tid < nfd) {
-    my = a.fd_list[static_cast<size_t>(fb) * kTileCells + tid];
-    lst[tid] = static_cast<short>(my);
-    myph = tid < cnt.x ? 0 : (tid < cnt.y ? 1 : 2);
+  if (e >= 0) {
+    my = a.fd_list[static_cast<size_t>(fb) * kTileCells + e];
+    lst[e] = static_cast<short>(my);
+    myph = e < cnt.x ? 0 : (e < cnt.y ? 1 : 2);
     ix = my >> 6;
     iy = (my >> 3) & 7;
     iz = my & 7;
@@ -1155,28 +1165,28 @@ __device__ inline void FdBlockBody(const FdArgs<T>& a, const int* fd_blocks, con
       }
       T pi[6];
       PiFromGrad(rho, omega, G[0], G[1], G[2], pi);
-      crho[tid] = rho;
-      cusq[tid] = myu[0] * myu[0] + myu[1] * myu[1] + myu[2] * myu[2];
-      cu[tid][0] = myu[0];
-      cu[tid][1] = myu[1];
-      cu[tid][2] = myu[2];
+      crho[e] = rho;
+      cusq[e] = myu[0] * myu[0] + myu[1] * myu[1] + myu[2] * myu[2];
+      cu[e][0] = myu[0];
+      cu[e][1] = myu[1];
+      cu[e][2] = myu[2];
 #pragma unroll
-      for (int q = 0; q < 6; ++q) cpi[tid][q] = pi[q];
+      for (int q = 0; q < 6; ++q) cpi[e][q] = pi[q];
     }
     __syncthreads();
     // WriteFromPi (:446-453) as (cell, population) tasks
     const int tasks = (end - begin) * kQ;
     for (int w = tid; w < tasks; w += kTileCells) {
-      const int e = begin + w / kQ, i = w % kQ;
-      const T f = FdPopulation<T>(ctab[i], wtab[i], crho[e], cu[e], cusq[e], cpi[e]);
-      fbuf[e][i] = f;
-      cur[static_cast<size_t>(i) * kTileCells + lst[e]] = f;
+      const int te = begin + w / kQ, i = w % kQ;
+      const T f = FdPopulation<T>(ctab[i], wtab[i], crho[te], cu[te], cusq[te], cpi[te]);
+      fbuf[te][i] = f;
+      cur[static_cast<size_t>(i) * kTileCells + lst[te]] = f;
     }
     __syncthreads();
     if (myph == ph) {
       const int k = kind[my];
       const T pu[3] = {ux[my], uy[my], uz[my]};
-      rho_post[my] = FdRhoOf(k, fbuf[tid], pu, sc, ix, iy, iz);
+      rho_post[my] = FdRhoOf(k, fbuf[e], pu, sc, ix, iy, iz);
     }
     __syncthreads();
   }
