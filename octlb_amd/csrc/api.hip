@@ -272,6 +272,13 @@ struct octlb_dev {
   template <typename T>
   void Couple(CoupleArgs<T> c, const int* list, int count, int mode);
   bool fd_pending = false;   // blocked mode: FD pass of the last streamed level not launched yet
+  const int* fd_pend_idx = nullptr;  // its FD block indices (into d_fd_blocks) and count
+  int fd_pend_cnt = 0;
+  // per level: FD blocks of that level / of the other levels, as index runs in d_fd_idx
+  int* d_fd_idx = nullptr;
+  int fd_same_off[kMaxLevels] = {}, fd_same_cnt[kMaxLevels] = {};
+  int fd_other_off[kMaxLevels] = {}, fd_other_cnt[kMaxLevels] = {};
+  int fd_all_off = 0;
   bool fd_merge_ok = false;  // FD blocks and coupled blocks are disjoint
   template <typename T>
   void CoarseStep();
@@ -556,6 +563,9 @@ void octlb_dev::Prepare() {
   nfd_blocks = 0;
   fd_merge_ok = false;
   fd_pending = false;
+  fd_pend_cnt = 0;
+  d_fd_idx = nullptr;
+  for (int lv = 0; lv < kMaxLevels; ++lv) fd_same_cnt[lv] = fd_other_cnt[lv] = 0;
   d_fd_blocks = nullptr;
   d_fd_slot = nullptr;
   d_fd_list = nullptr;
@@ -763,6 +773,23 @@ void octlb_dev::Prepare() {
       Upload(d_fd_list, lists, stream);
       d_fd_count = PAlloc<int4>(counts.size());
       Upload(d_fd_count, counts, stream);
+      {
+        std::vector<int> idx;
+        for (int lv = 0; lv <= max_level; ++lv) {
+          fd_same_off[lv] = static_cast<int>(idx.size());
+          for (int k = 0; k < static_cast<int>(fdb.size()); ++k)
+            if (level[fdb[k]] == lv) idx.push_back(k);
+          fd_same_cnt[lv] = static_cast<int>(idx.size()) - fd_same_off[lv];
+          fd_other_off[lv] = static_cast<int>(idx.size());
+          for (int k = 0; k < static_cast<int>(fdb.size()); ++k)
+            if (level[fdb[k]] != lv) idx.push_back(k);
+          fd_other_cnt[lv] = static_cast<int>(idx.size()) - fd_other_off[lv];
+        }
+        fd_all_off = static_cast<int>(idx.size());
+        for (int k = 0; k < static_cast<int>(fdb.size()); ++k) idx.push_back(k);
+        d_fd_idx = PAlloc<int>(idx.size());
+        Upload(d_fd_idx, idx, stream);
+      }
       // the FD pass may share a launch with a coupling step only if no FD
       // block is read or written by a coupling point
       std::vector<char> is_fd(nb, 0);
@@ -861,6 +888,16 @@ void octlb_dev::Level(int l) {
   if (cnt > 0) {
     TileArgs<T> a = MakeTileArgs<T>(l);
     a.step_next = TakeInc(true);
+    if (!padded && fd_other_cnt[l] > 0) {
+      // this level's FD pass over boundary blocks of other levels does not
+      // depend on this level's update: it rides in the same launch
+      a.fd = MakeFdArgs<T>();
+      a.fd_blocks = d_fd_blocks;
+      a.fd_slot = d_fd_slot;
+      a.fd_idx = d_fd_idx + fd_other_off[l];
+      a.fd_cnt = fd_other_cnt[l];
+      a.fd_wgs = (fd_other_cnt[l] + 7) & ~7;  // keep the tiles' XCD placement
+    }
     const bool timed = timing && !capturing;
     if (timed) {
       while (ev.size() < ev_used + 2) {
@@ -890,7 +927,13 @@ void octlb_dev::Level(int l) {
       f.step_next = TakeInc(true);
       LaunchFd<T>(f, fd_phase, stream);
     } else {
-      fd_pending = true;  // launched with the coupling step that follows, or alone
+      // the FD pass over this level's own boundary blocks (all of them when
+      // the level has none to update) follows the update: it is launched
+      // with the coupling step that comes next, or alone
+      const bool split = cnt > 0;
+      fd_pend_idx = d_fd_idx + (split ? fd_same_off[l] : fd_all_off);
+      fd_pend_cnt = split ? fd_same_cnt[l] : nfd_blocks;
+      fd_pending = fd_pend_cnt > 0;
     }
   }
   if (l >= max_level) return;
@@ -938,8 +981,8 @@ void octlb_dev::FlushFd() {
   if (!fd_pending) return;
   fd_pending = false;
   FdArgs<T> f = MakeFdArgs<T>();
-  f.step_next = TakeInc(nfd_blocks > 0);
-  LaunchFdBlocks<T>(f, d_fd_blocks, d_fd_slot, nfd_blocks, stream);
+  f.step_next = TakeInc(true);
+  LaunchFdBlocks<T>(f, d_fd_blocks, d_fd_slot, fd_pend_idx, fd_pend_cnt, stream);
 }
 
 // One coupling step (mode 1 = prolongation at half time, 0 = at full time,
@@ -952,7 +995,7 @@ void octlb_dev::Couple(CoupleArgs<T> c, const int* list, int count, int mode) {
     fd_pending = false;
     FdArgs<T> f = MakeFdArgs<T>();
     f.step_next = TakeInc(true);
-    LaunchFdCouple<T>(f, d_fd_blocks, d_fd_slot, nfd_blocks, c, list, count, mode, stream);
+    LaunchFdCouple<T>(f, d_fd_blocks, d_fd_slot, fd_pend_idx, fd_pend_cnt, c, list, count, mode, stream);
     return;
   }
   FlushFd<T>();

@@ -111,6 +111,13 @@ __device__ __forceinline__ T StreamPop(int i, const T* pl, const T* po, const ui
 // 152 KB FP64; gfx950 gives a workgroup up to 160 KB).  After one barrier the
 // cell threads pull-stream and store, and apron threads of unexchanged slots
 // do the halo-shell torus pull (block_lattice.cpp:595-611).
+// FD boundary pass, defined with the FD kernels below.
+template <typename T>
+struct FdSmem;
+template <typename T>
+__device__ inline void FdBlockBody(const FdArgs<T>& a, const int* fd_blocks, const short* fd_slot, int fb,
+                                   FdSmem<T>& sm);
+
 // FP32: two workgroups per CU (2 x 77 KB LDS) need <= 64 VGPRs (8 waves per
 // SIMD); FP64: one workgroup per CU (153 KB LDS), 4 waves per SIMD.
 template <typename T>
@@ -127,7 +134,15 @@ __global__ __launch_bounds__(1024, BlockOccupancy<T>::kWavesPerSimd) void KBlock
   int* red_c = reinterpret_cast<int*>(red_s + 1024);
 
   BumpStep(a.step, a.step_next);
-  const int ti = XcdTile(blockIdx.x, a.ntiles);
+  if (static_cast<int>(blockIdx.x) < a.fd_wgs) {
+    // FD pass of a boundary block of another level: independent of this
+    // level's blocks, so it runs beside them instead of in a launch of its own
+    static_assert(sizeof(FdSmem<T>) <= sizeof(plane), "FD pass LDS must fit the box");
+    if (static_cast<int>(blockIdx.x) < a.fd_cnt)
+      FdBlockBody<T>(a.fd, a.fd_blocks, a.fd_slot, a.fd_idx[blockIdx.x], *reinterpret_cast<FdSmem<T>*>(&plane[0][0]));
+    return;
+  }
+  const int ti = XcdTile(static_cast<int>(blockIdx.x) - a.fd_wgs, a.ntiles);
   const int b = a.tiles[ti].block;
   const SimConst& sc = a.sc;
   constexpr int n = kTile, ncore = kTileCells;
@@ -933,18 +948,43 @@ constexpr int kMaxFdCells = 6 * 36 + 12 * 6 + 8;  // face, edge and corner cells
 // and corners extrapolate densities that earlier phases rewrote, as the
 // reference's in-place (ix,iy,iz) sweep does), and the 19 populations of every
 // cell of the phase are written by all 512 threads as (cell, population) tasks.
+// LDS of one FD block pass; carved from the caller's shared buffer so the
+// pass can also run inside the fused kernel's workgroups (aliasing its box).
 template <typename T>
-__device__ inline void FdBlockBody(const FdArgs<T>& a, const int* fd_blocks, const short* fd_slot, int fb) {
-  __shared__ T rho_pre[kTileCells], rho_post[kTileCells];
-  __shared__ T ux[kTileCells], uy[kTileCells], uz[kTileCells];
-  __shared__ short slot[kTileCells];
-  __shared__ uint8_t kind[kTileCells];
-  __shared__ T crho[kMaxFdCells], cusq[kMaxFdCells];
-  __shared__ T cu[kMaxFdCells][3], cpi[kMaxFdCells][6];
-  __shared__ T fbuf[kMaxFdCells][kQ];
-  __shared__ short lst[kMaxFdCells];
-  __shared__ int ctab[kQ][3];
-  __shared__ T wtab[kQ];
+struct FdSmem {
+  T rho_pre[kTileCells], rho_post[kTileCells];
+  T ux[kTileCells], uy[kTileCells], uz[kTileCells];
+  T crho[kMaxFdCells], cusq[kMaxFdCells];
+  T cu[kMaxFdCells][3], cpi[kMaxFdCells][6];
+  T fbuf[kMaxFdCells][kQ];
+  T wtab[kQ];
+  int ctab[kQ][3];
+  short slot[kTileCells];
+  short lst[kMaxFdCells];
+  uint8_t kind[kTileCells];
+};
+
+// One FD block pass by a workgroup of 512 or more threads (threads beyond
+// the block's 512 cells take part in the barriers and the task loop only).
+template <typename T>
+__device__ inline void FdBlockBody(const FdArgs<T>& a, const int* fd_blocks, const short* fd_slot, int fb,
+                                   FdSmem<T>& sm) {
+  T* rho_pre = sm.rho_pre;
+  T* rho_post = sm.rho_post;
+  T* ux = sm.ux;
+  T* uy = sm.uy;
+  T* uz = sm.uz;
+  short* slot = sm.slot;
+  uint8_t* kind = sm.kind;
+  T* crho = sm.crho;
+  T* cusq = sm.cusq;
+  T(*cu)[3] = sm.cu;
+  T(*cpi)[6] = sm.cpi;
+  T(*fbuf)[kQ] = sm.fbuf;
+  short* lst = sm.lst;
+  int(*ctab)[3] = sm.ctab;
+  T* wtab = sm.wtab;
+  const bool act = threadIdx.x < kTileCells;
   if (threadIdx.x < kQ) {
     // velocity set and weights for the (cell, population) task loop
     const int i = threadIdx.x;
@@ -957,7 +997,7 @@ __device__ inline void FdBlockBody(const FdArgs<T>& a, const int* fd_blocks, con
   const int tid = threadIdx.x;
   const SimConst& sc = a.sc;
   T* cur = (a.par.v[a.level[b]] ? a.pops1 : a.pops0) + static_cast<size_t>(b) * kQ * kTileCells;
-  {
+  if (act) {
     const int ix = tid >> 6, iy = (tid >> 3) & 7, iz = tid & 7;
     const int k = a.kind[static_cast<size_t>(b) * kTileCells + tid];
     kind[tid] = static_cast<uint8_t>(k);
@@ -994,7 +1034,7 @@ __device__ inline void FdBlockBody(const FdArgs<T>& a, const int* fd_blocks, con
     // ExtrapolatedU (:297-316): a pressure cell takes the velocity of the
     // first non-pressure cell inward along its face normal; only pressure
     // entries are rewritten, and walks only read non-pressure entries.
-    const int k = kind[tid];
+    const int k = act ? kind[tid] : kBulk;
     T pux = T(0), puy = T(0), puz = T(0);
     if (k == kPressure) {
       int x = tid >> 6, y = (tid >> 3) & 7, z = tid & 7;
@@ -1176,7 +1216,7 @@ __device__ inline void FdBlockBody(const FdArgs<T>& a, const int* fd_blocks, con
     __syncthreads();
     // WriteFromPi (:446-453) as (cell, population) tasks
     const int tasks = (end - begin) * kQ;
-    for (int w = tid; w < tasks; w += kTileCells) {
+    for (int w = tid; w < tasks; w += static_cast<int>(blockDim.x)) {
       const int te = begin + w / kQ, i = w % kQ;
       const T f = FdPopulation<T>(ctab[i], wtab[i], crho[te], cu[te], cusq[te], cpi[te]);
       fbuf[te][i] = f;
@@ -1193,9 +1233,11 @@ __device__ inline void FdBlockBody(const FdArgs<T>& a, const int* fd_blocks, con
 }
 
 template <typename T>
-__global__ __launch_bounds__(512) void KFdBlock(FdArgs<T> a, const int* fd_blocks, const short* fd_slot) {
+__global__ __launch_bounds__(512) void KFdBlock(FdArgs<T> a, const int* fd_blocks, const short* fd_slot,
+                                                const int* fd_idx) {
+  __shared__ FdSmem<T> sm;
   BumpStep(a.step, a.step_next);
-  FdBlockBody<T>(a, fd_blocks, fd_slot, static_cast<int>(blockIdx.x));
+  FdBlockBody<T>(a, fd_blocks, fd_slot, fd_idx[blockIdx.x], sm);
 }
 
 template <typename T>
@@ -1348,12 +1390,14 @@ __global__ __launch_bounds__(256) void KRestrict(CoupleArgs<T> a, const int* lis
 constexpr int kMergedGroups = 16;  // 512 threads / 32-lane restriction groups
 
 template <typename T>
-__global__ __launch_bounds__(512) void KFdCouple(FdArgs<T> fa, const int* fd_blocks, const short* fd_slot, int nfd,
-                                                 CoupleArgs<T> ca, const int* list, int count, int mode) {
+__global__ __launch_bounds__(512) void KFdCouple(FdArgs<T> fa, const int* fd_blocks, const short* fd_slot,
+                                                 const int* fd_idx, int nfd, CoupleArgs<T> ca, const int* list,
+                                                 int count, int mode) {
+  __shared__ FdSmem<T> sm;
   BumpStep(fa.step, fa.step_next);
   const int w = static_cast<int>(blockIdx.x);
   if (w < nfd) {
-    FdBlockBody<T>(fa, fd_blocks, fd_slot, w);
+    FdBlockBody<T>(fa, fd_blocks, fd_slot, fd_idx[w], sm);
   } else if (mode == 2) {
     RestrictBody<T, kMergedGroups>(ca, list, count, w - nfd);
   } else {
@@ -1463,9 +1507,9 @@ void LaunchTile(const TileArgs<T>& a, hipStream_t s) {
       hipLaunchKernelGGL((KPadded<T, false>), dim3(a.ntiles), dim3(512), 0, s, a);
     }
   } else if (a.sc.bouzidi) {
-    hipLaunchKernelGGL((KBlock<T, true>), dim3(a.ntiles), dim3(1024), 0, s, a);
+    hipLaunchKernelGGL((KBlock<T, true>), dim3(a.ntiles + a.fd_wgs), dim3(1024), 0, s, a);
   } else {
-    hipLaunchKernelGGL((KBlock<T, false>), dim3(a.ntiles), dim3(1024), 0, s, a);
+    hipLaunchKernelGGL((KBlock<T, false>), dim3(a.ntiles + a.fd_wgs), dim3(1024), 0, s, a);
   }
 }
 
@@ -1500,9 +1544,10 @@ void LaunchInitField(T* pops, int nb, const SimConst& sc, const double* rho, con
 }
 
 template <typename T>
-void LaunchFdBlocks(const FdArgs<T>& a, const int* fd_blocks, const short* fd_slot, int nfd, hipStream_t s) {
+void LaunchFdBlocks(const FdArgs<T>& a, const int* fd_blocks, const short* fd_slot, const int* fd_idx, int nfd,
+                    hipStream_t s) {
   if (nfd <= 0) return;
-  hipLaunchKernelGGL(KFdBlock<T>, dim3(nfd), dim3(kTileCells), 0, s, a, fd_blocks, fd_slot);
+  hipLaunchKernelGGL(KFdBlock<T>, dim3(nfd), dim3(kTileCells), 0, s, a, fd_blocks, fd_slot, fd_idx);
 }
 
 template <typename T>
@@ -1526,13 +1571,13 @@ void LaunchProlong(const CoupleArgs<T>& a, const int* list, int count, int half,
 }
 
 template <typename T>
-void LaunchFdCouple(const FdArgs<T>& fa, const int* fd_blocks, const short* fd_slot, int nfd, const CoupleArgs<T>& ca,
-                    const int* list, int count, int mode, hipStream_t s) {
+void LaunchFdCouple(const FdArgs<T>& fa, const int* fd_blocks, const short* fd_slot, const int* fd_idx, int nfd,
+                    const CoupleArgs<T>& ca, const int* list, int count, int mode, hipStream_t s) {
   const int per = mode == 2 ? kMergedGroups : 512;
   const int ncw = (count + per - 1) / per;
   if (nfd + ncw <= 0) return;
-  hipLaunchKernelGGL(KFdCouple<T>, dim3(nfd + ncw), dim3(512), 0, s, fa, fd_blocks, fd_slot, nfd, ca, list, count,
-                     mode);
+  hipLaunchKernelGGL(KFdCouple<T>, dim3(nfd + ncw), dim3(512), 0, s, fa, fd_blocks, fd_slot, fd_idx, nfd, ca, list,
+                     count, mode);
 }
 
 template <typename T>
@@ -1567,12 +1612,12 @@ void LaunchMass(const T* p0, const T* p1, const ParArray& par, const BlockTab* b
   template void LaunchInitField<T>(T*, int, const SimConst&, const double*, const double*,        \
                                    hipStream_t);                                                  \
   template void LaunchFd<T>(const FdArgs<T>&, const int*, hipStream_t);                           \
-  template void LaunchFdBlocks<T>(const FdArgs<T>&, const int*, const short*, int, hipStream_t);  \
+  template void LaunchFdBlocks<T>(const FdArgs<T>&, const int*, const short*, const int*, int, hipStream_t); \
   template void LaunchCouplerInit<T>(const CoupleArgs<T>&, int, hipStream_t);                     \
   template void LaunchProlong<T>(const CoupleArgs<T>&, const int*, int, int, hipStream_t);        \
   template void LaunchRestrict<T>(const CoupleArgs<T>&, const int*, int, hipStream_t);            \
-  template void LaunchFdCouple<T>(const FdArgs<T>&, const int*, const short*, int, const CoupleArgs<T>&, \
-                                  const int*, int, int, hipStream_t);                              \
+  template void LaunchFdCouple<T>(const FdArgs<T>&, const int*, const short*, const int*, int,     \
+                                  const CoupleArgs<T>&, const int*, int, int, hipStream_t);         \
   template void LaunchDrag<T>(const DragArgs<T>&, int, const int*, int, double*, double*,         \
                               hipStream_t);                                                       \
   template void LaunchMass<T>(const T*, const T*, const ParArray&, const BlockTab*,               \

@@ -85,6 +85,25 @@ struct ParArray {
 };
 
 template <typename T>
+struct FdArgs {
+  const int2* cells;     // (block, cell) per FD entry, phases plane|edge|corner
+  const int* smap;       // [fd block][ncore] -> entry or -1
+  const int* fd_index;   // block -> fd block or -1
+  const int* level;      // block -> level
+  const BlockTab* bt;
+  const uint8_t* kind;
+  T* pops0;
+  T* pops1;
+  T* post;               // [entries][19]
+  const int* step;
+  int* step_next;
+  const short* fd_list;  // blocked mode: [fd block][512] FD cells, plane|edge|corner order
+  const int4* fd_count;  // blocked mode: [fd block] end offsets of the three phases, total
+  ParArray par;
+  SimConst sc;
+};
+
+template <typename T>
 struct TileArgs {
   const Tile* tiles;
   int ntiles;
@@ -110,27 +129,17 @@ struct TileArgs {
   int* step_next;         // non-null in the first kernel of a step: *step_next = *step + 1
   int want_stats;         // 0 none, 1 parallel partial sums, 2 reference order
   int snap_on;            // write the post-collide drag snapshot
+  // FD passes of boundary blocks of other levels, run by the first fd_wgs
+  // workgroups of the launch (fd_cnt of them busy, the rest pad to 8)
+  FdArgs<T> fd;
+  const int* fd_blocks;
+  const short* fd_slot;
+  const int* fd_idx;
+  int fd_cnt;
+  int fd_wgs;
   SimConst sc;
 };
 
-template <typename T>
-struct FdArgs {
-  const int2* cells;     // (block, cell) per FD entry, phases plane|edge|corner
-  const int* smap;       // [fd block][ncore] -> entry or -1
-  const int* fd_index;   // block -> fd block or -1
-  const int* level;      // block -> level
-  const BlockTab* bt;
-  const uint8_t* kind;
-  T* pops0;
-  T* pops1;
-  T* post;               // [entries][19]
-  const int* step;
-  int* step_next;
-  const short* fd_list;  // blocked mode: [fd block][512] FD cells, plane|edge|corner order
-  const int4* fd_count;  // blocked mode: [fd block] end offsets of the three phases, total
-  ParArray par;
-  SimConst sc;
-};
 
 template <typename T>
 struct CoupleArgs {
@@ -352,12 +361,13 @@ void LaunchInitField(T* pops, int nb, const SimConst& sc, const double* rho, con
 template <typename T>
 void LaunchFd(const FdArgs<T>& a, const int* phase_begin, hipStream_t s);
 template <typename T>
-void LaunchFdBlocks(const FdArgs<T>& a, const int* fd_blocks, const short* fd_slot, int nfd, hipStream_t s);
+void LaunchFdBlocks(const FdArgs<T>& a, const int* fd_blocks, const short* fd_slot, const int* fd_idx, int nfd,
+                    hipStream_t s);
 // FD pass + the coupling step after it in one launch; mode 0/1 = prolongation
 // (full / half time), 2 = restriction.
 template <typename T>
-void LaunchFdCouple(const FdArgs<T>& fa, const int* fd_blocks, const short* fd_slot, int nfd, const CoupleArgs<T>& ca,
-                    const int* list, int count, int mode, hipStream_t s);
+void LaunchFdCouple(const FdArgs<T>& fa, const int* fd_blocks, const short* fd_slot, const int* fd_idx, int nfd,
+                    const CoupleArgs<T>& ca, const int* list, int count, int mode, hipStream_t s);
 template <typename T>
 void LaunchCouplerInit(const CoupleArgs<T>& a, int npts, hipStream_t s);
 template <typename T>
