@@ -270,10 +270,11 @@ struct octlb_dev {
   template <typename T>
   void FlushFd();
   template <typename T>
-  void Couple(CoupleArgs<T> c, const int* list, int count, int mode);
+  void Couple(CoupleArgs<T> c, const int* list, int count, int mode, int next_level);
   bool fd_pending = false;   // blocked mode: FD pass of the last streamed level not launched yet
   const int* fd_pend_idx = nullptr;  // its FD block indices (into d_fd_blocks) and count
   int fd_pend_cnt = 0;
+  int fd_pend_level = -1;            // level whose own FD blocks are pending, -1 = all levels
   // per level: FD blocks of that level / of the other levels, as index runs in d_fd_idx
   int* d_fd_idx = nullptr;
   int fd_same_off[kMaxLevels] = {}, fd_same_cnt[kMaxLevels] = {};
@@ -888,6 +889,17 @@ void octlb_dev::Level(int l) {
   if (cnt > 0) {
     TileArgs<T> a = MakeTileArgs<T>(l);
     a.step_next = TakeInc(true);
+    a.fd_rep_level = -1;
+    if (fd_pending) {
+      if (fd_pend_level >= 0 && fd_pend_level != l && fd_other_cnt[l] > 0) {
+        // a pass deferred past the coupling step: it runs first, in the
+        // same workgroups as this launch's pass over those blocks
+        a.fd_rep_level = fd_pend_level;
+        fd_pending = false;
+      } else {
+        FlushFd<T>();
+      }
+    }
     if (!padded && fd_other_cnt[l] > 0) {
       // this level's FD pass over boundary blocks of other levels does not
       // depend on this level's update: it rides in the same launch
@@ -933,6 +945,7 @@ void octlb_dev::Level(int l) {
       const bool split = cnt > 0;
       fd_pend_idx = d_fd_idx + (split ? fd_same_off[l] : fd_all_off);
       fd_pend_cnt = split ? fd_same_cnt[l] : nfd_blocks;
+      fd_pend_level = split ? l : -1;
       fd_pending = fd_pend_cnt > 0;
     }
   }
@@ -948,11 +961,11 @@ void octlb_dev::Level(int l) {
   c.step = d_step + step_slot;
   const int npro = static_cast<int>(prolong_active[l].size());
   const int nres = static_cast<int>(restrict_active[l + 1].size());
-  Couple<T>(c, d_prolong + prolong_off[l], npro, 1);
+  Couple<T>(c, d_prolong + prolong_off[l], npro, 1, l + 1);
   Level<T>(l + 1);
-  Couple<T>(c, d_prolong + prolong_off[l], npro, 0);
+  Couple<T>(c, d_prolong + prolong_off[l], npro, 0, l + 1);
   Level<T>(l + 1);
-  Couple<T>(c, d_restrict + restrict_off[l + 1], nres, 2);
+  Couple<T>(c, d_restrict + restrict_off[l + 1], nres, 2, -1);
 }
 
 template <typename T>
@@ -989,8 +1002,22 @@ void octlb_dev::FlushFd() {
 // 2 = restriction); a deferred FD pass rides in the same launch when its
 // blocks and the coupled blocks are disjoint.
 template <typename T>
-void octlb_dev::Couple(CoupleArgs<T> c, const int* list, int count, int mode) {
+void octlb_dev::Couple(CoupleArgs<T> c, const int* list, int count, int mode, int next_level) {
   c.par = Par();
+  // a pending pass over one level's FD blocks can wait for the next update
+  // launch when that launch is of another level (it then runs beside it)
+  const bool defer = fd_pending && fd_merge_ok && fd_pend_level >= 0 && next_level >= 0 &&
+                     next_level != fd_pend_level && tile_count[next_level] > 0 &&
+                     fd_other_cnt[next_level] > 0;
+  if (defer) {
+    c.step_next = TakeInc(count > 0);
+    if (mode == 2) {
+      LaunchRestrict<T>(c, list, count, stream);
+    } else {
+      LaunchProlong<T>(c, list, count, mode, stream);
+    }
+    return;
+  }
   if (fd_pending && fd_merge_ok) {
     fd_pending = false;
     FdArgs<T> f = MakeFdArgs<T>();

@@ -138,8 +138,12 @@ __global__ __launch_bounds__(1024, BlockOccupancy<T>::kWavesPerSimd) void KBlock
     // FD pass of a boundary block of another level: independent of this
     // level's blocks, so it runs beside them instead of in a launch of its own
     static_assert(sizeof(FdSmem<T>) <= sizeof(plane), "FD pass LDS must fit the box");
-    if (static_cast<int>(blockIdx.x) < a.fd_cnt)
-      FdBlockBody<T>(a.fd, a.fd_blocks, a.fd_slot, a.fd_idx[blockIdx.x], *reinterpret_cast<FdSmem<T>*>(&plane[0][0]));
+    if (static_cast<int>(blockIdx.x) < a.fd_cnt) {
+      const int fb = a.fd_idx[blockIdx.x];
+      const int reps = (a.fd_rep_level >= 0 && a.fd.level[a.fd_blocks[fb]] == a.fd_rep_level) ? 2 : 1;
+      for (int r = 0; r < reps; ++r)  // each pass ends on a barrier: the next one sees its stores
+        FdBlockBody<T>(a.fd, a.fd_blocks, a.fd_slot, fb, *reinterpret_cast<FdSmem<T>*>(&plane[0][0]));
+    }
     return;
   }
   const int ti = XcdTile(static_cast<int>(blockIdx.x) - a.fd_wgs, a.ntiles);

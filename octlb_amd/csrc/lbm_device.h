@@ -137,6 +137,7 @@ struct TileArgs {
   const int* fd_idx;
   int fd_cnt;
   int fd_wgs;
+  int fd_rep_level;       // blocks of this level get two passes (one deferred from earlier), or -1
   SimConst sc;
 };
 
