@@ -200,7 +200,10 @@ struct octlb_dev {
   long long tile_launches = 0;
   double tile_updates = 0.0;
   // ---- graphs (one per level-0 parity)
-  hipGraphExec_t gexec[2] = {nullptr, nullptr};
+  // captured coarse steps: [0..1] one step from level-0 parity p,
+  // [2..3] kGraphSteps steps (an even count, so parities come back)
+  static constexpr int kGraphSteps = 16;
+  hipGraphExec_t gexec[4] = {nullptr, nullptr, nullptr, nullptr};
 
   template <typename P>
   P* Alloc(size_t count) {
@@ -1394,31 +1397,46 @@ int octlb_dev_advance_graph(octlb_dev* d, int32_t steps) {
     CheckDev(d);
     if (steps < 0) throw ArgError("negative step count");
     EnsureReady(d);
-    for (int s = 0; s < steps; ++s) {
-      const int p0 = d->par[0];
-      if (!d->gexec[p0]) {
-        int saved[kMaxLevels];
-        std::memcpy(saved, d->par, sizeof(saved));
-        const int saved_slot = d->step_slot;
-        hipGraph_t g = nullptr;
-        d->capturing = true;
-        HIP_CHECK(hipStreamBeginCapture(d->stream, hipStreamCaptureModeThreadLocal));
+    // Replays captured hipGraphs: kGraphSteps coarse steps per launch while
+    // enough remain (one graph launch per coarse step would leave a launch gap
+    // between every step), single steps for the rest.
+    auto capture = [&](int slot, int count) {
+      int saved[kMaxLevels];
+      std::memcpy(saved, d->par, sizeof(saved));
+      const int saved_slot = d->step_slot;
+      hipGraph_t g = nullptr;
+      d->capturing = true;
+      HIP_CHECK(hipStreamBeginCapture(d->stream, hipStreamCaptureModeThreadLocal));
+      for (int k = 0; k < count; ++k) {
         if (d->prec == 64) {
           d->CoarseStep<double>();
         } else {
           d->CoarseStep<float>();
         }
-        HIP_CHECK(hipStreamEndCapture(d->stream, &g));
-        d->capturing = false;
-        std::memcpy(d->par, saved, sizeof(saved));
-        d->step_slot = saved_slot;
-        HIP_CHECK(hipGraphInstantiate(&d->gexec[p0], g, nullptr, nullptr, 0));
-        HIP_CHECK(hipGraphDestroy(g));
       }
+      HIP_CHECK(hipStreamEndCapture(d->stream, &g));
+      d->capturing = false;
+      std::memcpy(d->par, saved, sizeof(saved));
+      d->step_slot = saved_slot;
+      HIP_CHECK(hipGraphInstantiate(&d->gexec[slot], g, nullptr, nullptr, 0));
+      HIP_CHECK(hipGraphDestroy(g));
+    };
+    while (steps > 0) {
+      const int p0 = d->par[0];
+      if (steps >= octlb_dev::kGraphSteps) {
+        // an even number of coarse steps leaves every level's parity as it was
+        if (!d->gexec[2 + p0]) capture(2 + p0, octlb_dev::kGraphSteps);
+        HIP_CHECK(hipGraphLaunch(d->gexec[2 + p0], d->stream));
+        d->step += octlb_dev::kGraphSteps;
+        steps -= octlb_dev::kGraphSteps;
+        continue;
+      }
+      if (!d->gexec[p0]) capture(p0, 1);
       HIP_CHECK(hipGraphLaunch(d->gexec[p0], d->stream));
       d->par[0] ^= 1;  // finer levels step an even number of times per coarse step
       d->step_slot = d->par[0];
       ++d->step;
+      --steps;
     }
     d->Sync();
   });

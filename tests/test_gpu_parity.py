@@ -114,6 +114,11 @@ def test_graph_replay_matches_eager():
     g1.advance(2)
     g2.advance_graph(2)
     np.testing.assert_array_equal(g1.download_core(), g2.download_core())
+    # 16-step graphs (two replays) followed by single-step graphs
+    g1.advance(35)
+    g2.advance_graph(35)
+    np.testing.assert_array_equal(g1.download_core(), g2.download_core())
+    np.testing.assert_array_equal(g1.drag(case.drag_bbox), g2.drag(case.drag_bbox))
 
 
 def test_fd_blocks_on_coarse_fine_interface():
