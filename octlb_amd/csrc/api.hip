@@ -45,14 +45,8 @@ constexpr int kOpposite[6] = {1, 0, 3, 2, 5, 4};
 // Halo slot order of the blocked shell: padded (10^3) slots that are not core,
 // in padded linear order (same enumeration as the device apron table).
 std::vector<int> ShellSlots() {
-  std::vector<int> s;
-  for (int x = 0; x < kBox; ++x)
-    for (int y = 0; y < kBox; ++y)
-      for (int z = 0; z < kBox; ++z) {
-        const bool in = x >= 1 && x <= kTile && y >= 1 && y <= kTile && z >= 1 && z <= kTile;
-        if (!in) s.push_back((x * kBox + y) * kBox + z);
-      }
-  return s;
+  constexpr ApronTable t = MakeApron();
+  return std::vector<int>(t.idx, t.idx + kApron);
 }
 
 // Block that supplies halo cell (x,y,z) (core coordinates, one of them

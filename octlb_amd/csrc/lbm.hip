@@ -17,20 +17,6 @@
 namespace octlb {
 
 // ------------------------------------------------------------------ apron map
-struct ApronTable {
-  short idx[kApron];
-};
-constexpr ApronTable MakeApron() {
-  ApronTable t{};
-  int k = 0;
-  for (int x = 0; x < kBox; ++x)
-    for (int y = 0; y < kBox; ++y)
-      for (int z = 0; z < kBox; ++z) {
-        const bool in = x >= 1 && x <= kTile && y >= 1 && y <= kTile && z >= 1 && z <= kTile;
-        if (!in) t.idx[k++] = static_cast<short>((x * kBox + y) * kBox + z);
-      }
-  return t;
-}
 __constant__ ApronTable kApronTab = MakeApron();
 
 __device__ inline int EdgeSlot(int a1, int s1, int a2, int s2) {

@@ -31,6 +31,47 @@ constexpr int kBoxCells = kBox * kBox * kBox;  // 1000
 constexpr int kTileCells = kTile * kTile * kTile;
 constexpr int kApron = kBoxCells - kTileCells;  // 488
 constexpr int kMaxLevels = 16;
+// Apron slots (the halo of the 10^3 box, padded coordinates 0..9) in
+// workgroup-thread and shell-store order: the six faces x-, x+, y-, y+, z-,
+// z+ (64 slots each, the later in-plane axis fastest, so a wave reads one
+// neighbour block's boundary layer in its memory order), then the 12 edges
+// (8 slots each), then the 8 corners.
+struct ApronTable {
+  short idx[kApron];
+};
+constexpr ApronTable MakeApron() {
+  ApronTable t{};
+  int k = 0;
+  for (int f = 0; f < 6; ++f) {
+    const int axis = f / 2, p = (f & 1) ? kBox - 1 : 0;
+    const int o1 = axis == 0 ? 1 : 0, o2 = axis == 2 ? 1 : 2;
+    for (int a = 1; a <= kTile; ++a)
+      for (int c = 1; c <= kTile; ++c) {
+        int v[3] = {0, 0, 0};
+        v[axis] = p;
+        v[o1] = a;
+        v[o2] = c;
+        t.idx[k++] = static_cast<short>((v[0] * kBox + v[1]) * kBox + v[2]);
+      }
+  }
+  for (int a1 = 0; a1 < 3; ++a1)
+    for (int a2 = a1 + 1; a2 < 3; ++a2)
+      for (int s1 = 0; s1 < 2; ++s1)
+        for (int s2 = 0; s2 < 2; ++s2)
+          for (int r = 1; r <= kTile; ++r) {
+            int v[3] = {0, 0, 0};
+            v[a1] = s1 ? kBox - 1 : 0;
+            v[a2] = s2 ? kBox - 1 : 0;
+            v[3 - a1 - a2] = r;
+            t.idx[k++] = static_cast<short>((v[0] * kBox + v[1]) * kBox + v[2]);
+          }
+  for (int x = 0; x < 2; ++x)
+    for (int y = 0; y < 2; ++y)
+      for (int z = 0; z < 2; ++z)
+        t.idx[k++] = static_cast<short>(((x ? kBox - 1 : 0) * kBox + (y ? kBox - 1 : 0)) * kBox + (z ? kBox - 1 : 0));
+  return t;
+}
+
 constexpr int kMacro = 4 + kQ;  // rho, u[3], f_neq[19]
 
 // Resolved DomainBcSpec of one face (FindSpec, interpolated_velocity.h:38-45).
