@@ -223,10 +223,14 @@ def main():
         "dtype": "f32" if head == 32 else "f64",
         "data": "synthetic",
         "config": {
-            "workload": ("cylinder channel, 2-level AMR, coarse 128x32x32 in 8^3 blocks, fine box "
-                         "2D upstream to 6D downstream, D=16 fine cells, Bouzidi links, inlet "
-                         "u=0.04, pressure outlet, tau=0.55")
-            if args.workload == "cylinder" else "lid-driven cavity 64^3, tau=0.6, lid 0.05",
+            "workload": {
+                "cylinder": ("cylinder channel, 2-level AMR, coarse 128x32x32 in 8^3 blocks, fine box "
+                             "2D upstream to 6D downstream, D=16 fine cells, Bouzidi links, inlet "
+                             "u=0.04, pressure outlet, tau=0.55"),
+                "cavity": "lid-driven cavity 64^3, one padded block (halo 3), tau=0.6, lid 0.05",
+                "synthetic": ("unit cube, 8^3 level-0 bricks, 3 levels refined on a Gaussian-blob sum, "
+                              "divergence-free Fourier start, bounce-back walls, tau 0.6"),
+            }[args.workload],
             "blocks": case.topo.num_blocks,
             "levels": int(case.topo.max_level) + 1,
             "actual_updates_per_step": r["actual_per_step"],
