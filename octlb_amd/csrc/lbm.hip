@@ -6,12 +6,12 @@
 //               -> [ConstRho statistics] -> post-stream FD boundaries
 //               -> if l < max: prolong(half) advance(l+1) prolong(full)
 //                              advance(l+1) restrict(l+1)
-// The fused kernels keep the reference's phase semantics: they read the
-// post-stream state S of step n-1, collide the tile's cells and its one-cell
-// apron (neighbour cells are re-collided from their own pre-collide state;
-// halo slots the reference's ghost exchange does not refresh come from the
-// block's shell store), stream through LDS one population plane at a time
-// and write S of step n to the other buffer.
+// The fused kernel keeps the reference's phase semantics: it reads the
+// post-stream state S of step n-1, collides the block's cells and its
+// one-cell apron (neighbour cells are re-collided from their own pre-collide
+// state; halo slots the reference's ghost exchange does not refresh come
+// from the block's shell store), pull-streams through an LDS copy of the
+// whole 10^3 box and writes S of step n to the other buffer.
 #include "lbm_device.h"
 
 namespace octlb {
