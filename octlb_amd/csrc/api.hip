@@ -131,6 +131,9 @@ struct octlb_dev {
   void* shells[2] = {nullptr, nullptr};
   uint8_t* d_kind = nullptr;
   uint8_t* d_rule = nullptr;  // per core cell: needs the per-cell stream rules
+  int* d_pfull = nullptr;     // per core cell: active prolongation point targeting it, or -1
+  int pfull_fuse_level = -1;  // the next update launch of this level applies the full-time prolongation
+  bool pfull_fused = false;
   BlockTab* d_bt = nullptr;
   Tile* d_tiles = nullptr;
   int* d_level = nullptr;
@@ -617,6 +620,21 @@ void octlb_dev::Prepare() {
   }
   d_rule = PAlloc<uint8_t>(rule.size());
   Upload(d_rule, rule, stream);
+  // fine-side targets of the (deduplicated) prolongation points: the update
+  // launch just before a full-time prolongation applies it to its own cells
+  {
+    std::vector<int> pfull(static_cast<size_t>(nb) * ncore, -1);
+    if (!padded) {
+      for (int lv = 0; lv < max_level; ++lv)
+        for (int i : prolong_active[lv]) {
+          const int* r = &plan[12 * i];
+          pfull[static_cast<size_t>(r[4]) * ncore + (r[5] * n + r[6]) * n + r[7]] = i;
+          bt[r[4]].flags |= kFlagProlong;
+        }
+    }
+    d_pfull = PAlloc<int>(pfull.size());
+    Upload(d_pfull, pfull, stream);
+  }
   // ---- drag links: kBulk cells with an in-block kSolid/kBouzidi neighbour at
   // x + c_i, in the order of drag_postprocessor.h:77-110 (iz, iy, ix, iPop)
   std::vector<uint32_t> dmask(static_cast<size_t>(nb) * ncore, 0);
@@ -887,6 +905,24 @@ void octlb_dev::Level(int l) {
     TileArgs<T> a = MakeTileArgs<T>(l);
     a.step_next = TakeInc(true);
     a.fd_rep_level = -1;
+    if (pfull_fuse_level == l) {
+      const double tau = 1.0 / sc.omega;  // level_coupler.cpp:226-228
+      CoupleArgs<T> c{};
+      c.plan = d_plan;
+      c.pops0 = static_cast<T*>(pops[0]);
+      c.pops1 = static_cast<T*>(pops[1]);
+      c.prev = static_cast<T*>(d_prev);
+      c.prolong_scale = static_cast<T>((tau - 0.25) / tau);
+      c.par = Par();
+      c.par.v[l] ^= 1;  // the fine targets are in this launch's output buffer
+      a.pfull = d_pfull;
+      a.pc = c;
+      a.pc_list = d_prolong + prolong_off[l - 1];
+      a.pc_count = static_cast<int>(prolong_active[l - 1].size());
+      a.pc_wgs = ((a.pc_count + 1023) / 1024 + 7) & ~7;  // keep the tiles' XCD placement
+      pfull_fuse_level = -1;
+      pfull_fused = true;
+    }
     if (fd_pending) {
       if (fd_pend_level >= 0 && fd_pend_level != l && fd_other_cnt[l] > 0) {
         // a pass deferred past the coupling step: it runs first, in the
@@ -958,9 +994,19 @@ void octlb_dev::Level(int l) {
   c.step = d_step + step_slot;
   const int npro = static_cast<int>(prolong_active[l].size());
   const int nres = static_cast<int>(restrict_active[l + 1].size());
+  // When level l+1 is the finest and has no FD blocks of its own, its first
+  // update launch is the last thing before the full-time prolongation and
+  // does that prolongation itself (nothing reads the fine interface cells in
+  // between; FD blocks and coupled blocks are disjoint).
+  const bool fuse_full = !padded && l + 1 == max_level && npro > 0 && tile_count[l + 1] > 0 &&
+                         fd_same_cnt[l + 1] == 0 && (nfd_blocks == 0 || fd_merge_ok);
   Couple<T>(c, d_prolong + prolong_off[l], npro, 1, l + 1);
+  pfull_fuse_level = fuse_full ? l + 1 : -1;
+  pfull_fused = false;
   Level<T>(l + 1);
-  Couple<T>(c, d_prolong + prolong_off[l], npro, 0, l + 1);
+  if (!pfull_fused) Couple<T>(c, d_prolong + prolong_off[l], npro, 0, l + 1);
+  pfull_fused = false;
+  pfull_fuse_level = -1;
   Level<T>(l + 1);
   Couple<T>(c, d_restrict + restrict_off[l + 1], nres, 2, -1);
 }

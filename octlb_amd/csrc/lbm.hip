@@ -97,6 +97,15 @@ __device__ __forceinline__ T StreamPop(int i, const T* pl, const T* po, const ui
 // 152 KB FP64; gfx950 gives a workgroup up to 160 KB).  After one barrier the
 // cell threads pull-stream and store, and apron threads of unexchanged slots
 // do the halo-shell torus pull (block_lattice.cpp:595-611).
+// Lagrava macroscopic read / prolongation write, defined with the coupler below.
+template <typename T>
+__device__ inline void ReadMacro(const T* base, int plane, int c, T* m);
+template <typename T>
+__device__ inline void WriteMacro(T* base, int plane, int c, const T* m, T scaling);
+
+template <typename T>
+__device__ inline void ProlongPoint(const CoupleArgs<T>& a, const int* list, int count, int half, int j);
+
 // FD boundary pass, defined with the FD kernels below.
 template <typename T>
 struct FdSmem;
@@ -132,7 +141,15 @@ __global__ __launch_bounds__(1024, BlockOccupancy<T>::kWavesPerSimd) void KBlock
     }
     return;
   }
-  const int ti = XcdTile(static_cast<int>(blockIdx.x) - a.fd_wgs, a.ntiles);
+  if (static_cast<int>(blockIdx.x) < a.fd_wgs + a.pc_wgs) {
+    // LevelCoupler::apply_full_time (level_coupler.cpp:438-478) on the fine
+    // cells this launch's blocks leave unwritten
+    ProlongPoint<T>(a.pc, a.pc_list, a.pc_count, 0,
+                    (static_cast<int>(blockIdx.x) - a.fd_wgs) * static_cast<int>(blockDim.x) +
+                        static_cast<int>(threadIdx.x));
+    return;
+  }
+  const int ti = XcdTile(static_cast<int>(blockIdx.x) - a.fd_wgs - a.pc_wgs, a.ntiles);
   const int b = a.tiles[ti].block;
   const SimConst& sc = a.sc;
   constexpr int n = kTile, ncore = kTileCells;
@@ -234,9 +251,15 @@ __global__ __launch_bounds__(1024, BlockOccupancy<T>::kWavesPerSimd) void KBlock
   }
   __syncthreads();
 
+  // a cell that the full-time prolongation of this launch overwrites is
+  // left to the coupling workgroups
+  const bool keep = !(own && a.pfull != nullptr && (bflags & kFlagProlong) &&
+                      a.pfull[static_cast<size_t>(b) * ncore + tid] >= 0);
   if (own) {
     T* dst = a.dst + static_cast<size_t>(b) * kQ * ncore + tid;
-    if (!PERCELL || !rules) {
+    if (!keep) {
+      // overwritten by the full-time prolongation of this launch
+    } else if (!PERCELL || !rules) {
       // plain pull: no rule of the per-cell path changes any population
 #pragma unroll
       for (int i = 0; i < kQ; ++i)
@@ -1497,9 +1520,9 @@ void LaunchTile(const TileArgs<T>& a, hipStream_t s) {
       hipLaunchKernelGGL((KPadded<T, false>), dim3(a.ntiles), dim3(512), 0, s, a);
     }
   } else if (a.sc.bouzidi) {
-    hipLaunchKernelGGL((KBlock<T, true>), dim3(a.ntiles + a.fd_wgs), dim3(1024), 0, s, a);
+    hipLaunchKernelGGL((KBlock<T, true>), dim3(a.ntiles + a.fd_wgs + a.pc_wgs), dim3(1024), 0, s, a);
   } else {
-    hipLaunchKernelGGL((KBlock<T, false>), dim3(a.ntiles + a.fd_wgs), dim3(1024), 0, s, a);
+    hipLaunchKernelGGL((KBlock<T, false>), dim3(a.ntiles + a.fd_wgs + a.pc_wgs), dim3(1024), 0, s, a);
   }
 }
 

@@ -104,6 +104,7 @@ enum : int {
   kFlagShellLive = 2,  // block's shell is read by its core: evolve and persist it
   kFlagDrag = 4,       // block has drag-snapshot links
   kFlagPlain = 8,      // every cell streams by the plain pull (no per-cell rule applies)
+  kFlagProlong = 16,   // block holds fine-side targets of a prolongation
 };
 
 struct BlockTab {
@@ -145,6 +146,19 @@ struct FdArgs {
 };
 
 template <typename T>
+struct CoupleArgs {
+  const int* plan;  // 12 ints per coupling point
+  T* pops0;
+  T* pops1;
+  T* prev;          // [points][kMacro]
+  ParArray par;
+  T prolong_scale;
+  T restrict_scale;
+  const int* step;
+  int* step_next;
+};
+
+template <typename T>
 struct TileArgs {
   const Tile* tiles;
   int ntiles;
@@ -179,22 +193,18 @@ struct TileArgs {
   int fd_cnt;
   int fd_wgs;
   int fd_rep_level;       // blocks of this level get two passes (one deferred from earlier), or -1
+  // full-time prolongation in the same launch, when it is the last update
+  // before it: pc_wgs workgroups after the FD ones run the coupling points,
+  // and the cell threads leave the targeted cells to them (pfull non-null)
+  const int* pfull;       // [block][512] active coupling point targeting the cell, or -1
+  CoupleArgs<T> pc;
+  const int* pc_list;
+  int pc_count;
+  int pc_wgs;
   SimConst sc;
 };
 
 
-template <typename T>
-struct CoupleArgs {
-  const int* plan;  // 12 ints per coupling point
-  T* pops0;
-  T* pops1;
-  T* prev;          // [points][kMacro]
-  ParArray par;
-  T prolong_scale;
-  T restrict_scale;
-  const int* step;
-  int* step_next;
-};
 
 template <typename T>
 struct DragArgs {

@@ -143,3 +143,21 @@ def test_synthetic_three_level():
     a, b = g.download_core(), o.download_core()
     assert rel_err(a, b) <= 1e-12
     assert abs(g.mass() - o.mass()) <= 1e-12 * o.mass()
+
+
+def test_two_level_cylinder_fine_region_interior():
+    """Fine level away from inlet and outlet (no FD blocks of its own): the
+    fine update launch before the full-time prolongation performs it."""
+    case = cases.cylinder_channel(bricks_x=2, base_level=1, fine=True, xc=1.0, radius=0.1,
+                                  refine_lo=0.6, refine_hi=1.4)
+    lv = case.topo.level
+    fd = [b for b in range(case.topo.num_blocks)
+          if ((case.kinds[b] == abi.KIND_VELOCITY_DIRICHLET) | (case.kinds[b] == abi.KIND_PRESSURE_DIRICHLET)).any()]
+    assert case.topo.num_cf > 0 and all(lv[b] == lv.min() for b in fd)
+    o, g = _run_both(case, 12)
+    assert rel_err(g.download_core(), o.download_core()) <= 1e-12
+    fd_, fo = g.drag(case.drag_bbox), o.drag(case.drag_bbox)
+    assert np.abs(fd_ - fo).max() <= 1e-12 * max(np.abs(fo).max(), 1e-30)
+    g2 = make_gpu(case)
+    g2.advance_graph(12)
+    np.testing.assert_array_equal(g2.download_core(), g.download_core())
