@@ -489,9 +489,27 @@ void octlb_dev::Build(const octlb_topology* t, const octlb_params* p) {
   tile_count.assign(kMaxLevels, 0);
   tiles.clear();
   const int tpa = padded ? (NP + kTile - 1) / kTile : 1;
+  // Blocks of a level are visited in lexicographic order of their origin, z
+  // fastest: z-neighbours (whose shared face is the strided one in the
+  // [19][x][y][z] layout) are then consecutive tiles on the same XCD, so the
+  // apron re-read hits L2 (4 % faster than octant order on the cylinder).
+  // With ConstRho the octant order is kept: the average is summed in tile
+  // order, which must be the reference's.
+  std::vector<int> order(nb);
+  for (int b = 0; b < nb; ++b) order[b] = b;
+  if (!padded && !sc.const_rho) {
+    std::stable_sort(order.begin(), order.end(), [&](int a, int c) {
+      const double* pa = &bounds[6 * static_cast<size_t>(a)];
+      const double* pc = &bounds[6 * static_cast<size_t>(c)];
+      if (pa[0] != pc[0]) return pa[0] < pc[0];
+      if (pa[1] != pc[1]) return pa[1] < pc[1];
+      return pa[2] < pc[2];
+    });
+  }
   for (int l = 0; l <= max_level; ++l) {
     tile_begin[l] = static_cast<int>(tiles.size());
-    for (int b = 0; b < nb; ++b) {
+    for (int bi = 0; bi < nb; ++bi) {
+      const int b = order[bi];
       if (level[b] != l) continue;
       for (int x = 0; x < tpa; ++x)
         for (int y = 0; y < tpa; ++y)
