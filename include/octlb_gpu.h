@@ -238,6 +238,25 @@ int octlb_dev_enable_timing(octlb_dev* dev, int32_t on);
 int octlb_dev_time_kernel(octlb_dev* dev, int32_t level, int32_t reps, double* avg_ms,
                           double* cell_updates);
 
+/* ---- Bouzidi wall fractions from a triangle soup ----------------------
+ * Replaces BouzidiLinkData::Build(forest, material, assembly)
+ * (bouzidi_link_data.cpp:139-233), run on the current HIP device.
+ *   bounds      6 doubles per block (x_min, y_min, z_min, x_max, y_max,
+ *               z_max) = OctreeForest::quadrant_bounds
+ *   material    MaterialKind bytes (0 fluid, 1 solid, 2 boundary) per block,
+ *               n^3 each in MaterialField order (k*n + j)*n + i
+ *   triangles   9 doubles each (v0, v1, v2) of the merged soup, parts in
+ *               assembly order (MergeAssemblySoup); normals are not used
+ * Output: *num_links = number of links, and when links and q are both
+ * non-null (capacity >= *num_links, else OCTLB_EINVAL) the links as
+ * (block, cell (ix*n + iy)*n + iz, population) int32 triples with their q,
+ * in (block, cell, population) order -- the entries of the dense q_frac
+ * array that differ from 1.0, ready for octlb_dev_set_links.  Pass null
+ * links and q to query the count. */
+int octlb_build_links(int32_t num_blocks, int32_t n, const double* bounds, const uint8_t* material,
+                      int64_t num_triangles, const double* triangles, int64_t capacity,
+                      int64_t* num_links, int32_t* links, double* q);
+
 #ifdef __cplusplus
 }
 #endif

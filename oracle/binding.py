@@ -61,6 +61,8 @@ def lib():
             "ol_coupler_init": ([vp], C.c_int),
             "ol_apply_prolongation": ([vp, C.c_int32, C.c_int32], C.c_int),
             "ol_apply_restriction": ([vp, C.c_int32], C.c_int),
+            "ol_build_links": ([C.c_int32, C.c_int32, f64p, u8p, C.c_int64, f64p, C.c_int64, i32p, f64p],
+                               C.c_int64),
         }
         for name, (a, r) in sig.items():
             fn = getattr(h, name)
@@ -189,6 +191,26 @@ class OracleSim:
 
     def apply_restriction(self, fine_level):
         _ok(lib().ol_apply_restriction(self._h, int(fine_level)), "restriction")
+
+
+def build_links(bounds, n, material, triangles):
+    """BouzidiLinkData::Build restated (links (block, cell, pop) int32 [L,3],
+    q float64 [L]); material in MaterialField order [block][k][j][i]."""
+    h = lib()
+    bounds = np.ascontiguousarray(bounds, np.float64).reshape(-1, 6)
+    material = np.ascontiguousarray(material, np.uint8)
+    tri = np.ascontiguousarray(triangles, np.float64).reshape(-1, 9)
+    nb = bounds.shape[0]
+    f64p, u8p, i32p = C.POINTER(C.c_double), C.POINTER(C.c_uint8), C.POINTER(C.c_int32)
+    args = (nb, n, bounds.ctypes.data_as(f64p), material.ctypes.data_as(u8p), tri.shape[0],
+            tri.ctypes.data_as(f64p) if tri.size else None)
+    cnt = h.ol_build_links(*args, 0, None, None)
+    if cnt < 0:
+        raise RuntimeError(h.ol_last_error().decode())
+    link = np.zeros((max(cnt, 1), 3), np.int32)
+    q = np.zeros(max(cnt, 1))
+    h.ol_build_links(*args, cnt, link.ctypes.data_as(i32p), q.ctypes.data_as(f64p))
+    return link[:cnt], q[:cnt]
 
 
 def setup(sim, case, precision_note=None):

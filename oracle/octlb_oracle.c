@@ -13,6 +13,7 @@
  */
 #include "octlb_oracle.h"
 
+#include <float.h>
 #include <math.h>
 #include <stdio.h>
 #include <stdlib.h>
@@ -1555,4 +1556,116 @@ int ol_apply_restriction(ol_sim* s, int32_t fine_level) {
   if (!s) return fail("null sim");
   apply_restriction(s, fine_level);
   return 0;
+}
+
+/* ------------------------------------------------------------------
+ * Bouzidi link build from a triangle soup (bouzidi_link_data.cpp:15-233).
+ * Every expression keeps the reference's operand order (built without FMA
+ * contraction). */
+
+/* ClosestPointOnSegment (bouzidi_link_data.cpp:19-40); std::clamp(t,0,1). */
+static void seg_closest(const double* p, const double* a, const double* b, double* out) {
+  const double abx = b[0] - a[0], aby = b[1] - a[1], abz = b[2] - a[2];
+  const double apx = p[0] - a[0], apy = p[1] - a[1], apz = p[2] - a[2];
+  const double ab2 = abx * abx + aby * aby + abz * abz;
+  double t = 0.0;
+  if (ab2 > 0.0) {
+    t = (apx * abx + apy * aby + apz * abz) / ab2;
+    t = t < 0.0 ? 0.0 : (1.0 < t ? 1.0 : t);
+  }
+  out[0] = a[0] + t * abx;
+  out[1] = a[1] + t * aby;
+  out[2] = a[2] + t * abz;
+}
+
+/* ClosestPointOnTriangle (bouzidi_link_data.cpp:42-76): the nearest of the
+ * three edge points, first one kept on ties; v0 if none is finite. */
+static void tri_closest(const double* tri, const double* p, double* out) {
+  double best = DBL_MAX;
+  out[0] = tri[0];
+  out[1] = tri[1];
+  out[2] = tri[2];
+  for (int e = 0; e < 3; ++e) {
+    const double* a = tri + 3 * e;
+    const double* b = tri + 3 * ((e + 1) % 3);
+    double c[3];
+    seg_closest(p, a, b, c);
+    const double dx = p[0] - c[0], dy = p[1] - c[1], dz = p[2] - c[2];
+    const double d2 = dx * dx + dy * dy + dz * dz;
+    if (d2 < best) {
+      best = d2;
+      out[0] = c[0];
+      out[1] = c[1];
+      out[2] = c[2];
+    }
+  }
+}
+
+int64_t ol_build_links(int32_t nb, int32_t n, const double* bounds, const uint8_t* material, int64_t ntri,
+                       const double* tris, int64_t cap, int32_t* links, double* q) {
+  if (nb < 0 || n < 1 || (nb > 0 && (!bounds || !material)) || ntri < 0 || (ntri > 0 && !tris)) {
+    fail("bad link-build arguments");
+    return -1;
+  }
+  if (ntri == 0) return 0; /* soup.empty(): no links */
+  const int64_t ncell = (int64_t)n * n * n;
+  double* dense = (double*)malloc(sizeof(double) * (size_t)ncell * Q);
+  int64_t count = 0;
+  for (int32_t b = 0; b < nb; ++b) {
+    const double* bb = bounds + 6 * (size_t)b;
+    const uint8_t* mat = material + (size_t)b * ncell;
+    for (int64_t e = 0; e < ncell * Q; ++e) dense[e] = -1.0;
+    const double dx = (bb[3] - bb[0]) / (double)n;
+    const double dy = (bb[4] - bb[1]) / (double)n;
+    const double dz = (bb[5] - bb[2]) / (double)n;
+    for (int k = 0; k < n; ++k)
+      for (int j = 0; j < n; ++j)
+        for (int i = 0; i < n; ++i) {
+          if (mat[((size_t)k * n + j) * n + i] != 0) continue; /* kFluid only */
+          /* CellCenter (bouzidi_link_data.cpp:85-94) */
+          const double p[3] = {bb[0] + ((double)i + 0.5) * dx, bb[1] + ((double)j + 0.5) * dy,
+                               bb[2] + ((double)k + 0.5) * dz};
+          double w[3] = {p[0], p[1], p[2]};
+          double best = DBL_MAX;
+          for (int64_t t = 0; t < ntri; ++t) {
+            double c[3];
+            tri_closest(tris + 9 * t, p, c);
+            const double d2 = (p[0] - c[0]) * (p[0] - c[0]) + (p[1] - c[1]) * (p[1] - c[1]) +
+                              (p[2] - c[2]) * (p[2] - c[2]);
+            if (d2 < best) {
+              best = d2;
+              w[0] = c[0];
+              w[1] = c[1];
+              w[2] = c[2];
+            }
+          }
+          for (int ip = 1; ip < Q; ++ip) {
+            const int* e = C[OPP[ip]];
+            const int ni = i + e[0], nj = j + e[1], nk = k + e[2];
+            if (ni < 0 || ni >= n || nj < 0 || nj >= n || nk < 0 || nk >= n) continue; /* kFluid */
+            const uint8_t m = mat[((size_t)nk * n + nj) * n + ni];
+            if (m != 1 && m != 2) continue; /* kSolid, kBoundary */
+            const double dir_x = (double)e[0] * dx, dir_y = (double)e[1] * dx, dir_z = (double)e[2] * dx;
+            const double dn2 = dir_x * dir_x + dir_y * dir_y + dir_z * dir_z;
+            if (dn2 <= 1.0e-30) continue;
+            const double tx = w[0] - p[0], ty = w[1] - p[1], tz = w[2] - p[2];
+            const double proj = (tx * dir_x + ty * dir_y + tz * dir_z) / dn2;
+            if (proj > 1.0e-8 && proj < 1.0 - 1.0e-8) dense[(((size_t)i * n + j) * n + k) * Q + ip] = proj;
+          }
+        }
+    for (int64_t c = 0; c < ncell; ++c)
+      for (int ip = 1; ip < Q; ++ip) {
+        const double v = dense[c * Q + ip];
+        if (v < 0.0) continue;
+        if (count < cap) {
+          links[3 * count] = b;
+          links[3 * count + 1] = (int32_t)c;
+          links[3 * count + 2] = ip;
+          q[count] = v;
+        }
+        ++count;
+      }
+  }
+  free(dense);
+  return count;
 }

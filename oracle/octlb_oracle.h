@@ -72,6 +72,17 @@ int ol_coupler_init(ol_sim* sim);
 int ol_apply_prolongation(ol_sim* sim, int32_t level, int32_t half_time);
 int ol_apply_restriction(ol_sim* sim, int32_t fine_level);
 
+/* BouzidiLinkData::Build (bouzidi_link_data.cpp:139-233): Bouzidi wall
+ * fractions of every fluid cell whose in-block neighbour x + c_opp(i) is
+ * solid-like, from the nearest point of a triangle soup (triangle edges
+ * only).  bounds: 6 per block; material: MaterialKind bytes per block in
+ * MaterialField order (k*n + j)*n + i; triangles: v0, v1, v2 (9 doubles)
+ * each.  Links come out in (block, (i*n + j)*n + k, pop) order, the dense
+ * q_frac order.  Returns the number of links; writes at most `capacity`. */
+int64_t ol_build_links(int32_t num_blocks, int32_t n, const double* bounds, const uint8_t* material,
+                       int64_t num_triangles, const double* triangles, int64_t capacity, int32_t* links,
+                       double* q);
+
 #ifdef __cplusplus
 }
 #endif
