@@ -123,6 +123,66 @@ def cpu_baseline(case, budget_s):
                       f"restatement, 1 thread, {dt:.1f} s"}
 
 
+FP64_VALU_PEAK_GFLOPS = 78600.0
+FLOPS_PER_CELL_TRIANGLE = 101  # 3 edges x (closest point 23 + distance 8) + distance 8
+
+
+def run_link_build(with_cpu, segments=512, rings=8, reps=3, cpu_blocks=24):
+    """Bouzidi link build from a triangle soup (BouzidiLinkData::Build) on
+    the headline forest: z-spanning cylinder of D = 16 fine cells as
+    2 * segments * rings triangles, material from the analytic inside test.
+    GPU: wall time of octlb_build_links (uploads, kernels, downloads).  CPU:
+    the oracle restatement (the reference's loop: every fluid cell against
+    every triangle), timed on `cpu_blocks` blocks and scaled per fluid cell."""
+    from octlb_amd import cases, geometry
+
+    case = cases.cylinder_channel(bricks_x=16, bricks_yz=4, base_level=0, fine=True, xc=1.0,
+                                  radius=0.125, u_in=0.04, tau=0.55)
+    topo = case.topo
+    mat = geometry.material_field(topo, lambda x, y, z: (x - 1.0) ** 2 + (y - 0.5) ** 2 < 0.125 ** 2)
+    tris = geometry.cylinder_soup(1.0, 0.5, 0.125, 0.0, 1.0, segments, rings)
+    link, q = geometry.build_links(topo.bounds, 8, mat, tris)  # warm-up (code object, allocator)
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        link, q = geometry.build_links(topo.bounds, 8, mat, tris)
+    gpu_s = (time.perf_counter() - t0) / reps
+    # candidate cells: fluid with a solid-like in-block neighbour
+    solid = mat != 0
+    pad = np.pad(solid, ((0, 0), (1, 1), (1, 1), (1, 1)), constant_values=False)
+    near = np.zeros_like(solid)
+    for i in range(1, 19):
+        ex, ey, ez = cases.C[i]
+        near |= pad[:, 1 + ez:9 + ez, 1 + ey:9 + ey, 1 + ex:9 + ex]
+    ncand = int((near & ~solid).sum())
+    gflops = ncand * tris.shape[0] * FLOPS_PER_CELL_TRIANGLE / gpu_s / 1e9
+    out = {
+        "workload": ("cylinder-channel forest (1,152 blocks of 8^3), z-spanning cylinder D=16 fine cells "
+                     f"as {tris.shape[0]} triangles ({segments} segments x {rings} rings)"),
+        "triangles": int(tris.shape[0]),
+        "candidate_cells": ncand,
+        "links": int(len(link)),
+        "gpu_ms": round(gpu_s * 1e3, 3),
+        "roofline": {"bound": "fp64 valu", "achieved": round(gflops, 1), "peak": FP64_VALU_PEAK_GFLOPS,
+                     "unit": "GFLOP/s", "frac": round(gflops / FP64_VALU_PEAK_GFLOPS, 4),
+                     "flops_per_cell_triangle": FLOPS_PER_CELL_TRIANGLE},
+    }
+    if with_cpu:
+        from oracle.binding import build_links as oracle_links
+
+        fluid = (~solid).reshape(topo.num_blocks, -1).sum(1)
+        order = np.argsort(-near.reshape(topo.num_blocks, -1).sum(1), kind="stable")[:cpu_blocks]
+        t0 = time.perf_counter()
+        oracle_links(topo.bounds[order], 8, mat[order], tris)
+        dt = time.perf_counter() - t0
+        per_cell = dt / max(int(fluid[order].sum()), 1)
+        out["cpu_baseline"] = {"est_full_s": round(per_cell * int(fluid.sum()), 2), "cores": 1, "kind": "port",
+                               "sample": (f"oracle restatement on the {cpu_blocks} blocks with most candidate "
+                                          f"cells ({int(fluid[order].sum())} fluid cells, {dt:.1f} s), "
+                                          "scaled per fluid cell to the whole forest")}
+        out["speedup_vs_cpu"] = round(out["cpu_baseline"]["est_full_s"] / gpu_s, 1)
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -136,6 +196,8 @@ def main():
     ap.add_argument("--no-synthetic", action="store_true",
                     help="skip the 3-level synthetic AMR measurement reported beside the headline")
     ap.add_argument("--synthetic-steps", type=int, default=40)
+    ap.add_argument("--no-links", action="store_true",
+                    help="skip the Bouzidi link-build measurement")
     ap.add_argument("--precision", default="both", choices=["both", "32", "64"])
     ap.add_argument("--calibrate-copy-mb", type=int, default=0,
                     help="run one device-to-device copy of this many MiB first (profiler "
@@ -205,6 +267,11 @@ def main():
                          "cells_per_launch": sd["cells_per_launch"],
                          "avg_launch_us": round(sd["avg_launch_us"], 3)},
         }
+    links = None
+    if not args.no_links and args.workload == "cylinder":
+        log("bouzidi link build")
+        links = run_link_build(not args.no_cpu)
+        log(f"  {links['links']} links from {links['triangles']} triangles in {links['gpu_ms']} ms")
     cpu = None
     if not args.no_cpu:
         log("cpu baseline")
@@ -259,6 +326,7 @@ def main():
         },
         "cpu_baseline": cpu,
         "synthetic_amr": synth,
+        "link_build": links,
         "drag_fp32": res[32]["drag"] if 32 in res else None,
         "drag_fp64": res[64]["drag"] if 64 in res else None,
     }
