@@ -238,6 +238,21 @@ int octlb_dev_enable_timing(octlb_dev* dev, int32_t on);
 int octlb_dev_time_kernel(octlb_dev* dev, int32_t level, int32_t reps, double* avg_ms,
                           double* cell_updates);
 
+/* ---- Ensembles ---------------------------------------------------------
+ * No counterpart in the reference, which runs one case per TimeLoop: many
+ * independent cases (members) in one device state, stepped by the same
+ * launches and the same hipGraph.  topo holds every member's blocks, member
+ * by member: blocks [member_begin[m], member_begin[m+1]) are member m, no
+ * block face joins two members, and every member is a single-level forest
+ * of 8^3 blocks with halo 1.  params[m] gives member m's relaxation rate and
+ * boundary specs; precision, handler, ConstRho, Bouzidi, snapshot and
+ * statistics mode must be the same for all members.  Every other call works
+ * on the whole ensemble; octlb_dev_average_rho reports member 0. */
+int octlb_dev_create_ensemble(const octlb_topology* topo, int32_t num_members, const int32_t* member_begin,
+                              const octlb_params* params, octlb_dev** out);
+/* ConstRho average of one member (TimeLoop::average_rho of that case). */
+int octlb_dev_member_average_rho(octlb_dev* dev, int32_t member, double* out);
+
 /* ---- Bouzidi wall fractions from a triangle soup ----------------------
  * Replaces BouzidiLinkData::Build(forest, material, assembly)
  * (bouzidi_link_data.cpp:139-233), run on the current HIP device.

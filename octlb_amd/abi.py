@@ -182,6 +182,9 @@ _SIGNATURES = {
     "octlb_dev_kernel_timing": ([_VP, _F64P, C.POINTER(C.c_int64), _F64P], C.c_int),
     "octlb_dev_enable_timing": ([_VP, C.c_int32], C.c_int),
     "octlb_dev_time_kernel": ([_VP, C.c_int32, C.c_int32, _F64P, _F64P], C.c_int),
+    "octlb_dev_create_ensemble": ([C.POINTER(Topology), C.c_int32, _I32P, C.POINTER(Params),
+                                   C.POINTER(C.c_void_p)], C.c_int),
+    "octlb_dev_member_average_rho": ([_VP, C.c_int32, _F64P], C.c_int),
     "octlb_build_links": ([C.c_int32, C.c_int32, _F64P, _U8P, C.c_int64, _F64P, C.c_int64,
                            C.POINTER(C.c_int64), _I32P, _F64P], C.c_int),
 }

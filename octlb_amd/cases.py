@@ -315,3 +315,83 @@ def synthetic_amr(bricks: int = 8, max_level: int = 2, frac: tuple = (0.5, 0.25)
                 const_rho=0, bouzidi=0, snapshot=0, specs=[], kinds=kinds,
                 init_field=(rho.reshape(-1), u.reshape(-1)),
                 info={"seed": seed, "bricks": bricks, "max_level": max_level, "frac": frac})
+
+
+# ------------------------------------------------------ cavity ensemble (C4)
+def cavity_blocked(cells: int, tau: float = 0.6, u_lid: float = 0.05, const_rho: int = 1,
+                   x0: float = 0.0) -> Case:
+    """Lid-driven cavity of cells^3 cells as a uniform forest of 8^3 blocks
+    (halo 1): interpolated-velocity (Skordos FD) walls on all six faces, lid
+    u_lid (lattice units) along x on the y-max face, ConstRhoBGK bulk when
+    const_rho, start at rest.  The domain is [x0, x0+1] x [0,1]^2, so members
+    of an ensemble can sit side by side."""
+    if cells % 8:
+        raise ValueError("cells must be a multiple of 8")
+    k = cells // 8
+    f = Forest([x0, 0.0, 0.0, x0 + 1.0, 1.0, 1.0], (k, k, k))
+    topo = f.topology(8, 1)
+    specs = [abi.make_spec(face, abi.BC_INTERP_VELOCITY) for face in
+             (abi.XMIN, abi.XMAX, abi.YMIN, abi.ZMIN, abi.ZMAX)]
+    specs.append(abi.make_spec(abi.YMAX, abi.BC_INTERP_VELOCITY, u_wall=(u_lid, 0.0, 0.0)))
+    kinds = np.zeros((topo.num_blocks, 8, 8, 8), np.uint8)
+    stamp_tree_boundary(kinds, topo.tree_boundary, specs)
+    return Case(name=f"cavity{cells}", topo=topo, omega=1.0 / tau, handler=abi.HANDLER_CONCRETE,
+                const_rho=const_rho, bouzidi=0, snapshot=0, specs=specs, kinds=kinds,
+                init_mask=0xFF, u_ref=u_lid, info={"cells": cells, "tau": tau, "u_lid": u_lid})
+
+
+@dataclass
+class Ensemble:
+    topo: abi.TopologyArrays
+    member_begin: np.ndarray   # int32 [members + 1]
+    members: list              # the member Cases, in block order
+    kinds: np.ndarray
+    fast_stats: int = 0
+
+    def params_for(self, precision: int = 64):
+        out = []
+        for c in self.members:
+            p = c.params(precision)
+            p.fast_stats = self.fast_stats
+            out.append(p)
+        return out
+
+
+def ensemble(members, fast_stats: int = 0) -> Ensemble:
+    """Concatenate single-level member cases into one block list (block ids
+    of member m offset by the blocks before it; no face joins members)."""
+    n, halo = members[0].topo.n, members[0].topo.halo
+    level, bounds, same, tb, begin = [], [], [], [], [0]
+    for c in members:
+        t = c.topo
+        if t.n != n or t.halo != halo or t.num_cf or t.max_level:
+            raise ValueError("ensemble members must be single-level forests of the same block shape")
+        off = begin[-1]
+        level.append(t.level)
+        bounds.append(t.bounds)
+        s = t.same_level.copy()
+        s[:, 0] += off
+        s[:, 2] += off
+        same.append(s)
+        b = t.tree_boundary.copy()
+        b[:, 0] += off
+        tb.append(b)
+        begin.append(off + t.num_blocks)
+    topo = abi.TopologyArrays(n, halo, np.concatenate(level), np.concatenate(bounds),
+                              np.concatenate(same), [], np.concatenate(tb))
+    kinds = np.concatenate([c.kinds for c in members])
+    return Ensemble(topo=topo, member_begin=np.array(begin, np.int32), members=list(members), kinds=kinds,
+                    fast_stats=fast_stats)
+
+
+def cavity_ensemble(count: int = 64, sizes=(32, 48, 64, 96), tau=(0.52, 0.8), lid=(0.02, 0.08), seed: int = 11,
+                    const_rho: int = 1, fast_stats: int = 1) -> Ensemble:
+    """`count` cavities with cells drawn from `sizes`, tau and lid speed
+    uniform in the given ranges, one seed; member m sits at x0 = 2m."""
+    rng = np.random.default_rng(seed)
+    ns = rng.choice(np.asarray(sizes), size=count)
+    taus = rng.uniform(tau[0], tau[1], size=count)
+    lids = rng.uniform(lid[0], lid[1], size=count)
+    members = [cavity_blocked(int(ns[m]), float(taus[m]), float(lids[m]), const_rho, x0=2.0 * m)
+               for m in range(count)]
+    return ensemble(members, fast_stats=fast_stats)

@@ -153,7 +153,14 @@ struct octlb_dev {
   double* d_stat_val = nullptr;
   uint8_t* d_stat_use = nullptr;
   bool fast_stats = false;
-  double* d_avg = nullptr;
+  double* d_avg = nullptr;  // ConstRho average per ensemble member
+  // ---- ensemble members (one member unless octlb_dev_create_ensemble)
+  int nmembers = 1;
+  std::vector<int> member_begin;  // [nmembers + 1] block offsets
+  std::vector<SimConst> scm;      // constants per member
+  SimConst* d_scm = nullptr;
+  int* d_stat_tiles = nullptr;    // [nmembers + 1] stats-level tile offsets per member
+  int* d_stat_cells = nullptr;    // [nmembers + 1] statistic slots per member
   int* d_step = nullptr;   // [2]: double-buffered coarse-step counter
   int step_slot = 0;       // slot holding the current step's time
   int* inc_next = nullptr; // next-step slot, handed to the step's first kernel
@@ -258,6 +265,7 @@ struct octlb_dev {
   }
 
   void Build(const octlb_topology* t, const octlb_params* p);
+  void SetMembers(int m, const int32_t* begin, const octlb_params* params);
   void Prepare();
   template <typename T>
   TileArgs<T> MakeTileArgs(int l);
@@ -547,6 +555,11 @@ void octlb_dev::Build(const octlb_topology* t, const octlb_params* p) {
   d_avg = Alloc<double>(1);
   const double one = 1.0;
   HIP_CHECK(hipMemcpyAsync(d_avg, &one, sizeof(double), hipMemcpyHostToDevice, stream));
+  nmembers = 1;
+  member_begin = {0, nb};
+  scm.assign(1, sc);
+  d_scm = Alloc<SimConst>(1);
+  Upload(d_scm, scm, stream);
   d_step = Alloc<int>(2);
   HIP_CHECK(hipMemsetAsync(d_step, 0, 2 * sizeof(int), stream));
   d_out = Alloc<double>(8);
@@ -853,9 +866,77 @@ void octlb_dev::Prepare() {
   d_dterm = PAlloc<double>(dlinks.size());
   d_duse = PAlloc<uint8_t>(dlinks.size());
   d_dblock_out = PAlloc<double>(3 * static_cast<size_t>(std::max(ndblocks, 1)));
+  // ConstRho statistics of the finest level, per member: the member's
+  // tiles are contiguous in tile order (octant order under ConstRho)
+  {
+    const int l = max_level;
+    const int tpb = padded ? ((NP + kTile - 1) / kTile) * ((NP + kTile - 1) / kTile) * ((NP + kTile - 1) / kTile) : 1;
+    std::vector<int> tiles_of(nmembers + 1, 0);
+    int prev = 0;
+    for (int i = 0; i < tile_count[l]; ++i) {
+      const int m = bt[tiles[tile_begin[l] + i].block].member;
+      if (m < prev) throw ArgError("ensemble members must be contiguous in block order");
+      prev = m;
+      tiles_of[m + 1] += 1;
+    }
+    std::vector<int> cells(nmembers + 1, 0);
+    for (int m = 0; m < nmembers; ++m) {
+      tiles_of[m + 1] += tiles_of[m];
+      cells[m + 1] = tiles_of[m + 1] / tpb * ncore;
+    }
+    d_stat_tiles = PAlloc<int>(nmembers + 1);
+    Upload(d_stat_tiles, tiles_of, stream);
+    d_stat_cells = PAlloc<int>(nmembers + 1);
+    Upload(d_stat_cells, cells, stream);
+  }
   Upload(d_bt, bt, stream);
   Sync();
   prepared = true;
+  InvalidateGraphs();
+}
+
+// Independent cases in one device state: blocks [begin[m], begin[m+1]) are
+// member m, whose relaxation rate and boundary specs come from params[m].
+// Members share the solver's structure (precision, handler, ConstRho,
+// Bouzidi, snapshot, statistics mode) and are single-level, with no block
+// face between members.
+void octlb_dev::SetMembers(int m, const int32_t* begin, const octlb_params* params) {
+  if (m < 1) throw ArgError("an ensemble needs at least one member");
+  if (padded) throw ArgError("ensembles need 8^3 blocks with halo 1");
+  if (begin[0] != 0 || begin[m] != nb) throw ArgError("member block ranges must cover the blocks");
+  for (int i = 0; i < m; ++i)
+    if (begin[i + 1] <= begin[i]) throw ArgError("member block ranges must be increasing and non-empty");
+  if (m > 1 && (max_level != 0 || !cf.empty())) throw ArgError("ensemble members must be single-level forests");
+  const octlb_params& p0 = params[0];
+  std::vector<SimConst> v(m, sc);
+  for (int i = 0; i < m; ++i) {
+    const octlb_params& p = params[i];
+    if (p.precision != p0.precision || p.handler != p0.handler || p.const_rho != p0.const_rho ||
+        p.stats_scope != p0.stats_scope || p.bouzidi != p0.bouzidi || p.snapshot != p0.snapshot ||
+        p.fast_stats != p0.fast_stats)
+      throw ArgError("ensemble members must share precision, handler, ConstRho, Bouzidi, snapshot and statistics mode");
+    if (!(p.omega > 0.0)) throw ArgError("omega must be positive");
+    v[i].omega = p.omega;
+    for (int f = 0; f < 6; ++f) v[i].spec[f] = ResolveSpec(&p, f);
+  }
+  std::vector<int> owner(nb);
+  for (int i = 0; i < m; ++i)
+    for (int b = begin[i]; b < begin[i + 1]; ++b) owner[b] = i;
+  for (size_t e = 0; e < same.size() / 3; ++e)
+    if (owner[same[3 * e]] != owner[same[3 * e + 2]]) throw ArgError("a block face joins two ensemble members");
+  nmembers = m;
+  member_begin.assign(begin, begin + m + 1);
+  scm = v;
+  sc = v[0];
+  for (int b = 0; b < nb; ++b) bt[b].member = owner[b];
+  d_scm = Alloc<SimConst>(m);
+  Upload(d_scm, scm, stream);
+  d_avg = Alloc<double>(m);
+  const std::vector<double> ones(m, 1.0);
+  Upload(d_avg, ones, stream);
+  Upload(d_bt, bt, stream);
+  Sync();
+  prepared = false;
   InvalidateGraphs();
 }
 
@@ -885,6 +966,7 @@ TileArgs<T> octlb_dev::MakeTileArgs(int l) {
     a.tiles_per_block = padded ? tpa * tpa * tpa : 1;
   }
   a.avg_rho = d_avg;
+  a.scm = d_scm;
   a.step = d_step + step_slot;
   a.want_stats = (sc.const_rho && l >= max_level) ? (fast_stats ? 1 : 2) : 0;
   a.snap_on = 1;
@@ -981,8 +1063,8 @@ void octlb_dev::Level(int l) {
       tile_updates += static_cast<double>(cnt / tpb) * static_cast<double>(ncore);
     }
     par[l] ^= 1;
-    if (a.want_stats == 1) LaunchStatsFinalize<T>(d_stats, cnt, d_avg, stream);
-    if (a.want_stats == 2) LaunchStatsSequential(d_stat_val, d_stat_use, (cnt / a.tiles_per_block) * ncore, d_avg, stream);
+    if (a.want_stats == 1) LaunchStatsFinalize<T>(d_stats, d_stat_tiles, nmembers, d_avg, stream);
+    if (a.want_stats == 2) LaunchStatsSequential(d_stat_val, d_stat_use, d_stat_cells, nmembers, d_avg, stream);
   }
   if (fd_phase[3] > 0) {
     if (padded) {
@@ -1042,6 +1124,7 @@ FdArgs<T> octlb_dev::MakeFdArgs() {
   f.pops1 = static_cast<T*>(pops[1]);
   f.post = static_cast<T*>(d_fd_post);
   f.step = d_step + step_slot;
+  f.scm = d_scm;
   f.fd_list = d_fd_list;
   f.fd_count = d_fd_count;
   f.par = Par();
@@ -1307,6 +1390,35 @@ int octlb_dev_create(const octlb_topology* topo, const octlb_params* params, oct
   }
   *out = d;
   return OCTLB_OK;
+}
+
+int octlb_dev_create_ensemble(const octlb_topology* topo, int32_t num_members, const int32_t* member_begin,
+                              const octlb_params* params, octlb_dev** out) {
+  if (out) *out = nullptr;
+  if (!topo || !params || !member_begin || !out) return SetError(OCTLB_EINVAL, "null argument");
+  octlb_dev* d = new (std::nothrow) octlb_dev();
+  if (!d) return SetError(OCTLB_ENOMEM, "host allocation failed");
+  const int rc = Guard([&] {
+    d->Build(topo, &params[0]);
+    d->SetMembers(num_members, member_begin, params);
+  });
+  if (rc != OCTLB_OK) {
+    d->FreeAll();
+    delete d;
+    return rc;
+  }
+  *out = d;
+  return OCTLB_OK;
+}
+
+int octlb_dev_member_average_rho(octlb_dev* d, int32_t member, double* out) {
+  return Guard([&] {
+    CheckDev(d);
+    if (!out) throw ArgError("null argument");
+    if (member < 0 || member >= d->nmembers) throw ArgError("member out of range");
+    d->Sync();
+    HIP_CHECK(hipMemcpy(out, d->d_avg + member, sizeof(double), hipMemcpyDeviceToHost));
+  });
 }
 
 int octlb_dev_destroy(octlb_dev* dev) {

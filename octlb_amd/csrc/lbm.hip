@@ -151,12 +151,12 @@ __global__ __launch_bounds__(1024, BlockOccupancy<T>::kWavesPerSimd) void KBlock
   }
   const int ti = XcdTile(static_cast<int>(blockIdx.x) - a.fd_wgs - a.pc_wgs, a.ntiles);
   const int b = a.tiles[ti].block;
-  const SimConst& sc = a.sc;
   constexpr int n = kTile, ncore = kTileCells;
   const int tid = threadIdx.x;
   const bool own = tid < ncore;
   const int ap = tid - ncore;  // apron slot (shell index) when !own
   const BlockTab& bt = a.bt[b];
+  const SimConst& sc = a.scm[bt.member];  // apron cells belong to the same member
   const int bflags = bt.flags;
   const int bshell = bt.shell;
 
@@ -227,7 +227,7 @@ __global__ __launch_bounds__(1024, BlockOccupancy<T>::kWavesPerSimd) void KBlock
   int scnt = 0;
   if (src_b >= 0) {
     const double t = static_cast<double>(*a.step);
-    const T avg = sc.const_rho ? static_cast<T>(*a.avg_rho) : T(1);
+    const T avg = sc.const_rho ? static_cast<T>(a.avg_rho[bt.member]) : T(1);
     const BlockTab& st = a.bt[src_b];
     const Origin og{st.origin[0], st.origin[1], st.origin[2], st.cw};
     CollideCell<T>(f, k, sc, og, cx, cy, cz, t, avg, own && a.want_stats != 0, ssum, scnt);
@@ -439,29 +439,33 @@ __global__ __launch_bounds__(512) void KPadded(TileArgs<T> a) {
 }
 
 // ------------------------------------------------------------ small kernels
-__global__ void KStatsFinalize(const double* partial, int ntiles, double* avg) {
-  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+// One thread per ensemble member.
+__global__ void KStatsFinalize(const double* partial, const int* tile_begin, int members, double* avg) {
+  const int m = static_cast<int>(blockIdx.x * blockDim.x + threadIdx.x);
+  if (m >= members) return;
   double s = 0.0, c = 0.0;
-  for (int i = 0; i < ntiles; ++i) {
+  for (int i = tile_begin[m]; i < tile_begin[m + 1]; ++i) {
     s += partial[2 * i];
     c += partial[2 * i + 1];
   }
-  *avg = c > 0.0 ? s / c : 1.0;  // CollideRhoStats::average (block_lattice.h:31)
+  avg[m] = c > 0.0 ? s / c : 1.0;  // CollideRhoStats::average (block_lattice.h:31)
 }
 
-// CollideRhoStats accumulated in the reference's cell order
-// (blocks of the level in order, cells (ix,iy,iz) lexicographic).
-__global__ void KStatsSequential(const double* val, const uint8_t* use, int count, double* avg) {
-  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+// CollideRhoStats accumulated in the reference's cell order (blocks of the
+// level in order, cells (ix,iy,iz) lexicographic); one thread per member.
+__global__ void KStatsSequential(const double* val, const uint8_t* use, const int* cell_begin, int members,
+                                 double* avg) {
+  const int m = static_cast<int>(blockIdx.x * blockDim.x + threadIdx.x);
+  if (m >= members) return;
   double s = 0.0;
   int c = 0;
-  for (int k = 0; k < count; ++k) {
+  for (int k = cell_begin[m]; k < cell_begin[m + 1]; ++k) {
     if (use[k]) {
       s += val[k];
       ++c;
     }
   }
-  *avg = c > 0 ? s / static_cast<double>(c) : 1.0;
+  avg[m] = c > 0 ? s / static_cast<double>(c) : 1.0;
 }
 
 __global__ void KStepInc(const int* step, int* step_next) {
@@ -1008,7 +1012,7 @@ __device__ inline void FdBlockBody(const FdArgs<T>& a, const int* fd_blocks, con
   }
   const int b = fd_blocks[fb];
   const int tid = threadIdx.x;
-  const SimConst& sc = a.sc;
+  const SimConst& sc = a.scm[a.bt[b].member];
   T* cur = (a.par.v[a.level[b]] ? a.pops1 : a.pops0) + static_cast<size_t>(b) * kQ * kTileCells;
   if (act) {
     const int ix = tid >> 6, iy = (tid >> 3) & 7, iz = tid & 7;
@@ -1529,12 +1533,13 @@ void LaunchTile(const TileArgs<T>& a, hipStream_t s) {
 static inline dim3 Grid(long long n, int bs) { return dim3(static_cast<unsigned>((n + bs - 1) / bs)); }
 
 template <typename T>
-void LaunchStatsFinalize(const double* partial, int ntiles, double* avg, hipStream_t s) {
-  hipLaunchKernelGGL(KStatsFinalize, dim3(1), dim3(1), 0, s, partial, ntiles, avg);
+void LaunchStatsFinalize(const double* partial, const int* tile_begin, int members, double* avg, hipStream_t s) {
+  hipLaunchKernelGGL(KStatsFinalize, Grid(members, 64), dim3(64), 0, s, partial, tile_begin, members, avg);
 }
 
-void LaunchStatsSequential(const double* val, const uint8_t* use, int count, double* avg, hipStream_t s) {
-  hipLaunchKernelGGL(KStatsSequential, dim3(1), dim3(1), 0, s, val, use, count, avg);
+void LaunchStatsSequential(const double* val, const uint8_t* use, const int* cell_begin, int members, double* avg,
+                           hipStream_t s) {
+  hipLaunchKernelGGL(KStatsSequential, Grid(members, 64), dim3(64), 0, s, val, use, cell_begin, members, avg);
 }
 
 void LaunchStepInc(const int* step, int* step_next, hipStream_t s) {
@@ -1619,7 +1624,7 @@ void LaunchMass(const T* p0, const T* p1, const ParArray& par, const BlockTab* b
 
 #define OCTLB_INSTANTIATE(T)                                                                      \
   template void LaunchTile<T>(const TileArgs<T>&, hipStream_t);                                   \
-  template void LaunchStatsFinalize<T>(const double*, int, double*, hipStream_t);                 \
+  template void LaunchStatsFinalize<T>(const double*, const int*, int, double*, hipStream_t);     \
   template void LaunchInitEq<T>(T*, const uint8_t*, int, const SimConst&, double, const double*,  \
                                 uint32_t, hipStream_t);                                           \
   template void LaunchInitField<T>(T*, int, const SimConst&, const double*, const double*,        \

@@ -111,7 +111,7 @@ struct BlockTab {
   int level;
   int shell;    // shell index or -1
   int flags;
-  int pad_;
+  int member;   // ensemble member whose constants (SimConst) the block uses
   int nbr[18];  // faces 0..5, edges 6..17; -1 = halo part not exchanged
   double origin[3];
   double cw;
@@ -139,6 +139,7 @@ struct FdArgs {
   T* post;               // [entries][19]
   const int* step;
   int* step_next;
+  const SimConst* scm;   // per ensemble member, indexed by BlockTab::member
   const short* fd_list;  // blocked mode: [fd block][512] FD cells, plane|edge|corner order
   const int4* fd_count;  // blocked mode: [fd block] end offsets of the three phases, total
   ParArray par;
@@ -179,7 +180,8 @@ struct TileArgs {
   double* stat_val;       // per-cell statistic in reference order (want_stats == 2)
   uint8_t* stat_use;
   int tiles_per_block;
-  const double* avg_rho;  // previous step's ConstRho average
+  const double* avg_rho;  // previous step's ConstRho average, per ensemble member
+  const SimConst* scm;    // per ensemble member (relaxation rate, boundary specs), indexed by BlockTab::member
   const int* step;        // coarse-step counter (time of the inlet ramps)
   int* step_next;         // non-null in the first kernel of a step: *step_next = *step + 1
   int want_stats;         // 0 none, 1 parallel partial sums, 2 reference order
@@ -400,9 +402,13 @@ __device__ __forceinline__ void CollideCell(T* f, int k, const SimConst& sc, con
 // --------------------------------------------------------------- launchers
 template <typename T>
 void LaunchTile(const TileArgs<T>& a, hipStream_t s);
+// ConstRho averages of the members: member m owns tiles [tile_begin[m],
+// tile_begin[m+1]) of the partial sums, or statistic slots [cell_begin[m],
+// cell_begin[m+1]) in the reference's order.
 template <typename T>
-void LaunchStatsFinalize(const double* partial, int ntiles, double* avg, hipStream_t s);
-void LaunchStatsSequential(const double* val, const uint8_t* use, int count, double* avg, hipStream_t s);
+void LaunchStatsFinalize(const double* partial, const int* tile_begin, int members, double* avg, hipStream_t s);
+void LaunchStatsSequential(const double* val, const uint8_t* use, const int* cell_begin, int members, double* avg,
+                           hipStream_t s);
 void LaunchStepInc(const int* step, int* step_next, hipStream_t s);
 template <typename T>
 void LaunchInitEq(T* pops, const uint8_t* kind, int nb, const SimConst& sc, double rho0,

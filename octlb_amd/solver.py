@@ -29,6 +29,35 @@ class GpuSolver:
         abi.check(self._lib.octlb_dev_create(C.byref(t), C.byref(params), C.byref(h)), "dev create")
         self._h = h
 
+    @classmethod
+    def ensemble(cls, ens, precision: int = 64):
+        """Many independent cases in one device state (``cases.Ensemble``),
+        stepped by the same launches: octlb_dev_create_ensemble."""
+        self = cls.__new__(cls)
+        self._lib = abi.lib()
+        self.topo = ens.topo
+        params_list = ens.params_for(precision)
+        self.params = params_list[0]
+        self.n = ens.topo.n
+        self.halo = ens.topo.halo
+        self.nb = ens.topo.num_blocks
+        self.ncore = self.n ** 3
+        self.np_ = self.n + 2 * self.halo
+        self.members = len(params_list)
+        begin = np.ascontiguousarray(ens.member_begin, dtype=np.int32)
+        plist = (abi.Params * self.members)(*params_list)
+        h = C.c_void_p()
+        t = ens.topo.struct()
+        abi.check(self._lib.octlb_dev_create_ensemble(C.byref(t), self.members, abi.i32p(begin), plist, C.byref(h)),
+                  "ensemble create")
+        self._h = h
+        return self
+
+    def member_average_rho(self, m):
+        out = np.zeros(1)
+        abi.check(self._lib.octlb_dev_member_average_rho(self._h, int(m), abi.f64p(out)), "member_average_rho")
+        return float(out[0])
+
     def close(self):
         if getattr(self, "_h", None) is not None and self._h.value:
             self._lib.octlb_dev_destroy(self._h)
