@@ -98,6 +98,45 @@ def run_gpu(case, precision, steps, warmup, timing_steps):
     return res
 
 
+def run_ensemble(precs, steps, warmup, timing_steps, count=64):
+    """64 independent lid-driven cavities (cells per side drawn from
+    {32, 48, 64, 96}, tau uniform in [0.52, 0.8], lid speed uniform in
+    [0.02, 0.08], one seed) stepped as one device state: one fused-kernel
+    launch, one FD launch and the per-member ConstRho averages per step,
+    replayed as hipGraphs."""
+    from octlb_amd import cases
+    from octlb_amd.solver import GpuSolver
+
+    ens = cases.cavity_ensemble(count=count)
+    cells = int(ens.topo.num_blocks) * 512
+    out = {"workload": ("64 lid-driven cavities in one launch, cells per side from {32,48,64,96}, tau "
+                        "U[0.52,0.8], lid U[0.02,0.08], seed 11, FD walls, ConstRhoBGK with per-member "
+                        "tree-reduced averages"),
+           "members": count, "blocks": int(ens.topo.num_blocks), "cells": cells,
+           "sizes": [int(c.info["cells"]) for c in ens.members]}
+    for p in precs:
+        sim = GpuSolver.ensemble(ens, p)
+        sim.set_bc_kinds(ens.kinds)
+        sim.init_equilibrium(1.0, (0.0, 0.0, 0.0), 0xFF)
+        sim.advance_graph(max(warmup, 1))
+        sim.synchronize()
+        t0 = time.perf_counter()
+        sim.advance_graph(steps)
+        sim.synchronize()
+        dt = time.perf_counter() - t0
+        ms, upd = sim.time_kernel(0, timing_steps)
+        ach = BYTES_PER_UPDATE[p] * upd / (ms * 1e-3) / 1e9
+        out[f"fp{p}"] = {"mlups": round(cells * steps / dt / 1e6, 3), "ms_per_step": round(1e3 * dt / steps, 6),
+                         "steps": steps,
+                         "roofline": {"bound": "hbm", "achieved": round(ach, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                                      "frac": round(ach / HBM_PEAK_GBS, 4), "kernel": "fused collide+stream (KBlock)",
+                                      "cells_per_launch": upd, "avg_launch_us": round(1e3 * ms, 3)}}
+        sim.close()
+    if 32 in precs and 64 in precs:
+        out["fp32_over_fp64"] = round(out["fp32"]["mlups"] / out["fp64"]["mlups"], 3)
+    return out
+
+
 def cpu_baseline(case, budget_s):
     """FP64 oracle restatement, single thread, on a bounded number of steps."""
     from oracle.binding import OracleSim, setup
@@ -198,6 +237,9 @@ def main():
     ap.add_argument("--synthetic-steps", type=int, default=40)
     ap.add_argument("--no-links", action="store_true",
                     help="skip the Bouzidi link-build measurement")
+    ap.add_argument("--no-ensemble", action="store_true",
+                    help="skip the 64-cavity ensemble measurement")
+    ap.add_argument("--ensemble-steps", type=int, default=40)
     ap.add_argument("--precision", default="both", choices=["both", "32", "64"])
     ap.add_argument("--calibrate-copy-mb", type=int, default=0,
                     help="run one device-to-device copy of this many MiB first (profiler "
@@ -267,6 +309,12 @@ def main():
                          "cells_per_launch": sd["cells_per_launch"],
                          "avg_launch_us": round(sd["avg_launch_us"], 3)},
         }
+    ens = None
+    if not args.no_ensemble and args.workload == "cylinder":
+        log("cavity ensemble")
+        ens = run_ensemble(precs, args.ensemble_steps, 5, min(args.timing_steps, 10))
+        for p in precs:
+            log(f"  fp{p}: {ens[f'fp{p}']['mlups']:.1f} MLUPS, {ens[f'fp{p}']['ms_per_step']:.4f} ms/step")
     links = None
     if not args.no_links and args.workload == "cylinder":
         log("bouzidi link build")
@@ -326,6 +374,7 @@ def main():
         },
         "cpu_baseline": cpu,
         "synthetic_amr": synth,
+        "ensemble": ens,
         "link_build": links,
         "drag_fp32": res[32]["drag"] if 32 in res else None,
         "drag_fp64": res[64]["drag"] if 64 in res else None,

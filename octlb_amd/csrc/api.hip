@@ -966,7 +966,7 @@ TileArgs<T> octlb_dev::MakeTileArgs(int l) {
     a.tiles_per_block = padded ? tpa * tpa * tpa : 1;
   }
   a.avg_rho = d_avg;
-  a.scm = d_scm;
+  a.scm = (decltype(a.scm))d_scm;  // read-only table (constant address space on the device)
   a.step = d_step + step_slot;
   a.want_stats = (sc.const_rho && l >= max_level) ? (fast_stats ? 1 : 2) : 0;
   a.snap_on = 1;
@@ -1124,7 +1124,7 @@ FdArgs<T> octlb_dev::MakeFdArgs() {
   f.pops1 = static_cast<T*>(pops[1]);
   f.post = static_cast<T*>(d_fd_post);
   f.step = d_step + step_slot;
-  f.scm = d_scm;
+  f.scm = (decltype(f.scm))d_scm;
   f.fd_list = d_fd_list;
   f.fd_count = d_fd_count;
   f.par = Par();

@@ -156,7 +156,7 @@ __global__ __launch_bounds__(1024, BlockOccupancy<T>::kWavesPerSimd) void KBlock
   const bool own = tid < ncore;
   const int ap = tid - ncore;  // apron slot (shell index) when !own
   const BlockTab& bt = a.bt[b];
-  const SimConst& sc = a.scm[bt.member];  // apron cells belong to the same member
+  const SimConst& sc = *(const SimConst*)&a.scm[bt.member];  // apron cells belong to the same member
   const int bflags = bt.flags;
   const int bshell = bt.shell;
 
@@ -1012,7 +1012,7 @@ __device__ inline void FdBlockBody(const FdArgs<T>& a, const int* fd_blocks, con
   }
   const int b = fd_blocks[fb];
   const int tid = threadIdx.x;
-  const SimConst& sc = a.scm[a.bt[b].member];
+  const SimConst& sc = *(const SimConst*)&a.scm[a.bt[b].member];
   T* cur = (a.par.v[a.level[b]] ? a.pops1 : a.pops0) + static_cast<size_t>(b) * kQ * kTileCells;
   if (act) {
     const int ix = tid >> 6, iy = (tid >> 3) & 7, iz = tid & 7;

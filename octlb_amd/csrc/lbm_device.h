@@ -25,6 +25,15 @@
 
 namespace octlb {
 
+// Pointer qualifier for read-only kernel tables: in device code the
+// constant address space, whose loads never alias the kernel's stores, so
+// the compiler keeps them as hoistable scalar loads like kernel arguments.
+#if defined(__HIP_DEVICE_COMPILE__)
+#define OCTLB_CONST_TABLE __attribute__((address_space(4)))
+#else
+#define OCTLB_CONST_TABLE
+#endif
+
 constexpr int kTile = 8;                       // tile edge (cells)
 constexpr int kBox = kTile + 2;                // tile + one-cell apron
 constexpr int kBoxCells = kBox * kBox * kBox;  // 1000
@@ -139,7 +148,7 @@ struct FdArgs {
   T* post;               // [entries][19]
   const int* step;
   int* step_next;
-  const SimConst* scm;   // per ensemble member, indexed by BlockTab::member
+  const OCTLB_CONST_TABLE SimConst* scm;  // per ensemble member, indexed by BlockTab::member
   const short* fd_list;  // blocked mode: [fd block][512] FD cells, plane|edge|corner order
   const int4* fd_count;  // blocked mode: [fd block] end offsets of the three phases, total
   ParArray par;
@@ -181,7 +190,7 @@ struct TileArgs {
   uint8_t* stat_use;
   int tiles_per_block;
   const double* avg_rho;  // previous step's ConstRho average, per ensemble member
-  const SimConst* scm;    // per ensemble member (relaxation rate, boundary specs), indexed by BlockTab::member
+  const OCTLB_CONST_TABLE SimConst* scm;  // per ensemble member (relaxation rate, boundary specs), by BlockTab::member
   const int* step;        // coarse-step counter (time of the inlet ramps)
   int* step_next;         // non-null in the first kernel of a step: *step_next = *step + 1
   int want_stats;         // 0 none, 1 parallel partial sums, 2 reference order
