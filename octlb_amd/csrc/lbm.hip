@@ -32,6 +32,14 @@ __device__ inline int XcdTile(int w, int nt) {
   return (w < (q8 << 3)) ? (w & 7) * q8 + (w >> 3) : w;
 }
 
+// A launch's read-only inputs seen through the constant address space:
+// their loads cannot alias the launch's stores, so the compiler may hoist
+// and batch them (and keep uniform ones as scalar loads).
+template <typename P>
+__device__ __forceinline__ const OCTLB_CONST_TABLE P* Ro(const P* p) {
+  return (const OCTLB_CONST_TABLE P*)p;
+}
+
 // The coarse-step counter is double-buffered: a step's kernels read *step;
 // the first kernel of the step also writes the next step's slot, which
 // nothing in this step reads.
@@ -150,12 +158,12 @@ __global__ __launch_bounds__(1024, BlockOccupancy<T>::kWavesPerSimd) void KBlock
     return;
   }
   const int ti = XcdTile(static_cast<int>(blockIdx.x) - a.fd_wgs - a.pc_wgs, a.ntiles);
-  const int b = a.tiles[ti].block;
+  const int b = Ro(a.tiles)[ti].block;
   constexpr int n = kTile, ncore = kTileCells;
   const int tid = threadIdx.x;
   const bool own = tid < ncore;
   const int ap = tid - ncore;  // apron slot (shell index) when !own
-  const BlockTab& bt = a.bt[b];
+  const BlockTab& bt = *(const BlockTab*)&Ro(a.bt)[b];
   const SimConst& sc = *(const SimConst*)&a.scm[bt.member];  // apron cells belong to the same member
   const int bflags = bt.flags;
   const int bshell = bt.shell;
@@ -203,22 +211,22 @@ __global__ __launch_bounds__(1024, BlockOccupancy<T>::kWavesPerSimd) void KBlock
   if (src_b >= 0) {
     const size_t base = static_cast<size_t>(src_b) * kQ * ncore + cc;
 #pragma unroll
-    for (int i = 0; i < kQ; ++i) f[i] = a.src[base + static_cast<size_t>(i) * ncore];
-    k = a.kind[static_cast<size_t>(src_b) * ncore + cc];
+    for (int i = 0; i < kQ; ++i) f[i] = Ro(a.src)[base + static_cast<size_t>(i) * ncore];
+    k = Ro(a.kind)[static_cast<size_t>(src_b) * ncore + cc];
   } else if (l >= 0 && bshell >= 0) {
     const size_t base = static_cast<size_t>(bshell) * kQ * kApron + ap;
 #pragma unroll
-    for (int i = 0; i < kQ; ++i) f[i] = a.shell_src[base + static_cast<size_t>(i) * kApron];
+    for (int i = 0; i < kQ; ++i) f[i] = Ro(a.shell_src)[base + static_cast<size_t>(i) * kApron];
   } else {
 #pragma unroll
     for (int i = 0; i < kQ; ++i) f[i] = T(0);
   }
-  const bool rules = PERCELL && own && !(bflags & kFlagPlain) && a.rule[static_cast<size_t>(b) * ncore + tid] != 0;
+  const bool rules = PERCELL && own && !(bflags & kFlagPlain) && Ro(a.rule)[static_cast<size_t>(b) * ncore + tid] != 0;
   uint32_t qm = 0;
   const double* qv = a.qval;
   if (PERCELL && own && (bflags & kFlagLinks)) {
-    qm = a.qmask[static_cast<size_t>(b) * ncore + tid];
-    qv = a.qval + a.qoff[static_cast<size_t>(b) * ncore + tid];
+    qm = Ro(a.qmask)[static_cast<size_t>(b) * ncore + tid];
+    qv = a.qval + Ro(a.qoff)[static_cast<size_t>(b) * ncore + tid];
   }
 
   // ---- collide real cells (own and exchanged apron); halo-shell slots are
@@ -226,9 +234,9 @@ __global__ __launch_bounds__(1024, BlockOccupancy<T>::kWavesPerSimd) void KBlock
   double ssum = 0.0;
   int scnt = 0;
   if (src_b >= 0) {
-    const double t = static_cast<double>(*a.step);
-    const T avg = sc.const_rho ? static_cast<T>(a.avg_rho[bt.member]) : T(1);
-    const BlockTab& st = a.bt[src_b];
+    const double t = static_cast<double>(*Ro(a.step));
+    const T avg = sc.const_rho ? static_cast<T>(Ro(a.avg_rho)[bt.member]) : T(1);
+    const BlockTab& st = *(const BlockTab*)&Ro(a.bt)[src_b];
     const Origin og{st.origin[0], st.origin[1], st.origin[2], st.cw};
     CollideCell<T>(f, k, sc, og, cx, cy, cz, t, avg, own && a.want_stats != 0, ssum, scnt);
   }
@@ -240,9 +248,9 @@ __global__ __launch_bounds__(1024, BlockOccupancy<T>::kWavesPerSimd) void KBlock
   if (own && (bflags & kFlagDrag) && a.snap_on) {
     // post-collide snapshot of the outgoing populations on wall links
     // (take_post_collide_snapshot, block_lattice.cpp:309-316)
-    const uint32_t dm = a.dmask[static_cast<size_t>(b) * ncore + tid];
+    const uint32_t dm = Ro(a.dmask)[static_cast<size_t>(b) * ncore + tid];
     if (dm) {
-      T* sp = a.snap + a.doff[static_cast<size_t>(b) * ncore + tid];
+      T* sp = a.snap + Ro(a.doff)[static_cast<size_t>(b) * ncore + tid];
       int j = 0;
 #pragma unroll
       for (int i = 0; i < kQ; ++i)
@@ -254,7 +262,7 @@ __global__ __launch_bounds__(1024, BlockOccupancy<T>::kWavesPerSimd) void KBlock
   // a cell that the full-time prolongation of this launch overwrites is
   // left to the coupling workgroups
   const bool keep = !(own && a.pfull != nullptr && (bflags & kFlagProlong) &&
-                      a.pfull[static_cast<size_t>(b) * ncore + tid] >= 0);
+                      Ro(a.pfull)[static_cast<size_t>(b) * ncore + tid] >= 0);
   if (own) {
     T* dst = a.dst + static_cast<size_t>(b) * kQ * ncore + tid;
     if (!keep) {
