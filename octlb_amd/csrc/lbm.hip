@@ -447,16 +447,30 @@ __global__ __launch_bounds__(512) void KPadded(TileArgs<T> a) {
 }
 
 // ------------------------------------------------------------ small kernels
-// One thread per ensemble member.
-__global__ void KStatsFinalize(const double* partial, const int* tile_begin, int members, double* avg) {
-  const int m = static_cast<int>(blockIdx.x * blockDim.x + threadIdx.x);
-  if (m >= members) return;
+// Tree reduction of the per-tile partial sums (fast_stats: not the
+// reference's summation order, but a fixed one): one workgroup per member,
+// threads take strided tiles in order, then a pairwise reduction.
+constexpr int kStatsThreads = 256;
+__global__ __launch_bounds__(kStatsThreads) void KStatsFinalize(const double* partial, const int* tile_begin,
+                                                                double* avg) {
+  __shared__ double rs[kStatsThreads], rc[kStatsThreads];
+  const int m = blockIdx.x, tid = threadIdx.x;
   double s = 0.0, c = 0.0;
-  for (int i = tile_begin[m]; i < tile_begin[m + 1]; ++i) {
+  for (int i = tile_begin[m] + tid; i < tile_begin[m + 1]; i += kStatsThreads) {
     s += partial[2 * i];
     c += partial[2 * i + 1];
   }
-  avg[m] = c > 0.0 ? s / c : 1.0;  // CollideRhoStats::average (block_lattice.h:31)
+  rs[tid] = s;
+  rc[tid] = c;
+  __syncthreads();
+  for (int w = kStatsThreads / 2; w > 0; w >>= 1) {
+    if (tid < w) {
+      rs[tid] += rs[tid + w];
+      rc[tid] += rc[tid + w];
+    }
+    __syncthreads();
+  }
+  if (tid == 0) avg[m] = rc[0] > 0.0 ? rs[0] / rc[0] : 1.0;  // CollideRhoStats::average (block_lattice.h:31)
 }
 
 // CollideRhoStats accumulated in the reference's cell order (blocks of the
@@ -1542,7 +1556,7 @@ static inline dim3 Grid(long long n, int bs) { return dim3(static_cast<unsigned>
 
 template <typename T>
 void LaunchStatsFinalize(const double* partial, const int* tile_begin, int members, double* avg, hipStream_t s) {
-  hipLaunchKernelGGL(KStatsFinalize, Grid(members, 64), dim3(64), 0, s, partial, tile_begin, members, avg);
+  hipLaunchKernelGGL(KStatsFinalize, dim3(members), dim3(kStatsThreads), 0, s, partial, tile_begin, avg);
 }
 
 void LaunchStatsSequential(const double* val, const uint8_t* use, const int* cell_begin, int members, double* avg,
