@@ -166,11 +166,16 @@ __device__ __forceinline__ void TriClosest(const double* t, const double* p, dou
 }
 
 // One wave per candidate cell: nearest soup point, then the 18 populations.
+// Triangles are staged in LDS with a bounding box; a lane skips the
+// triangles that the wave's best point so far already beats strictly (the
+// result is the same as the full scan: a skipped triangle has no point as
+// close as that best, so it can be neither a closer nor an equal minimum).
 // qd[cand][i-1] = q or -1; cnt[cand] = number of links of the cell.
 __global__ __launch_bounds__(kLinkThreads) void KNearest(const int2* cand, int64_t ncand, int n, const double* bounds,
                                                          const uint8_t* material, const double* tris, int64_t ntri,
                                                          double* qd, int* cnt) {
   __shared__ double st[kTriChunk * 9];
+  __shared__ double box[kTriChunk][6];  // bounding box (lo, hi) of each staged triangle
   const int lane = threadIdx.x & 63;
   const int64_t ci = static_cast<int64_t>(blockIdx.x) * (kLinkThreads / 64) + (threadIdx.x >> 6);
   const bool live = ci < ncand;
@@ -194,15 +199,34 @@ __global__ __launch_bounds__(kLinkThreads) void KNearest(const int2* cand, int64
     p[2] = bb[2] + (static_cast<double>(k) + 0.5) * dz;
   }
   double bd = DBL_MAX;
+  double pb = DBL_MAX;  // the wave's best distance at the end of the last chunk
   int64_t bt = INT64_MAX;
   double w[3] = {p[0], p[1], p[2]};
   for (int64_t base = 0; base < ntri; base += kTriChunk) {
     const int64_t nt = ntri - base < kTriChunk ? ntri - base : kTriChunk;
     __syncthreads();  // the previous chunk is no longer read
     for (int e = threadIdx.x; e < nt * 9; e += kLinkThreads) st[e] = tris[base * 9 + e];
+    if (threadIdx.x < nt) {
+      const double* v = tris + (base + threadIdx.x) * 9;
+      for (int d = 0; d < 3; ++d) {
+        box[threadIdx.x][d] = fmin(v[d], fmin(v[3 + d], v[6 + d]));
+        box[threadIdx.x][3 + d] = fmax(v[d], fmax(v[3 + d], v[6 + d]));
+      }
+    }
     __syncthreads();
     if (live) {
       for (int r = lane; r < nt; r += 64) {
+        // Skip a triangle whose bounding box is farther (squared distance,
+        // lowered by 1e-12 relative -- far beyond its rounding error) than
+        // the wave's best point: every edge point of it is, so it can be
+        // neither a closer nor an equal minimum.
+        double lb = 0.0;
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+          const double g = fmax(fmax(box[r][d] - p[d], p[d] - box[r][3 + d]), 0.0);
+          lb += g * g;
+        }
+        if (lb * (1.0 - 1.0e-12) > fmin(bd, pb)) continue;
         double q[3];
         TriClosest(&st[r * 9], p, q);
         const double d2 = (p[0] - q[0]) * (p[0] - q[0]) + (p[1] - q[1]) * (p[1] - q[1]) +
@@ -215,6 +239,11 @@ __global__ __launch_bounds__(kLinkThreads) void KNearest(const int2* cand, int64
           w[2] = q[2];
         }
       }
+      // any lane's best bounds every lane's search: a triangle farther
+      // than it (strictly) cannot be the first minimum either
+      double m = bd;
+      for (int off = 32; off > 0; off >>= 1) m = fmin(m, __shfl_xor(m, off));
+      pb = m;
     }
   }
   if (!live) return;
