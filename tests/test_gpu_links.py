@@ -5,7 +5,8 @@ fractions, in (block, cell, population) order.  Cases: the reference's
 single-triangle wall, a sphere over a 2-level forest (cell widths differ per
 block), a z-spanning cylinder over the cylinder-channel forest with more
 triangles than one LDS chunk, duplicated triangles (ties), the empty soup,
-and the count query / capacity errors of the C ABI.
+and the count query / capacity errors of the C ABI; and the whole pipeline:
+device-built links driving the 2-level cylinder step, against the oracle.
 """
 import ctypes as C
 
@@ -79,3 +80,33 @@ def test_empty_soup_count_query_and_capacity():
     rc = h.octlb_build_links(*args, 0, C.byref(cnt), abi.i32p(small), abi.f64p(qs))
     assert rc == abi.EINVAL if hasattr(abi, "EINVAL") else rc < 0
     assert b"capacity" in h.octlb_last_error()
+
+
+def test_device_links_drive_the_solver():
+    """Geometry -> q -> step: links built on the device from a triangulated
+    cylinder are set on the 2-level cylinder case (kinds from the same
+    material field) and the solver matches the oracle run with those links
+    to 1e-12, drag included."""
+    from dataclasses import replace
+
+    from tests.util import make_gpu, make_oracle, rel_err
+
+    case = cases.cylinder_channel(bricks_x=2, base_level=1, fine=True, xc=0.5, radius=0.1,
+                                  refine_lo=0.25, refine_hi=1.0)
+    inside = lambda x, y, z: (x - 0.5) ** 2 + (y - 0.5) ** 2 < 0.1 ** 2  # noqa: E731
+    mat = geometry.material_field(case.topo, inside)
+    kinds = cases.material_to_kinds(mat.transpose(0, 3, 2, 1).astype(np.int64))
+    cases.stamp_tree_boundary(kinds, case.topo.tree_boundary, case.specs)
+    tris = geometry.cylinder_soup(0.5, 0.5, 0.1, 0.0, 1.0, 96, rings=2)
+    link, q = geometry.build_links(case.topo.bounds, 8, mat, tris)
+    ol, oq = oracle_links(case.topo.bounds, 8, mat, tris)
+    np.testing.assert_array_equal(link, ol)
+    np.testing.assert_array_equal(q, oq)
+    assert len(link) > 100
+    c2 = replace(case, kinds=kinds, link=link, q=q)
+    g, o = make_gpu(c2), make_oracle(c2)
+    g.advance(10)
+    o.advance(10)
+    assert rel_err(g.download_core(), o.download_core()) <= 1e-12
+    fd, fo = g.drag(c2.drag_bbox), o.drag(c2.drag_bbox)
+    assert np.abs(fd - fo).max() <= 1e-12 * max(np.abs(fo).max(), 1e-30)
