@@ -162,7 +162,7 @@ def cylinder_channel(bricks_x: int = 4, base_level: int = 2, fine: bool = True, 
                      bricks_yz: int = 1,
                      xc: float = 1.0, radius: float = 0.125, u_in: float = 0.04, tau: float = 0.55,
                      refine_lo: float = None, refine_hi: float = None, ramp_end_t: float = 0.0,
-                     refine_y: tuple = (-1.0, 2.0)) -> Case:
+                     refine_y: tuple = (-1.0, 2.0), open_ends: bool = True) -> Case:
     """Channel with a z-spanning cylinder on a 2-level brick forest.
 
     Coarse blocks at ``base_level`` fill [0,bricks_x]x[0,1]x[0,1]; blocks
@@ -171,7 +171,11 @@ def cylinder_channel(bricks_x: int = 4, base_level: int = 2, fine: bool = True, 
     (rho 1), lateral faces have no spec (default no-slip -> kBounceBack,
     bc_installer.h:44-58).  Cylinder cells are solid, its surface cells
     kBouzidi; q is the exact ray-circle fraction of every cut fluid link.
-    A single relaxation rate is used on all levels (time_loop.h:183-187)."""
+    A single relaxation rate is used on all levels (time_loop.h:183-187).
+
+    ``open_ends=False`` drops the inlet and outlet: every face is a
+    bounce-back wall and the fluid starts at feq(1, (u_in, 0, 0)) -- the
+    closed box the discrete adjoint differentiates (no FD boundary cells)."""
     dom = np.array([0.0, 0.0, 0.0, float(bricks_x) / bricks_yz, 1.0, 1.0])
     f = Forest(dom, (bricks_x, bricks_yz, bricks_yz))
     f.refine_uniform(base_level)
@@ -198,7 +202,7 @@ def cylinder_channel(bricks_x: int = 4, base_level: int = 2, fine: bool = True, 
     kinds = material_to_kinds(material)
     specs = [abi.make_spec(abi.XMIN, abi.BC_INTERP_VELOCITY, inlet=abi.INLET_UNIFORM,
                            inlet_u=(u_in, 0.0, 0.0), ramp_end_t=ramp_end_t),
-             abi.make_spec(abi.XMAX, abi.BC_INTERP_PRESSURE, rho_target=1.0)]
+             abi.make_spec(abi.XMAX, abi.BC_INTERP_PRESSURE, rho_target=1.0)] if open_ends else []
     stamp_tree_boundary(kinds, topo.tree_boundary, specs)
     # Bouzidi links of material-fluid cells whose in-block pull source x - c_i
     # is solid-like (bouzidi_link_data.cpp:188-225), q in (1e-8, 1-1e-8)
@@ -231,6 +235,7 @@ def cylinder_channel(bricks_x: int = 4, base_level: int = 2, fine: bool = True, 
     return Case(name="cylinder", topo=topo, omega=1.0 / tau, handler=abi.HANDLER_CONCRETE,
                 const_rho=0, bouzidi=1, snapshot=1, specs=specs, kinds=kinds, link=link, q=q,
                 init_mask=1 << abi.KIND_BULK, drag_bbox=bbox, area=area, u_ref=u_in,
+                init_u0=(0.0, 0.0, 0.0) if open_ends else (u_in, 0.0, 0.0),
                 info={"fine_dx": fine_dx, "D": d, "material_fluid": fluid})
 
 

@@ -63,6 +63,8 @@ def lib():
             "ol_apply_restriction": ([vp, C.c_int32], C.c_int),
             "ol_build_links": ([C.c_int32, C.c_int32, f64p, u8p, C.c_int64, f64p, C.c_int64, i32p, f64p],
                                C.c_int64),
+            "ol_adjoint": ([vp, C.c_int32, C.c_int32, C.c_int32, f64p, f64p, C.c_double, f64p, C.c_uint32,
+                            C.c_int64, i32p, f64p, f64p, f64p], C.c_int),
         }
         for name, (a, r) in sig.items():
             fn = getattr(h, name)
@@ -146,6 +148,21 @@ class OracleSim:
         out = np.zeros(3)
         _ok(lib().ol_drag(self._h, abi.f64p(b), abi.f64p(out)), "drag")
         return out
+
+    def adjoint(self, steps, window, direction, bbox, u0, rho0=1.0, kind_mask=1, link=None):
+        """J = mean over steps window[0]+1 .. window[1] of direction . drag(bbox)
+        from feq(rho0, u0); returns (J, dJ/du0, dJ/dq per link row)."""
+        d = np.ascontiguousarray(direction, dtype=np.float64)
+        bb = np.ascontiguousarray(bbox, dtype=np.float64)
+        u = np.ascontiguousarray(u0, dtype=np.float64)
+        link = np.zeros((0, 3), np.int32) if link is None else np.ascontiguousarray(link, dtype=np.int32).reshape(-1, 3)
+        J = np.zeros(1)
+        du = np.zeros(3)
+        dq = np.zeros(max(len(link), 1))
+        _ok(lib().ol_adjoint(self._h, int(steps), int(window[0]), int(window[1]), abi.f64p(d), abi.f64p(bb),
+                             float(rho0), abi.f64p(u), int(kind_mask), int(len(link)), abi.i32p(link),
+                             abi.f64p(J), abi.f64p(du), abi.f64p(dq)), "adjoint")
+        return float(J[0]), du, dq[: len(link)]
 
     def mass(self):
         return float(lib().ol_mass(self._h))

@@ -1669,3 +1669,282 @@ int64_t ol_build_links(int32_t nb, int32_t n, const double* bounds, const uint8_
   free(dense);
   return count;
 }
+
+/* ---------------- discrete adjoint of the single-level step -------------
+ * Not a restatement: the reference has no adjoint.  This is the exact
+ * transpose (FP64) of the functions above for one level with the concrete
+ * handler and Bouzidi streaming, written loop for loop against
+ * collide_block / exchange / stream_block so that it checks the device
+ * adjoint.  It is pinned by central differences of ol_advance itself. */
+
+/* y = olb_bgk(x, proxy_rho_u(x)) (block_lattice.h:113-125, dynamics/lbm.h:395-403):
+ * lx_j = (1-w) ly_j + w (A + sum_d G_d (c_jd - u_d) / rho) with
+ * A = sum_i ly_i dfeq_i/drho and G_d = sum_i ly_i dfeq_i/du_d. */
+static void bgk_vjp(const double* x, const double* ly, double omega, double* lx) {
+  double rho, u[3];
+  proxy_rho_u(x, &rho, u);
+  const double usqr = norm_sqr(u);
+  double A = 0.0, G[3] = {0.0, 0.0, 0.0};
+  for (int i = 0; i < Q; ++i) {
+    double cu = 0.0;
+    for (int d = 0; d < 3; ++d) cu += (double)C[i][d] * u[d];
+    const double t = TW(i);
+    A += ly[i] * t * (1.0 + 3.0 * cu + 4.5 * cu * cu - 1.5 * usqr);
+    for (int d = 0; d < 3; ++d) G[d] += ly[i] * rho * t * (3.0 * C[i][d] + 9.0 * cu * C[i][d] - 3.0 * u[d]);
+  }
+  for (int j = 0; j < Q; ++j) {
+    double g = A;
+    for (int d = 0; d < 3; ++d) g += G[d] * ((double)C[j][d] - u[d]) / rho;
+    lx[j] = (1.0 - omega) * ly[j] + omega * g;
+  }
+}
+
+/* transpose of stream_block's per-cell branch (Bouzidi path) for block b:
+ * ly += M_b^T lx, with y the post-exchange state the forward pulled from;
+ * dq (dense like s->q) += lx * d out / d q on Bouzidi rows. */
+static void stream_block_adj(const ol_sim* s, int b, const double* y, const double* lx, double* ly,
+                             double* dq) {
+  const int n = s->n, h = s->h, NP = s->NP, NS = s->nslots;
+  const double* f = y + (size_t)b * NS * Q;
+  const double* lxb = lx + (size_t)b * NS * Q;
+  double* lyb = ly + (size_t)b * NS * Q;
+  for (int hx = 0; hx < NP; ++hx)
+    for (int hy = 0; hy < NP; ++hy)
+      for (int hz = 0; hz < NP; ++hz) {
+        const int ix = hx - h, iy = hy - h, iz = hz - h;
+        const int core = ix >= 0 && ix < n && iy >= 0 && iy < n && iz >= 0 && iz < n;
+        const int dst = slot(s, hx, hy, hz);
+        const double* lo = lxb + (size_t)dst * Q;
+        if (!core) {
+          for (int i = 0; i < Q; ++i) {
+            const int sx = hx - C[i][0], sy = hy - C[i][1], sz = hz - C[i][2];
+            int src;
+            if (sx >= 0 && sx < NP && sy >= 0 && sy < NP && sz >= 0 && sz < NP) {
+              src = slot(s, sx, sy, sz);
+            } else {
+              const int off = C[i][0] * NP * NP + C[i][1] * NP + C[i][2];
+              src = floor_mod(dst - off, NS);
+            }
+            lyb[(size_t)src * Q + i] += lo[i];
+          }
+          continue;
+        }
+        const int k = kind_at(s, b, ix, iy, iz);
+        if (k == K_SOLID) {
+          for (int i = 0; i < Q; ++i) lyb[(size_t)dst * Q + i] += lo[i];
+          continue;
+        }
+        double* lself = lyb + (size_t)dst * Q;
+        const double* self = f + (size_t)dst * Q;
+        for (int i = 0; i < Q; ++i) {
+          const int sx = hx - C[i][0], sy = hy - C[i][1], sz = hz - C[i][2];
+          if (!(sx >= 0 && sx < NP && sy >= 0 && sy < NP && sz >= 0 && sz < NP)) {
+            lself[reflects_on_pull(k) ? OPP[i] : i] += lo[i];
+            continue;
+          }
+          const int sxi = sx - h, syi = sy - h, szi = sz - h;
+          const int src_interior = sxi >= 0 && sxi < n && syi >= 0 && syi < n && szi >= 0 && szi < n;
+          if (reflects_on_pull(k) && !src_interior) {
+            lself[OPP[i]] += lo[i];
+            continue;
+          }
+          const int src_kind = src_interior ? kind_at(s, b, sxi, syi, szi) : K_BULK;
+          if (src_kind == K_SOLID || src_kind == K_BOUZIDI) {
+            const size_t qi = ((size_t)b * s->ncells + (size_t)((ix * n + iy) * n + iz)) * Q + i;
+            const double qf = s->q[qi];
+            if (qf > 1.0e-10 && qf < 1.0 - 1.0e-10) {
+              const int nxh = hx + C[i][0], nyh = hy + C[i][1], nzh = hz + C[i][2];
+              const int nxi = nxh - h, nyi = nyh - h, nzi = nzh - h;
+              const int nk = (nxi >= 0 && nxi < n && nyi >= 0 && nyi < n && nzi >= 0 && nzi < n)
+                                 ? kind_at(s, b, nxi, nyi, nzi)
+                                 : K_BULK;
+              const size_t int_slot = nk == K_BULK ? (size_t)slot(s, nxh, nyh, nzh) : (size_t)dst;
+              const double f_bb = self[OPP[i]];
+              const double f_int = f[int_slot * Q + i];
+              if (qf >= 0.5 - 1.0e-14) {
+                const double a = 1.0 / (2.0 * qf);
+                lself[OPP[i]] += a * lo[i];
+                lyb[int_slot * Q + i] += (1.0 - a) * lo[i];
+                dq[qi] += lo[i] * (-(f_bb - f_int) / (2.0 * qf * qf));
+              } else {
+                lself[OPP[i]] += 2.0 * qf * lo[i];
+                lself[i] += (1.0 - 2.0 * qf) * lo[i];
+                dq[qi] += lo[i] * (2.0 * f_bb - 2.0 * self[i]);
+              }
+              continue;
+            }
+            lself[OPP[i]] += lo[i];
+            continue;
+          }
+          lyb[(size_t)slot(s, sx, sy, sz) * Q + i] += lo[i];
+        }
+      }
+}
+
+/* transpose of exchange(): every copy dst <- src becomes l[src] += l[dst], l[dst] = 0 */
+static void move_adj(double* l, size_t src, size_t dst) {
+  for (int i = 0; i < Q; ++i) {
+    l[src + i] += l[dst + i];
+    l[dst + i] = 0.0;
+  }
+}
+static void exchange_adj(const ol_sim* s, double* l) {
+  const int n = s->n, h = s->h;
+  const size_t bs = (size_t)s->nslots * Q;
+  for (int e = s->nedges - 1; e >= 0; --e) {
+    const int a = s->edges[4 * e], d1 = s->edges[4 * e + 1], d2 = s->edges[4 * e + 2], dd = s->edges[4 * e + 3];
+    const int a1 = d1 / 2, a2 = d2 / 2, a3 = 3 - a1 - a2;
+    const int max1 = d1 & 1, max2 = d2 & 1;
+    const int f1 = (max1 ? n - 1 : 0) + h, f2 = (max2 ? n - 1 : 0) + h;
+    const int g1 = (!max1) ? n + h : h - 1, g2 = (!max2) ? n + h : h - 1;
+    for (int k = 0; k < n; ++k) {
+      int ps[3], pd[3];
+      ps[a1] = f1;
+      ps[a2] = f2;
+      ps[a3] = k + h;
+      pd[a1] = g1;
+      pd[a2] = g2;
+      pd[a3] = k + h;
+      move_adj(l, (size_t)a * bs + (size_t)slot(s, ps[0], ps[1], ps[2]) * Q,
+               (size_t)dd * bs + (size_t)slot(s, pd[0], pd[1], pd[2]) * Q);
+    }
+  }
+  for (int e = s->nsame - 1; e >= 0; --e) {
+    const int src_b = s->same[3 * e], dir = s->same[3 * e + 1], dst_b = s->same[3 * e + 2];
+    int slo[3], shi[3], dlo[3], dhi[3];
+    face_ranges(s, dir, 0, slo, shi);
+    face_ranges(s, dir ^ 1, 1, dlo, dhi);
+    const int axis = dir / 2;
+    for (int x = slo[0]; x < shi[0]; ++x)
+      for (int y = slo[1]; y < shi[1]; ++y)
+        for (int z = slo[2]; z < shi[2]; ++z) {
+          int p[3] = {x, y, z};
+          p[axis] = dlo[axis];
+          move_adj(l, (size_t)src_b * bs + (size_t)slot(s, x, y, z) * Q,
+                   (size_t)dst_b * bs + (size_t)slot(s, p[0], p[1], p[2]) * Q);
+        }
+  }
+}
+
+/* seeds d(dir . F)/d(snap) and d(dir . F)/d(live) of ol_drag's sum, times w */
+static void drag_seed(const ol_sim* s, const double bbox[6], const double dir[3], double w, double* l_live,
+                      double* l_snap) {
+  const int n = s->n;
+  for (int b = 0; b < s->nb; ++b) {
+    const double* org = s->origin + 3 * (size_t)b;
+    const double cwb = s->cw[b];
+    for (int iz = 0; iz < n; ++iz)
+      for (int iy = 0; iy < n; ++iy)
+        for (int ix = 0; ix < n; ++ix) {
+          if (kind_at(s, b, ix, iy, iz) != K_BULK) continue;
+          const size_t o = (size_t)b * s->nslots * Q + (size_t)core_slot(s, ix, iy, iz) * Q;
+          for (int i = 1; i < Q; ++i) {
+            const int x = ix + C[i][0], y = iy + C[i][1], z = iz + C[i][2];
+            if (x < 0 || x >= n || y < 0 || y >= n || z < 0 || z >= n) continue;
+            const int nk = kind_at(s, b, x, y, z);
+            if (nk != K_SOLID && nk != K_BOUZIDI) continue;
+            const double px = org[0] + (x + 0.5) * cwb, py = org[1] + (y + 0.5) * cwb,
+                         pz = org[2] + (z + 0.5) * cwb;
+            if (!(px >= bbox[0] && px <= bbox[3] && py >= bbox[1] && py <= bbox[4] && pz >= bbox[2] &&
+                  pz <= bbox[5]))
+              continue;
+            const double g = -w * (dir[0] * C[i][0] + dir[1] * C[i][1] + dir[2] * C[i][2]);
+            l_snap[o + i] += g;
+            l_live[o + OPP[i]] += g;
+          }
+        }
+  }
+}
+
+int ol_adjoint(ol_sim* s, int32_t steps, int32_t win_begin, int32_t win_end, const double dir[3],
+               const double bbox[6], double rho0, const double u0[3], uint32_t kind_mask, int64_t nlinks,
+               const int32_t* link, double* J, double dJ_du0[3], double* dJ_dq) {
+  if (!s || !dir || !bbox || !u0 || !J || !dJ_du0 || (nlinks > 0 && (!link || !dJ_dq)))
+    return fail("null argument");
+  if (s->max_level != 0) return fail("adjoint: single level only");
+  if (s->handler != OCTLB_HANDLER_CONCRETE || s->const_rho || !s->bouzidi || !s->snapshot)
+    return fail("adjoint: needs the concrete handler, plain BGK, Bouzidi streaming and snapshots");
+  if (steps < 1 || win_begin < 0 || win_end <= win_begin || win_end > steps) return fail("adjoint: bad window");
+  const size_t tot = (size_t)s->nb * s->nslots * Q;
+  for (size_t c = 0; c < (size_t)s->nb * s->ncells; ++c)
+    if (is_fd_kind(s->kind[c])) return fail("adjoint: FD boundary kinds are not differentiated");
+  double* xs = (double*)malloc(sizeof(double) * tot * (size_t)(steps + 1));
+  double* ys = (double*)malloc(sizeof(double) * tot * (size_t)(steps + 1));
+  double* lx = (double*)calloc(tot, sizeof(double));
+  double* ly = (double*)calloc(tot, sizeof(double));
+  double* ls = (double*)calloc(tot, sizeof(double));
+  double* dq = (double*)calloc((size_t)s->nb * s->ncells * Q, sizeof(double));
+  if (!xs || !ys || !lx || !ly || !ls || !dq) {
+    free(xs); free(ys); free(lx); free(ly); free(ls); free(dq);
+    return fail("adjoint: out of memory");
+  }
+  /* forward from feq(rho0, u0), keeping every state */
+  memset(s->f, 0, sizeof(double) * tot);
+  s->step = 0;
+  ol_init_equilibrium(s, rho0, u0, kind_mask);
+  const double wgt = 1.0 / (double)(win_end - win_begin);
+  double jsum = 0.0;
+  memcpy(xs, s->f, sizeof(double) * tot);
+  for (int st = 1; st <= steps; ++st) {
+    ol_advance(s, 1);
+    memcpy(xs + tot * st, s->f, sizeof(double) * tot);
+    memcpy(ys + tot * st, s->snap, sizeof(double) * tot);
+    if (st > win_begin && st <= win_end) {
+      double F[3];
+      ol_drag(s, bbox, F);
+      jsum += dir[0] * F[0] + dir[1] * F[1] + dir[2] * F[2];
+    }
+  }
+  *J = jsum * wgt;
+  /* reverse sweep */
+  for (int st = steps; st >= 1; --st) {
+    memset(ls, 0, sizeof(double) * tot);
+    if (st > win_begin && st <= win_end) drag_seed(s, bbox, dir, wgt, lx, ls);
+    memset(ly, 0, sizeof(double) * tot);
+    for (int b = 0; b < s->nb; ++b) stream_block_adj(s, b, ys + tot * st, lx, ly, dq);
+    for (size_t e = 0; e < tot; ++e) ly[e] += ls[e];
+    exchange_adj(s, ly);
+    /* collide transpose at the pre-collide state xs[st-1] */
+    const double* x = xs + tot * (st - 1);
+    memcpy(lx, ly, sizeof(double) * tot);
+    const int n = s->n;
+    for (int b = 0; b < s->nb; ++b)
+      for (int ix = 0; ix < n; ++ix)
+        for (int iy = 0; iy < n; ++iy)
+          for (int iz = 0; iz < n; ++iz) {
+            const size_t o = (size_t)b * s->nslots * Q + (size_t)core_slot(s, ix, iy, iz) * Q;
+            const int k = kind_at(s, b, ix, iy, iz);
+            if (k == K_BULK) {
+              bgk_vjp(x + o, ly + o, s->omega, lx + o);
+            } else if (k == K_BB) {
+              for (int i = 0; i < Q; ++i) lx[o + i] = ly[o + OPP[i]];
+            }
+          }
+  }
+  /* x0 = feq(rho0, u0) on the masked cells */
+  dJ_du0[0] = dJ_du0[1] = dJ_du0[2] = 0.0;
+  {
+    const int n = s->n;
+    for (int b = 0; b < s->nb; ++b)
+      for (int ix = 0; ix < n; ++ix)
+        for (int iy = 0; iy < n; ++iy)
+          for (int iz = 0; iz < n; ++iz) {
+            if (!((kind_mask >> kind_at(s, b, ix, iy, iz)) & 1u)) continue;
+            const size_t o = (size_t)b * s->nslots * Q + (size_t)core_slot(s, ix, iy, iz) * Q;
+            for (int i = 0; i < Q; ++i) {
+              double cu = 0.0;
+              for (int d = 0; d < 3; ++d) cu += (double)C[i][d] * u0[d];
+              for (int d = 0; d < 3; ++d)
+                dJ_du0[d] += lx[o + i] * rho0 * TW(i) * (3.0 * C[i][d] + 9.0 * cu * C[i][d] - 3.0 * u0[d]);
+            }
+          }
+  }
+  for (int64_t k = 0; k < nlinks; ++k) {
+    const int b = link[3 * k], c = link[3 * k + 1], i = link[3 * k + 2];
+    dJ_dq[k] = (b >= 0 && b < s->nb && c >= 0 && c < s->ncells && i >= 0 && i < Q)
+                   ? dq[((size_t)b * s->ncells + (size_t)c) * Q + i]
+                   : 0.0;
+  }
+  free(xs); free(ys); free(lx); free(ly); free(ls); free(dq);
+  return 0;
+}

@@ -83,6 +83,17 @@ int64_t ol_build_links(int32_t num_blocks, int32_t n, const double* bounds, cons
                        int64_t num_triangles, const double* triangles, int64_t capacity, int32_t* links,
                        double* q);
 
+/* Discrete adjoint of ol_advance for one level (concrete handler, plain
+ * BGK, Bouzidi streaming, snapshots on, no FD boundary kinds).  Runs
+ * `steps` steps from feq(rho0, u0) on the cells whose kind is in kind_mask
+ * (all other slots zero) and returns J = mean over steps win_begin+1 ..
+ * win_end of dir . drag(bbox), dJ/du0 and dJ/dq of each listed link
+ * (block, cell, pop).  Not part of the reference: it checks the device
+ * adjoint and is pinned by central differences of ol_advance. */
+int ol_adjoint(ol_sim* sim, int32_t steps, int32_t win_begin, int32_t win_end, const double dir[3],
+               const double bbox[6], double rho0, const double u0[3], uint32_t kind_mask, int64_t nlinks,
+               const int32_t* link, double* J, double dJ_du0[3], double* dJ_dq);
+
 #ifdef __cplusplus
 }
 #endif
