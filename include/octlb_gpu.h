@@ -272,6 +272,38 @@ int octlb_build_links(int32_t num_blocks, int32_t n, const double* bounds, const
                       int64_t num_triangles, const double* triangles, int64_t capacity,
                       int64_t* num_links, int32_t* links, double* q);
 
+/* ------------------------------------------------------------------ */
+/* Discrete adjoint (no counterpart in the reference, which has none): */
+/* gradients of a time-averaged drag objective through the step that   */
+/* octlb_dev_advance runs, for one level of 8^3 blocks with the        */
+/* concrete handler, plain BGK and Bouzidi streaming, without FD       */
+/* boundary kinds.  The survey's adjoint entry octlb_adj_run(dev, ...)  */
+/* is split into create / run so the transposed stream operator is     */
+/* built once per adapt.                                               */
+/* ------------------------------------------------------------------ */
+typedef struct octlb_adj octlb_adj;
+
+/* Build the adjoint of dev's step from its topology, BcKinds and Bouzidi
+ * links as currently set (OCTLB_EINVAL if dev is outside the scope above). */
+int octlb_adj_create(octlb_dev* dev, octlb_adj** out);
+int octlb_adj_destroy(octlb_adj* adj);
+/* From feq(rho0, u0) on the core cells whose BcKind is in kind_mask (all
+ * other slots zero, step 0), run `steps` steps keeping a checkpoint every
+ * `checkpoint` steps, and return J = mean over steps win_begin+1 .. win_end
+ * of dir . drag(bbox) (octlb_dev_drag's sum), dJ/du0[3] and, for each of the
+ * num_links (block, cell, pop) triples in link, dJ/dq of that link's wall
+ * fraction (0 for a triple that is not a Bouzidi row).  FP64 throughout;
+ * device memory ~ (steps/checkpoint + checkpoint + 6) x 152 kB per block. */
+int octlb_adj_run(octlb_adj* adj, int32_t steps, int32_t win_begin, int32_t win_end, int32_t checkpoint,
+                  const double dir[3], const double bbox[6], double rho0, const double u0[3],
+                  uint32_t kind_mask, int64_t num_links, const int32_t* link, double* J,
+                  double dJ_du0[3], double* dJ_dq);
+/* Core populations after the last run's forward sweep, in download_core
+ * order [block][cell][19] (for checks against octlb_dev_advance). */
+int octlb_adj_final_core(octlb_adj* adj, double* out);
+/* Device time of the last run's forward sweep and reverse sweep (ms). */
+int octlb_adj_timing(const octlb_adj* adj, double* forward_ms, double* reverse_ms);
+
 #ifdef __cplusplus
 }
 #endif

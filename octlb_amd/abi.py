@@ -187,6 +187,12 @@ _SIGNATURES = {
     "octlb_dev_member_average_rho": ([_VP, C.c_int32, _F64P], C.c_int),
     "octlb_build_links": ([C.c_int32, C.c_int32, _F64P, _U8P, C.c_int64, _F64P, C.c_int64,
                            C.POINTER(C.c_int64), _I32P, _F64P], C.c_int),
+    "octlb_adj_create": ([_VP, C.POINTER(C.c_void_p)], C.c_int),
+    "octlb_adj_destroy": ([_VP], C.c_int),
+    "octlb_adj_run": ([_VP, C.c_int32, C.c_int32, C.c_int32, C.c_int32, _F64P, _F64P, C.c_double, _F64P,
+                       C.c_uint32, C.c_int64, _I32P, _F64P, _F64P, _F64P], C.c_int),
+    "octlb_adj_final_core": ([_VP, _F64P], C.c_int),
+    "octlb_adj_timing": ([_VP, _F64P, _F64P], C.c_int),
 }
 
 EXPORTED = tuple(_SIGNATURES)

@@ -1373,6 +1373,52 @@ void CheckDev(const octlb_dev* d) {
 
 }  // namespace
 
+namespace octlb {
+
+void ExportAdjTopo(const octlb_dev* d, AdjTopo* t) {
+  if (!d || !t) throw std::invalid_argument("null argument");
+  if (d->padded || d->n != kTile || d->h != 1)
+    throw std::invalid_argument("adjoint: needs blocks of 8^3 cells with a one-cell halo");
+  if (d->max_level != 0) throw std::invalid_argument("adjoint: one level only (no coupling adjoint)");
+  if (d->nmembers != 1) throw std::invalid_argument("adjoint: ensembles are not differentiated");
+  if (d->sc.handler != OCTLB_HANDLER_CONCRETE || d->sc.const_rho || !d->sc.bouzidi)
+    throw std::invalid_argument("adjoint: needs the concrete handler, plain BGK and Bouzidi streaming");
+  for (uint8_t k : d->kinds)
+    if (IsFd(k)) throw std::invalid_argument("adjoint: FD boundary kinds are not differentiated");
+  constexpr int NP = kTile + 2, NS = NP * NP * NP;
+  t->nb = d->nb;
+  t->omega = d->sc.omega;
+  t->kinds = d->kinds;
+  t->link_key.clear();
+  t->link_q.clear();
+  for (const auto& kv : d->links) {
+    t->link_key.push_back(kv.first);
+    t->link_q.push_back(kv.second);
+  }
+  t->origin_cw.resize(static_cast<size_t>(d->nb) * 4);
+  t->halo_src.assign(static_cast<size_t>(d->nb) * NS, -1);
+  for (int b = 0; b < d->nb; ++b) {
+    const BlockTab& r = d->bt[b];
+    for (int a = 0; a < 3; ++a) t->origin_cw[4 * b + a] = r.origin[a];
+    t->origin_cw[4 * b + 3] = r.cw;
+    for (int px = 0; px < NP; ++px)
+      for (int py = 0; py < NP; ++py)
+        for (int pz = 0; pz < NP; ++pz) {
+          const int x = px - 1, y = py - 1, z = pz - 1;
+          const bool core = x >= 0 && x < kTile && y >= 0 && y < kTile && z >= 0 && z < kTile;
+          if (core) continue;
+          int cc = 0;
+          const int nbk = ApronNeighbour(r, x, y, z, &cc);
+          if (nbk < 0) continue;
+          const int cx = cc / (kTile * kTile), cy = (cc / kTile) % kTile, cz = cc % kTile;
+          t->halo_src[static_cast<size_t>(b) * NS + (px * NP + py) * NP + pz] =
+              nbk * NS + ((cx + 1) * NP + cy + 1) * NP + cz + 1;
+        }
+  }
+}
+
+}  // namespace octlb
+
 extern "C" {
 
 const char* octlb_last_error(void) { return g_last_error.c_str(); }
