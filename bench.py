@@ -137,6 +137,57 @@ def run_ensemble(precs, steps, warmup, timing_steps, count=64):
     return out
 
 
+def run_adjoint(steps, checkpoint):
+    """Discrete adjoint of the single-level step (octlb_adj_run) on the
+    cylinder at the fine resolution of the 2-level channel, as one level:
+    256 x 64 x 64 cells in 2,048 blocks of 8^3, D = 16 cells, Bouzidi on all
+    cut links, bounce-back walls (closed box, fluid started at
+    feq(1, (0.04, 0, 0))), tau 0.55.  J = mean x-drag over the second half of
+    the run; gradients w.r.t. u0 and every link q, checkpoints every
+    `checkpoint` steps.  A step is one update of the blocks' level (level 3
+    here, so a coarse step of octlb_dev_advance is 8 of them).  The ratio
+    divides the adjoint run's device time (forward sweep + reverse sweep with
+    recompute) by the same number of production FP64 updates of the same case
+    (hipGraph replay of coarse steps)."""
+    from octlb_amd import cases
+    from octlb_amd.adjoint import DragAdjoint
+    from octlb_amd.solver import GpuSolver
+
+    case = cases.cylinder_channel(bricks_x=4, base_level=3, fine=False, bricks_yz=1, xc=1.0, radius=0.125,
+                                  u_in=0.04, tau=0.55, open_ends=False)
+    win = (steps // 2, steps)
+    adj = DragAdjoint(case, steps, win, checkpoint=checkpoint)
+    adj.evaluate()
+    t0 = time.perf_counter()
+    J, du, dq = adj.evaluate()
+    wall = time.perf_counter() - t0
+    fw, bw = adj.timing
+    per_coarse = 2 ** int(case.topo.level.max())  # updates of the one level per coarse step
+    coarse = max(1, steps // per_coarse)
+    fwd = {}
+    for p in (64, 32):
+        sim = GpuSolver(case.topo, case.params(p))
+        sim.set_bc_kinds(case.kinds)
+        sim.set_links(case.link, case.q)
+        sim.init_equilibrium(1.0, case.init_u0, case.init_mask)
+        sim.advance_graph(4)
+        sim.synchronize()
+        t0 = time.perf_counter()
+        sim.advance_graph(coarse)
+        sim.synchronize()
+        fwd[p] = 1e3 * (time.perf_counter() - t0) / (coarse * per_coarse)
+        sim.close()
+    return {"workload": ("cylinder D=16 cells in a closed 256x64x64 box (2,048 blocks of 8^3, one level), "
+                         "Bouzidi on all cut links, bounce-back walls, u0 = 0.04, tau 0.55"),
+            "steps": steps, "window": list(win), "checkpoint": checkpoint, "links": int(len(case.q)),
+            "J": J, "dJ_du0": [float(v) for v in du], "max_abs_dJ_dq": float(np.abs(dq).max()),
+            "adjoint_ms": round(fw + bw, 3), "forward_sweep_ms": round(fw, 3), "reverse_sweep_ms": round(bw, 3),
+            "adjoint_wall_ms": round(1e3 * wall, 3),
+            "forward_fp64_ms_per_update": round(fwd[64], 5), "forward_fp32_ms_per_update": round(fwd[32], 5),
+            "adjoint_over_forward_fp64": round((fw + bw) / (steps * fwd[64]), 3),
+            "adjoint_over_forward_fp32": round((fw + bw) / (steps * fwd[32]), 3)}
+
+
 def cpu_baseline(case, budget_s):
     """FP64 oracle restatement, single thread, on a bounded number of steps."""
     from oracle.binding import OracleSim, setup
@@ -240,6 +291,10 @@ def main():
     ap.add_argument("--no-ensemble", action="store_true",
                     help="skip the 64-cavity ensemble measurement")
     ap.add_argument("--ensemble-steps", type=int, default=40)
+    ap.add_argument("--no-adjoint", action="store_true",
+                    help="skip the adjoint/forward cost measurement")
+    ap.add_argument("--adjoint-steps", type=int, default=200)
+    ap.add_argument("--adjoint-checkpoint", type=int, default=20)
     ap.add_argument("--precision", default="both", choices=["both", "32", "64"])
     ap.add_argument("--calibrate-copy-mb", type=int, default=0,
                     help="run one device-to-device copy of this many MiB first (profiler "
@@ -320,6 +375,12 @@ def main():
         log("bouzidi link build")
         links = run_link_build(not args.no_cpu)
         log(f"  {links['links']} links from {links['triangles']} triangles in {links['gpu_ms']} ms")
+    adjo = None
+    if not args.no_adjoint and args.workload == "cylinder":
+        log("adjoint")
+        adjo = run_adjoint(args.adjoint_steps, args.adjoint_checkpoint)
+        log(f"  {adjo['steps']} steps: adjoint {adjo['adjoint_ms']} ms (forward sweep {adjo['forward_sweep_ms']}), "
+            f"forward FP64 {adjo['forward_fp64_ms_per_update']} ms/update -> ratio {adjo['adjoint_over_forward_fp64']}")
     cpu = None
     if not args.no_cpu:
         log("cpu baseline")
@@ -357,7 +418,7 @@ def main():
         "fp64_mlups": round(res[64]["mlups_feq"], 3) if 64 in res else None,
         "fp32_over_fp64": (round(res[32]["mlups_feq"] / res[64]["mlups_feq"], 3)
                            if 32 in res and 64 in res else None),
-        "adjoint_over_forward": None,
+        "adjoint_over_forward": adjo["adjoint_over_forward_fp64"] if adjo else None,
         "roofline": {
             "bound": "hbm",
             "achieved": round(achieved, 2),
@@ -376,6 +437,7 @@ def main():
         "synthetic_amr": synth,
         "ensemble": ens,
         "link_build": links,
+        "adjoint": adjo,
         "drag_fp32": res[32]["drag"] if 32 in res else None,
         "drag_fp64": res[64]["drag"] if 64 in res else None,
     }

@@ -288,7 +288,9 @@ typedef struct octlb_adj octlb_adj;
 int octlb_adj_create(octlb_dev* dev, octlb_adj** out);
 int octlb_adj_destroy(octlb_adj* adj);
 /* From feq(rho0, u0) on the core cells whose BcKind is in kind_mask (all
- * other slots zero, step 0), run `steps` steps keeping a checkpoint every
+ * other slots zero, step 0), run `steps` updates of the blocks' level (a
+ * forest whose blocks all sit on level L updates 2^L times per coarse step
+ * of octlb_dev_advance), keeping a checkpoint every
  * `checkpoint` steps, and return J = mean over steps win_begin+1 .. win_end
  * of dir . drag(bbox) (octlb_dev_drag's sum), dJ/du0[3] and, for each of the
  * num_links (block, cell, pop) triples in link, dJ/dq of that link's wall

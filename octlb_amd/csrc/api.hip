@@ -1379,7 +1379,8 @@ void ExportAdjTopo(const octlb_dev* d, AdjTopo* t) {
   if (!d || !t) throw std::invalid_argument("null argument");
   if (d->padded || d->n != kTile || d->h != 1)
     throw std::invalid_argument("adjoint: needs blocks of 8^3 cells with a one-cell halo");
-  if (d->max_level != 0) throw std::invalid_argument("adjoint: one level only (no coupling adjoint)");
+  for (int lv : d->level)
+    if (lv != d->max_level) throw std::invalid_argument("adjoint: one level only (no coupling adjoint)");
   if (d->nmembers != 1) throw std::invalid_argument("adjoint: ensembles are not differentiated");
   if (d->sc.handler != OCTLB_HANDLER_CONCRETE || d->sc.const_rho || !d->sc.bouzidi)
     throw std::invalid_argument("adjoint: needs the concrete handler, plain BGK and Bouzidi streaming");
