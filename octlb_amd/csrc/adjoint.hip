@@ -8,24 +8,30 @@
 // window of steps; the parameters are the initial flow velocity u0 of
 // feq(rho0, u0) and every Bouzidi wall fraction q (bouzidi_pull.h:8-26).
 //
-// Layout: one FP64 state holds every block's 10^3 halo-inclusive slots, the
-// reference BlockLattice's view of a block, as
-//   state[(block * 19 + pop) * 1000 + slot]
-// so that threads over slots read each population plane coalesced.  One step
-// is y = C(x) (collide, per cell) followed by x' = M y, where M composes the
-// halo exchange (faces and same-level edges, GhostSchedule) with the pull
-// stream, its halo-shell wrap and the Bouzidi and bounce-back rules
+// One step is y = C(x) (collide, per cell) followed by x' = M y, where M
+// composes the halo exchange (faces and same-level edges, GhostSchedule) with
+// the pull stream, its halo-shell wrap and the Bouzidi and bounce-back rules
 // (block_lattice.cpp:503-720).  M does not depend on the state, so the host
-// builds it once as rows -- a source entry per output entry, or a Bouzidi
-// link -- together with its transpose in CSR form:
-//   forward   KAdjCollide, KAdjStream (rows)
-//   adjoint   KAdjStreamT (CSR of M^T), KAdjLinkGrad (d out / d q on the
-//             Bouzidi rows), KAdjCollideT (the BGK vector-Jacobian product
-//             at the stored pre-collide state)
-// with drag seeds on the post-collide snapshot and the live state.  The
-// forward keeps a checkpoint every K steps; the reverse sweep recomputes each
-// segment of K states from its checkpoint, so memory is (steps/K + K) states.
-// Objective values, dJ/du0 and dJ/dq are accumulated in FP64 in fixed order.
+// builds it once over the reference BlockLattice view (10^3 halo-inclusive
+// slots per block) and keeps only the live part: the core cells and the
+// halo slots that unexchanged shells carry into them.  State layout: per
+// block, its live slots in ascending slot order as population planes,
+//   state[block_offset + pop * live_slots(block) + rank(slot)]
+// so that threads over a block's cells read each plane coalesced.
+//   forward  KAdjCollide (first step of a sweep), then KAdjStep per step:
+//            gather x' = M y through one source index per entry (Bouzidi
+//            rows through a link record), store x', collide it into the
+//            next y (double-buffered)
+//   reverse  KAdjReverse per step: lambda_y = M^T lambda_x through one
+//            reader index per entry (the few entries read more than once
+//            or by a Bouzidi row go through an overflow list), then the BGK
+//            vector-Jacobian product at the stored pre-collide state;
+//            KAdjLinkGrad adds d out / d q on the Bouzidi rows and
+//            KAdjSnapSeed the drag's post-collide seeds, both tiny
+// The forward keeps a checkpoint every K steps; the reverse sweep recomputes
+// each segment of K states from its checkpoint, so memory is (steps/K + K)
+// states.  Objective values, dJ/du0 and dJ/dq are accumulated in FP64 in a
+// fixed order.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -53,13 +59,14 @@ namespace {
 
 constexpr int kN = 8, kNP = 10, kNS = kNP * kNP * kNP, kNC = kN * kN * kN;
 constexpr int kThreads = 256;
-constexpr uint8_t kHalo = 255;  // slot kind of a halo slot
+constexpr uint8_t kHalo = 255;  // kind of a halo slot: carried, never collided
 
 struct AdjLink {
-  int row;     // output entry
-  int bb;      // f_bb_opp source entry
-  int other;   // f_interior (q >= 1/2) or f_same (q < 1/2) source entry
-  int upper;   // q >= 1/2 branch
+  int row;      // output entry
+  int bb;       // f_bb_opp source entry
+  int other;    // f_interior (q >= 1/2) or f_same (q < 1/2) source entry
+  int upper;    // q >= 1/2 branch
+  int bb_cell, bb_pop, ot_cell, ot_pop;  // the same sources as (cell, population)
   double q;
 };
 
@@ -69,9 +76,14 @@ struct DragTerm {
   double w;  // -(dir . c_i)
 };
 
-__device__ __forceinline__ void LoadCell(const double* s, size_t base, double* f) {
+struct Cell {
+  int base;    // entry of population 0
+  int stride;  // live slots of the block (population plane length)
+};
+
+__device__ __forceinline__ void LoadCell(const double* s, Cell c, double* f) {
 #pragma unroll
-  for (int i = 0; i < kQ; ++i) f[i] = s[base + static_cast<size_t>(i) * kNS];
+  for (int i = 0; i < kQ; ++i) f[i] = s[c.base + i * c.stride];
 }
 
 // dispatch_collide for the kinds the adjoint admits (bc_dispatcher.h:32-81)
@@ -89,43 +101,82 @@ __device__ __forceinline__ void Collide(double* f, uint8_t k, double omega) {
   }
 }
 
-__global__ __launch_bounds__(kThreads) void KAdjInit(double* x, const uint8_t* kslot, int nslots, double rho0,
-                                                     double u0x, double u0y, double u0z, uint32_t mask) {
+// l <- transpose of (ProxyRhoU + OlbBgk) at the pre-collide cell f:
+// l_j' = (1 - w) l_j + w (A + sum_d G_d (c_jd - u_d) / rho),
+// A = sum_i l_i dfeq_i/drho, G_d = sum_i l_i dfeq_i/du_d.
+__device__ __forceinline__ void BgkVjp(const double* f, double* l, double omega) {
+  double rho, u[3];
+  ProxyRhoU(f, rho, u);
+  const double usqr = NormSqr(u);
+  double A = 0.0, G0 = 0.0, G1 = 0.0, G2 = 0.0;
+#pragma unroll
+  for (int i = 0; i < kQ; ++i) {
+    const double cu = Cx(i) * u[0] + Cy(i) * u[1] + Cz(i) * u[2];
+    const double w = W<double>(i);
+    A += l[i] * w * (1.0 + 3.0 * cu + 4.5 * cu * cu - 1.5 * usqr);
+    const double g = l[i] * rho * w;
+    G0 += g * (3.0 * Cx(i) + 9.0 * cu * Cx(i) - 3.0 * u[0]);
+    G1 += g * (3.0 * Cy(i) + 9.0 * cu * Cy(i) - 3.0 * u[1]);
+    G2 += g * (3.0 * Cz(i) + 9.0 * cu * Cz(i) - 3.0 * u[2]);
+  }
+  const double base_g = A - (G0 * u[0] + G1 * u[1] + G2 * u[2]) / rho;
+#pragma unroll
+  for (int j = 0; j < kQ; ++j) {
+    const double g = base_g + (G0 * Cx(j) + G1 * Cy(j) + G2 * Cz(j)) / rho;
+    l[j] = (1.0 - omega) * l[j] + omega * g;
+  }
+}
+
+__global__ __launch_bounds__(kThreads) void KAdjInit(double* x, const Cell* cells, const uint8_t* ckind, int ncells,
+                                                     double rho0, double u0x, double u0y, double u0z, uint32_t mask) {
   const int t = blockIdx.x * kThreads + threadIdx.x;
-  if (t >= nslots) return;
-  const size_t base = static_cast<size_t>(t / kNS) * kQ * kNS + t % kNS;
-  const uint8_t k = kslot[t];
+  if (t >= ncells) return;
+  const Cell c = cells[t];
+  const uint8_t k = ckind[t];
   const bool on = k != kHalo && ((mask >> k) & 1u);
   const double u[3] = {u0x, u0y, u0z};
   const double usqr = NormSqr(u);
 #pragma unroll
-  for (int i = 0; i < kQ; ++i) x[base + static_cast<size_t>(i) * kNS] = on ? Feq(i, rho0, u, usqr) : 0.0;
+  for (int i = 0; i < kQ; ++i) x[c.base + i * c.stride] = on ? Feq(i, rho0, u, usqr) : 0.0;
 }
 
-__global__ __launch_bounds__(kThreads) void KAdjCollide(const double* x, double* y, const uint8_t* kslot,
-                                                        int nslots, double omega) {
+__global__ __launch_bounds__(kThreads) void KAdjCollide(const double* x, double* y, const Cell* cells,
+                                                        const uint8_t* ckind, int ncells, double omega) {
   const int t = blockIdx.x * kThreads + threadIdx.x;
-  if (t >= nslots) return;
-  const size_t base = static_cast<size_t>(t / kNS) * kQ * kNS + t % kNS;
+  if (t >= ncells) return;
+  const Cell c = cells[t];
   double f[kQ];
-  LoadCell(x, base, f);
-  Collide(f, kslot[t], omega);
+  LoadCell(x, c, f);
+  Collide(f, ckind[t], omega);
 #pragma unroll
-  for (int i = 0; i < kQ; ++i) y[base + static_cast<size_t>(i) * kNS] = f[i];
+  for (int i = 0; i < kQ; ++i) y[c.base + i * c.stride] = f[i];
 }
 
-__global__ __launch_bounds__(kThreads) void KAdjStream(const double* y, double* x, const int* src,
-                                                       const AdjLink* links, int ntot) {
-  const int e = blockIdx.x * kThreads + threadIdx.x;
-  if (e >= ntot) return;
-  const int s = src[e];
-  if (s >= 0) {
-    x[e] = y[s];
-  } else {
-    const AdjLink L = links[-1 - s];
-    x[e] = L.upper ? BouzidiPull(y[L.bb], y[L.other], y[L.other], L.q)
-                   : BouzidiPull(y[L.bb], y[L.bb], y[L.other], L.q);
+// x' = M y for this cell's entries, stored, then collided into the next y
+__global__ __launch_bounds__(kThreads) void KAdjStep(const double* y, double* x, double* yn, const int* src,
+                                                     const AdjLink* links, const Cell* cells, const uint8_t* ckind,
+                                                     int ncells, double omega) {
+  const int t = blockIdx.x * kThreads + threadIdx.x;
+  if (t >= ncells) return;
+  const Cell c = cells[t];
+  double f[kQ];
+#pragma unroll
+  for (int i = 0; i < kQ; ++i) {
+    const int e = c.base + i * c.stride;
+    const int s = src[e];
+    double v;
+    if (s >= 0) {
+      v = y[s];
+    } else {
+      const AdjLink L = links[-1 - s];
+      v = L.upper ? BouzidiPull(y[L.bb], y[L.other], y[L.other], L.q) : BouzidiPull(y[L.bb], y[L.bb], y[L.other], L.q);
+    }
+    f[i] = v;
+    x[e] = v;
   }
+  Collide(f, ckind[t], omega);
+#pragma unroll
+  for (int i = 0; i < kQ; ++i) yn[c.base + i * c.stride] = f[i];
 }
 
 // sum of w (snap + live) over the drag terms, fixed order (one workgroup)
@@ -146,125 +197,121 @@ __global__ __launch_bounds__(kThreads) void KAdjDrag(const double* x, const doub
   if (threadIdx.x == 0) *out = red[0];
 }
 
-// drag seeds: every term owns its snapshot and live entries (distinct pops)
-__global__ __launch_bounds__(kThreads) void KAdjSeed(double* l, const DragTerm* terms, int nterms, double scale,
-                                                     int live) {
+// live-state drag seeds: every term owns its entry (distinct pops)
+__global__ __launch_bounds__(kThreads) void KAdjSeedLive(double* lx, const DragTerm* terms, int nterms, double scale) {
   const int t = blockIdx.x * kThreads + threadIdx.x;
   if (t >= nterms) return;
   const DragTerm d = terms[t];
-  l[live ? d.live : d.snap] += d.w * scale;
+  lx[d.live] += d.w * scale;
 }
 
-// ly = M^T lx: each source entry gathers the outputs that read it
-__global__ __launch_bounds__(kThreads) void KAdjStreamT(const double* lx, double* ly, const int* off,
-                                                        const int2* ent, const AdjLink* links, int ntot) {
-  const int s = blockIdx.x * kThreads + threadIdx.x;
-  if (s >= ntot) return;
-  double acc = 0.0;
-  for (int k = off[s]; k < off[s + 1]; ++k) {
-    const int2 r = ent[k];
-    const double v = lx[r.x];
-    if (r.y < 0) {
-      acc += v;
-    } else {
-      const AdjLink L = links[r.y >> 1];
-      double c;
-      if (L.upper) {
-        const double a = 1.0 / (2.0 * L.q);
-        c = (r.y & 1) ? 1.0 - a : a;
-      } else {
-        c = (r.y & 1) ? 1.0 - 2.0 * L.q : 2.0 * L.q;
-      }
-      acc += c * v;
-    }
+__device__ __forceinline__ double ReaderCoef(const AdjLink& L, int which) {
+  if (L.upper) {
+    const double a = 1.0 / (2.0 * L.q);
+    return which ? 1.0 - a : a;
   }
-  ly[s] = acc;
+  return which ? 1.0 - 2.0 * L.q : 2.0 * L.q;
 }
 
-__device__ double PostCollide(const double* x, const uint8_t* kslot, int entry, double omega) {
-  const int b = entry / (kQ * kNS), rem = entry % (kQ * kNS);
-  const int pop = rem / kNS, slot = rem % kNS;
+// lxn = C^T (M^T lx) at the pre-collide state xp.  rd[e] >= 0: the one
+// output that copies e; -1: none; -2 - k: overflow list k (several readers
+// or a Bouzidi row) in ovo / ove (row, -1 for a copy or 2 link + which).
+__global__ __launch_bounds__(kThreads) void KAdjReverse(const double* lx, double* lxn, const double* xp,
+                                                        const int* rd, const int* ovo, const int2* ove,
+                                                        const AdjLink* links, const Cell* cells,
+                                                        const uint8_t* ckind, int ncells, double omega) {
+  const int t = blockIdx.x * kThreads + threadIdx.x;
+  if (t >= ncells) return;
+  const Cell c = cells[t];
+  double l[kQ];
+#pragma unroll
+  for (int i = 0; i < kQ; ++i) {
+    const int r = rd[c.base + i * c.stride];
+    double v = 0.0;
+    if (r >= 0) {
+      v = lx[r];
+    } else if (r < -1) {
+      const int k = -2 - r;
+      for (int j = ovo[k]; j < ovo[k + 1]; ++j) {
+        const int2 o = ove[j];
+        v += (o.y < 0 ? 1.0 : ReaderCoef(links[o.y >> 1], o.y & 1)) * lx[o.x];
+      }
+    }
+    l[i] = v;
+  }
+  const uint8_t k = ckind[t];
+  if (k == kBulk) {
+    double f[kQ];
+    LoadCell(xp, c, f);
+    BgkVjp(f, l, omega);
+  } else if (k == kBounceBack) {
+    double tmp[kQ];
+#pragma unroll
+    for (int i = 0; i < kQ; ++i) tmp[i] = l[i];
+#pragma unroll
+    for (int i = 0; i < kQ; ++i) l[i] = tmp[Opp(i)];
+  }
+#pragma unroll
+  for (int i = 0; i < kQ; ++i) lxn[c.base + i * c.stride] = l[i];
+}
+
+// post-collide drag seeds of the step: lxn += C^T s for the seeded (bulk)
+// cells, s = scale * per-population weights
+__global__ __launch_bounds__(kThreads) void KAdjSnapSeed(double* lxn, const double* xp, const int* seed_cell,
+                                                         const double* seed_w, int nseed, const Cell* cells,
+                                                         double omega, double scale) {
+  const int t = blockIdx.x * kThreads + threadIdx.x;
+  if (t >= nseed) return;
+  const Cell c = cells[seed_cell[t]];
+  double f[kQ], l[kQ];
+  LoadCell(xp, c, f);
+#pragma unroll
+  for (int i = 0; i < kQ; ++i) l[i] = scale * seed_w[static_cast<size_t>(t) * kQ + i];
+  BgkVjp(f, l, omega);
+#pragma unroll
+  for (int i = 0; i < kQ; ++i) lxn[c.base + i * c.stride] += l[i];
+}
+
+__device__ double PostCollide(const double* x, const Cell* cells, const uint8_t* ckind, int cell, int pop,
+                              double omega) {
   double f[kQ];
-  LoadCell(x, static_cast<size_t>(b) * kQ * kNS + slot, f);
-  Collide(f, kslot[b * kNS + slot], omega);
+  LoadCell(x, cells[cell], f);
+  Collide(f, ckind[cell], omega);
   double v = 0.0;
 #pragma unroll
   for (int i = 0; i < kQ; ++i) v = (i == pop) ? f[i] : v;
   return v;
 }
 
-// dq += lx[row] * d BouzidiPull / d q, with the pulled values recollided from
-// the pre-collide state of the step
-__global__ __launch_bounds__(kThreads) void KAdjLinkGrad(const double* lx, const double* xp, const uint8_t* kslot,
-                                                         const AdjLink* links, int nlinks, double omega,
-                                                         double* dq) {
+// dq += lx[row] * d BouzidiPull / d q, the pulled values recollided from the
+// step's pre-collide state
+__global__ __launch_bounds__(kThreads) void KAdjLinkGrad(const double* lx, const double* xp, const Cell* cells,
+                                                         const uint8_t* ckind, const AdjLink* links, int nlinks,
+                                                         double omega, double* dq) {
   const int k = blockIdx.x * kThreads + threadIdx.x;
   if (k >= nlinks) return;
   const AdjLink L = links[k];
-  const double fbb = PostCollide(xp, kslot, L.bb, omega);
-  const double fo = PostCollide(xp, kslot, L.other, omega);
+  const double fbb = PostCollide(xp, cells, ckind, L.bb_cell, L.bb_pop, omega);
+  const double fo = PostCollide(xp, cells, ckind, L.ot_cell, L.ot_pop, omega);
   const double d = L.upper ? -(fbb - fo) / (2.0 * L.q * L.q) : 2.0 * fbb - 2.0 * fo;
   dq[k] += lx[L.row] * d;
 }
 
-// lx = C^T ly at the pre-collide state xp.  BGK with (rho, u) from the cell:
-// lx_j = (1 - w) ly_j + w (A + sum_d G_d (c_jd - u_d) / rho),
-// A = sum_i ly_i dfeq_i/drho, G_d = sum_i ly_i dfeq_i/du_d.
-__global__ __launch_bounds__(kThreads) void KAdjCollideT(const double* xp, const double* ly, double* lx,
-                                                         const uint8_t* kslot, int nslots, double omega) {
-  const int t = blockIdx.x * kThreads + threadIdx.x;
-  if (t >= nslots) return;
-  const size_t base = static_cast<size_t>(t / kNS) * kQ * kNS + t % kNS;
-  const uint8_t k = kslot[t];
-  double l[kQ];
-  LoadCell(ly, base, l);
-  if (k == kBulk) {
-    double f[kQ];
-    LoadCell(xp, base, f);
-    double rho, u[3];
-    ProxyRhoU(f, rho, u);
-    const double usqr = NormSqr(u);
-    double A = 0.0, G0 = 0.0, G1 = 0.0, G2 = 0.0;
-#pragma unroll
-    for (int i = 0; i < kQ; ++i) {
-      const double cu = Cx(i) * u[0] + Cy(i) * u[1] + Cz(i) * u[2];
-      const double w = W<double>(i);
-      A += l[i] * w * (1.0 + 3.0 * cu + 4.5 * cu * cu - 1.5 * usqr);
-      const double g = l[i] * rho * w;
-      G0 += g * (3.0 * Cx(i) + 9.0 * cu * Cx(i) - 3.0 * u[0]);
-      G1 += g * (3.0 * Cy(i) + 9.0 * cu * Cy(i) - 3.0 * u[1]);
-      G2 += g * (3.0 * Cz(i) + 9.0 * cu * Cz(i) - 3.0 * u[2]);
-    }
-    const double base_g = A - (G0 * u[0] + G1 * u[1] + G2 * u[2]) / rho;
-#pragma unroll
-    for (int j = 0; j < kQ; ++j) {
-      const double g = base_g + (G0 * Cx(j) + G1 * Cy(j) + G2 * Cz(j)) / rho;
-      lx[base + static_cast<size_t>(j) * kNS] = (1.0 - omega) * l[j] + omega * g;
-    }
-  } else if (k == kBounceBack) {
-#pragma unroll
-    for (int j = 0; j < kQ; ++j) lx[base + static_cast<size_t>(j) * kNS] = l[Opp(j)];
-  } else {
-#pragma unroll
-    for (int j = 0; j < kQ; ++j) lx[base + static_cast<size_t>(j) * kNS] = l[j];
-  }
-}
-
 // dJ/du0 partial sums per workgroup: x0 = feq(rho0, u0) on the masked cells
-__global__ __launch_bounds__(kThreads) void KAdjInitGrad(const double* lx, const uint8_t* kslot, int nslots,
-                                                         double rho0, double u0x, double u0y, double u0z,
+__global__ __launch_bounds__(kThreads) void KAdjInitGrad(const double* lx, const Cell* cells, const uint8_t* ckind,
+                                                         int ncells, double rho0, double u0x, double u0y, double u0z,
                                                          uint32_t mask, double* partial) {
   __shared__ double red[3][kThreads];
   const int t = blockIdx.x * kThreads + threadIdx.x;
   double g[3] = {0.0, 0.0, 0.0};
-  if (t < nslots) {
-    const uint8_t k = kslot[t];
+  if (t < ncells) {
+    const uint8_t k = ckind[t];
     if (k != kHalo && ((mask >> k) & 1u)) {
-      const size_t base = static_cast<size_t>(t / kNS) * kQ * kNS + t % kNS;
+      const Cell c = cells[t];
       const double u[3] = {u0x, u0y, u0z};
 #pragma unroll
       for (int i = 0; i < kQ; ++i) {
-        const double l = lx[base + static_cast<size_t>(i) * kNS];
+        const double l = lx[c.base + i * c.stride];
         const double cu = Cx(i) * u[0] + Cy(i) * u[1] + Cz(i) * u[2];
         const double s = l * rho0 * W<double>(i);
         for (int d = 0; d < 3; ++d) g[d] += s * (3.0 * Ci(i, d) + 9.0 * cu * Ci(i, d) - 3.0 * u[d]);
@@ -290,22 +337,27 @@ inline unsigned Blocks(long long n) { return static_cast<unsigned>((n + kThreads
 using namespace octlb;
 
 struct octlb_adj {
-  int nb = 0, nslots = 0, ntot = 0, nlinks = 0;
+  int nb = 0, ncells = 0, ntot = 0, nlinks = 0;
   double omega = 0.0;
-  std::vector<uint8_t> kslot;            // [nb][1000]
+  std::vector<Cell> cells;
+  std::vector<uint8_t> ckind;
+  std::vector<int> cell_block, cell_slot;
+  std::vector<int> core_cell;  // [nb][512] -> cell
   std::vector<double> origin_cw;
   std::unordered_map<int64_t, int> link_index;  // (block*512 + cell)*19 + pop -> Bouzidi row
   hipStream_t stream = nullptr;
-  uint8_t* d_kslot = nullptr;
+  Cell* d_cells = nullptr;
+  uint8_t* d_ckind = nullptr;
   int* d_src = nullptr;
-  int* d_off = nullptr;
-  int2* d_ent = nullptr;
+  int* d_rd = nullptr;
+  int* d_ovo = nullptr;
+  int2* d_ove = nullptr;
   AdjLink* d_links = nullptr;
   double* d_dq = nullptr;
   double* d_x = nullptr;
-  double* d_y = nullptr;
+  double* d_y[2] = {nullptr, nullptr};
   double* d_lx = nullptr;
-  double* d_ly = nullptr;
+  double* d_ln = nullptr;
   double* d_final = nullptr;  // post-stream state after the last run's forward
   std::vector<void*> owned;
   double fwd_ms = 0.0, bwd_ms = 0.0;
@@ -327,20 +379,16 @@ struct octlb_adj {
   void Build(const AdjTopo& t) {
     nb = t.nb;
     omega = t.omega;
-    nslots = nb * kNS;
-    if (static_cast<long long>(nb) * kQ * kNS >= (1ll << 31)) throw std::invalid_argument("adjoint: too many blocks");
-    ntot = nb * kQ * kNS;
+    const long long nfull = static_cast<long long>(nb) * kQ * kNS;
+    if (nfull >= (1ll << 31)) throw std::invalid_argument("adjoint: too many blocks");
     origin_cw = t.origin_cw;
-    kslot.assign(static_cast<size_t>(nslots), kHalo);
-    auto kind = [&](int b, int ix, int iy, int iz) { return t.kinds[static_cast<size_t>(b) * kNC + (ix * kN + iy) * kN + iz]; };
-    for (int b = 0; b < nb; ++b)
-      for (int ix = 0; ix < kN; ++ix)
-        for (int iy = 0; iy < kN; ++iy)
-          for (int iz = 0; iz < kN; ++iz)
-            kslot[static_cast<size_t>(b) * kNS + ((ix + 1) * kNP + iy + 1) * kNP + iz + 1] = kind(b, ix, iy, iz);
-    std::unordered_map<int64_t, double> q;
-    for (size_t k = 0; k < t.link_key.size(); ++k) q[t.link_key[k]] = t.link_q[k];
-    // rows of M: stream_block (block_lattice.cpp:503-720) over the exchanged halo
+    auto kind = [&](int b, int ix, int iy, int iz) {
+      return t.kinds[static_cast<size_t>(b) * kNC + (ix * kN + iy) * kN + iz];
+    };
+    std::unordered_map<int64_t, double> qmap;
+    for (size_t k = 0; k < t.link_key.size(); ++k) qmap[t.link_key[k]] = t.link_q[k];
+    // ---- rows of M over the full halo view: stream_block
+    // (block_lattice.cpp:503-720) after the exchange
     auto resolve = [&](int b, int slot, int pop) {
       const int hs = t.halo_src[static_cast<size_t>(b) * kNS + slot];
       const int sb = hs >= 0 ? hs / kNS : b, ss = hs >= 0 ? hs % kNS : slot;
@@ -348,7 +396,7 @@ struct octlb_adj {
     };
     auto in_np = [](int v) { return v >= 0 && v < kNP; };
     auto in_core = [](int v) { return v >= 0 && v < kN; };
-    std::vector<int> src(static_cast<size_t>(ntot));
+    std::vector<int> srcf(static_cast<size_t>(nfull));
     std::vector<AdjLink> links;
     for (int b = 0; b < nb; ++b)
       for (int hx = 0; hx < kNP; ++hx)
@@ -369,136 +417,248 @@ struct octlb_adj {
                   const int off = Cx(i) * kNP * kNP + Cy(i) * kNP + Cz(i);
                   s = ((dst - off) % kNS + kNS) % kNS;
                 }
-                src[e] = resolve(b, s, i);
+                srcf[e] = resolve(b, s, i);
                 continue;
               }
               const int k = kind(b, ix, iy, iz);
               if (k == kSolid) {
-                src[e] = e;
+                srcf[e] = e;
                 continue;
               }
               const int self_opp = (b * kQ + Opp(i)) * kNS + dst;
               if (!s_in) {
-                src[e] = ReflectsOnPull(k) ? self_opp : e;
+                srcf[e] = ReflectsOnPull(k) ? self_opp : e;
                 continue;
               }
               const bool s_core = in_core(sx - 1) && in_core(sy - 1) && in_core(sz - 1);
               if (ReflectsOnPull(k) && !s_core) {
-                src[e] = self_opp;
+                srcf[e] = self_opp;
                 continue;
               }
               const int sk = s_core ? kind(b, sx - 1, sy - 1, sz - 1) : kBulk;
               if (sk == kSolid || sk == kBouzidi) {
                 const int64_t key = (static_cast<int64_t>(b) * kNC + (ix * kN + iy) * kN + iz) * kQ + i;
-                const auto it = q.find(key);
-                const double qf = it == q.end() ? 1.0 : it->second;
+                const auto it = qmap.find(key);
+                const double qf = it == qmap.end() ? 1.0 : it->second;
                 if (qf > 1.0e-10 && qf < 1.0 - 1.0e-10) {
                   const int nx = hx + Cx(i), ny = hy + Cy(i), nz = hz + Cz(i);
-                  const int nk = (in_core(nx - 1) && in_core(ny - 1) && in_core(nz - 1)) ? kind(b, nx - 1, ny - 1, nz - 1)
-                                                                                       : kBulk;
-                  AdjLink L;
+                  const int nk = (in_core(nx - 1) && in_core(ny - 1) && in_core(nz - 1))
+                                     ? kind(b, nx - 1, ny - 1, nz - 1)
+                                     : kBulk;
+                  AdjLink L{};
                   L.row = e;
                   L.bb = self_opp;
                   L.upper = qf >= 0.5 - 1.0e-14;
                   L.other = L.upper ? (nk == kBulk ? resolve(b, (nx * kNP + ny) * kNP + nz, i) : e) : e;
                   L.q = qf;
                   link_index[key] = static_cast<int>(links.size());
-                  src[e] = -1 - static_cast<int>(links.size());
+                  srcf[e] = -1 - static_cast<int>(links.size());
                   links.push_back(L);
                   continue;
                 }
-                src[e] = self_opp;
+                srcf[e] = self_opp;
                 continue;
               }
-              src[e] = resolve(b, (sx * kNP + sy) * kNP + sz, i);
+              srcf[e] = resolve(b, (sx * kNP + sy) * kNP + sz, i);
             }
           }
     nlinks = static_cast<int>(links.size());
-    // transpose in CSR form
-    std::vector<int> off(static_cast<size_t>(ntot) + 1, 0);
-    for (int e = 0; e < ntot; ++e) {
-      if (src[e] >= 0) {
-        ++off[src[e] + 1];
-      } else {
-        const AdjLink& L = links[-1 - src[e]];
-        ++off[L.bb + 1];
-        ++off[L.other + 1];
+    // ---- live slots: the core and whatever the core's rows reach
+    std::vector<uint8_t> live(static_cast<size_t>(nb) * kNS, 0);
+    std::vector<int> work;
+    auto mark = [&](int e) {
+      const int b = e / (kQ * kNS), slot = e % kNS;
+      const size_t s = static_cast<size_t>(b) * kNS + slot;
+      if (!live[s]) {
+        live[s] = 1;
+        work.push_back(static_cast<int>(s));
+      }
+    };
+    for (int b = 0; b < nb; ++b)
+      for (int c = 0; c < kNC; ++c) {
+        const int ix = c / (kN * kN), iy = (c / kN) % kN, iz = c % kN;
+        mark((b * kQ) * kNS + ((ix + 1) * kNP + iy + 1) * kNP + iz + 1);
+      }
+    while (!work.empty()) {
+      const int s = work.back();
+      work.pop_back();
+      const int b = s / kNS, slot = s % kNS;
+      for (int i = 0; i < kQ; ++i) {
+        const int r = srcf[(static_cast<size_t>(b) * kQ + i) * kNS + slot];
+        if (r >= 0) {
+          mark(r);
+        } else {
+          mark(links[-1 - r].bb);
+          mark(links[-1 - r].other);
+        }
       }
     }
-    for (int s = 0; s < ntot; ++s) off[s + 1] += off[s];
-    std::vector<int2> ent(static_cast<size_t>(off[ntot]));
-    std::vector<int> fill(off.begin(), off.end() - 1);
+    // ---- compact numbering
+    std::vector<int> rank(static_cast<size_t>(nb) * kNS, -1), boff(nb), cnt(nb), first_cell(nb);
+    int total = 0;
+    cells.clear();
+    ckind.clear();
+    cell_block.clear();
+    cell_slot.clear();
+    core_cell.assign(static_cast<size_t>(nb) * kNC, -1);
+    for (int b = 0; b < nb; ++b) {
+      int r = 0;
+      for (int slot = 0; slot < kNS; ++slot)
+        if (live[static_cast<size_t>(b) * kNS + slot]) rank[static_cast<size_t>(b) * kNS + slot] = r++;
+      cnt[b] = r;
+      boff[b] = total;
+      total += kQ * r;
+      first_cell[b] = static_cast<int>(cells.size());
+      for (int slot = 0; slot < kNS; ++slot) {
+        const int rk = rank[static_cast<size_t>(b) * kNS + slot];
+        if (rk < 0) continue;
+        const int px = slot / (kNP * kNP), py = (slot / kNP) % kNP, pz = slot % kNP;
+        const bool core = in_core(px - 1) && in_core(py - 1) && in_core(pz - 1);
+        if (core) core_cell[static_cast<size_t>(b) * kNC + ((px - 1) * kN + py - 1) * kN + pz - 1] =
+            static_cast<int>(cells.size());
+        cells.push_back(Cell{boff[b] + rk, r});
+        ckind.push_back(core ? kind(b, px - 1, py - 1, pz - 1) : kHalo);
+        cell_block.push_back(b);
+        cell_slot.push_back(slot);
+      }
+    }
+    ntot = total;
+    ncells = static_cast<int>(cells.size());
+    auto cmap = [&](int e) {
+      const int b = e / (kQ * kNS), i = (e / kNS) % kQ, slot = e % kNS;
+      return boff[b] + i * cnt[b] + rank[static_cast<size_t>(b) * kNS + slot];
+    };
+    auto cell_of = [&](int e) {
+      const int b = e / (kQ * kNS), slot = e % kNS;
+      return first_cell[b] + rank[static_cast<size_t>(b) * kNS + slot];
+    };
+    std::vector<int> src(static_cast<size_t>(ntot), 0);
+    for (int b = 0; b < nb; ++b)
+      for (int slot = 0; slot < kNS; ++slot) {
+        if (rank[static_cast<size_t>(b) * kNS + slot] < 0) continue;
+        for (int i = 0; i < kQ; ++i) {
+          const int e = (b * kQ + i) * kNS + slot;
+          src[cmap(e)] = srcf[e] >= 0 ? cmap(srcf[e]) : srcf[e];
+        }
+      }
+    for (AdjLink& L : links) {
+      L.bb_cell = cell_of(L.bb);
+      L.bb_pop = (L.bb / kNS) % kQ;
+      L.ot_cell = cell_of(L.other);
+      L.ot_pop = (L.other / kNS) % kQ;
+      L.row = cmap(L.row);
+      L.bb = cmap(L.bb);
+      L.other = cmap(L.other);
+    }
+    // ---- readers of every entry (the transpose of M)
+    std::vector<int> nread(static_cast<size_t>(ntot), 0), ncopy(static_cast<size_t>(ntot), 0);
     for (int e = 0; e < ntot; ++e) {
       if (src[e] >= 0) {
-        ent[fill[src[e]]++] = make_int2(e, -1);
+        ++nread[src[e]];
+        ++ncopy[src[e]];
+      } else {
+        const AdjLink& L = links[-1 - src[e]];
+        ++nread[L.bb];
+        ++nread[L.other];
+      }
+    }
+    std::vector<int> rd(static_cast<size_t>(ntot), -1), ovo(1, 0), ovidx(static_cast<size_t>(ntot), -1);
+    for (int s = 0; s < ntot; ++s) {
+      if (nread[s] == 0) continue;
+      if (nread[s] == 1 && ncopy[s] == 1) continue;  // filled below
+      ovidx[s] = static_cast<int>(ovo.size()) - 1;
+      ovo.push_back(ovo.back() + nread[s]);
+      rd[s] = -2 - ovidx[s];
+    }
+    std::vector<int2> ove(static_cast<size_t>(ovo.back()));
+    std::vector<int> ofill(ovo.begin(), ovo.end() - 1);
+    for (int e = 0; e < ntot; ++e) {
+      if (src[e] >= 0) {
+        const int s = src[e];
+        if (ovidx[s] < 0) {
+          rd[s] = e;
+        } else {
+          ove[ofill[ovidx[s]]++] = make_int2(e, -1);
+        }
       } else {
         const int lk = -1 - src[e];
         const AdjLink& L = links[lk];
-        ent[fill[L.bb]++] = make_int2(e, 2 * lk);
-        ent[fill[L.other]++] = make_int2(e, 2 * lk + 1);
+        ove[ofill[ovidx[L.bb]]++] = make_int2(e, 2 * lk);
+        ove[ofill[ovidx[L.other]]++] = make_int2(e, 2 * lk + 1);
       }
     }
     ADJ_CHECK(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
-    d_kslot = Alloc<uint8_t>(kslot.size());
+    d_cells = Alloc<Cell>(cells.size());
+    d_ckind = Alloc<uint8_t>(ckind.size());
     d_src = Alloc<int>(src.size());
-    d_off = Alloc<int>(off.size());
-    d_ent = Alloc<int2>(ent.size());
+    d_rd = Alloc<int>(rd.size());
+    d_ovo = Alloc<int>(ovo.size());
+    d_ove = Alloc<int2>(ove.size());
     d_links = Alloc<AdjLink>(links.size());
     d_dq = Alloc<double>(links.size());
     const size_t st = static_cast<size_t>(ntot);
     d_x = Alloc<double>(st);
-    d_y = Alloc<double>(st);
+    d_y[0] = Alloc<double>(st);
+    d_y[1] = Alloc<double>(st);
     d_lx = Alloc<double>(st);
-    d_ly = Alloc<double>(st);
+    d_ln = Alloc<double>(st);
     d_final = Alloc<double>(st);
-    ADJ_CHECK(hipMemcpy(d_kslot, kslot.data(), kslot.size(), hipMemcpyHostToDevice));
+    ADJ_CHECK(hipMemcpy(d_cells, cells.data(), cells.size() * sizeof(Cell), hipMemcpyHostToDevice));
+    ADJ_CHECK(hipMemcpy(d_ckind, ckind.data(), ckind.size(), hipMemcpyHostToDevice));
     ADJ_CHECK(hipMemcpy(d_src, src.data(), src.size() * sizeof(int), hipMemcpyHostToDevice));
-    ADJ_CHECK(hipMemcpy(d_off, off.data(), off.size() * sizeof(int), hipMemcpyHostToDevice));
-    if (!ent.empty()) ADJ_CHECK(hipMemcpy(d_ent, ent.data(), ent.size() * sizeof(int2), hipMemcpyHostToDevice));
+    ADJ_CHECK(hipMemcpy(d_rd, rd.data(), rd.size() * sizeof(int), hipMemcpyHostToDevice));
+    ADJ_CHECK(hipMemcpy(d_ovo, ovo.data(), ovo.size() * sizeof(int), hipMemcpyHostToDevice));
+    if (!ove.empty()) ADJ_CHECK(hipMemcpy(d_ove, ove.data(), ove.size() * sizeof(int2), hipMemcpyHostToDevice));
     if (!links.empty())
       ADJ_CHECK(hipMemcpy(d_links, links.data(), links.size() * sizeof(AdjLink), hipMemcpyHostToDevice));
   }
 
   // drag terms of ol_drag's predicate: fluid cell, in-block solid-like
-  // neighbour along c_i whose centre lies in bbox
-  std::vector<DragTerm> Terms(const double* dir, const double* bbox) const {
-    std::vector<DragTerm> out;
+  // neighbour along c_i whose centre lies in bbox; and per seeded cell the
+  // weights of its post-collide populations
+  void Terms(const double* dir, const double* bbox, std::vector<DragTerm>* terms, std::vector<int>* seed_cell,
+             std::vector<double>* seed_w) const {
     for (int b = 0; b < nb; ++b) {
       const double* o = &origin_cw[4 * static_cast<size_t>(b)];
       for (int ix = 0; ix < kN; ++ix)
         for (int iy = 0; iy < kN; ++iy)
           for (int iz = 0; iz < kN; ++iz) {
-            const int slot = ((ix + 1) * kNP + iy + 1) * kNP + iz + 1;
-            if (kslot[static_cast<size_t>(b) * kNS + slot] != kBulk) continue;
+            const int cell = core_cell[static_cast<size_t>(b) * kNC + (ix * kN + iy) * kN + iz];
+            if (ckind[cell] != kBulk) continue;
+            bool seeded = false;
             for (int i = 1; i < kQ; ++i) {
               const int x = ix + Cx(i), y = iy + Cy(i), z = iz + Cz(i);
               if (x < 0 || x >= kN || y < 0 || y >= kN || z < 0 || z >= kN) continue;
-              const uint8_t nk = kslot[static_cast<size_t>(b) * kNS + ((x + 1) * kNP + y + 1) * kNP + z + 1];
+              const uint8_t nk = ckind[core_cell[static_cast<size_t>(b) * kNC + (x * kN + y) * kN + z]];
               if (nk != kSolid && nk != kBouzidi) continue;
               const double px = o[0] + (x + 0.5) * o[3], py = o[1] + (y + 0.5) * o[3], pz = o[2] + (z + 0.5) * o[3];
               if (!(px >= bbox[0] && px <= bbox[3] && py >= bbox[1] && py <= bbox[4] && pz >= bbox[2] &&
                     pz <= bbox[5]))
                 continue;
+              const Cell c = cells[cell];
               DragTerm d;
-              d.snap = (b * kQ + i) * kNS + slot;
-              d.live = (b * kQ + Opp(i)) * kNS + slot;
+              d.snap = c.base + i * c.stride;
+              d.live = c.base + Opp(i) * c.stride;
               d.w = -(dir[0] * Cx(i) + dir[1] * Cy(i) + dir[2] * Cz(i));
-              out.push_back(d);
+              terms->push_back(d);
+              if (!seeded) {
+                seed_cell->push_back(cell);
+                seed_w->resize(seed_w->size() + kQ, 0.0);
+                seeded = true;
+              }
+              (*seed_w)[seed_w->size() - kQ + i] += d.w;
             }
           }
     }
-    return out;
-  }
-
-  void Step(const double* x, double* y, double* xo) {
-    hipLaunchKernelGGL(KAdjCollide, dim3(Blocks(nslots)), dim3(kThreads), 0, stream, x, y, d_kslot, nslots, omega);
-    hipLaunchKernelGGL(KAdjStream, dim3(Blocks(ntot)), dim3(kThreads), 0, stream, y, xo, d_src, d_links, ntot);
   }
 
   void Run(int steps, int w0, int w1, int K, const double* dir, const double* bbox, double rho0, const double* u0,
            uint32_t mask, double* J, double* du0) {
-    const std::vector<DragTerm> terms = Terms(dir, bbox);
-    const int nterms = static_cast<int>(terms.size());
+    std::vector<DragTerm> terms;
+    std::vector<int> seed_cell;
+    std::vector<double> seed_w;
+    Terms(dir, bbox, &terms, &seed_cell, &seed_w);
+    const int nterms = static_cast<int>(terms.size()), nseed = static_cast<int>(seed_cell.size());
     const int nck = (steps + K - 1) / K;
     const size_t st = static_cast<size_t>(ntot);
     std::vector<void*> run_owned;
@@ -510,57 +670,81 @@ struct octlb_adj {
     };
     try {
       DragTerm* d_terms = static_cast<DragTerm*>(alloc(terms.size() * sizeof(DragTerm)));
+      int* d_seed_cell = static_cast<int*>(alloc(seed_cell.size() * sizeof(int)));
+      double* d_seed_w = static_cast<double*>(alloc(seed_w.size() * sizeof(double)));
       double* d_jv = static_cast<double*>(alloc(static_cast<size_t>(steps + 1) * sizeof(double)));
       double* ck = static_cast<double*>(alloc(static_cast<size_t>(nck) * st * sizeof(double)));
       double* seg = static_cast<double*>(alloc(static_cast<size_t>(K) * st * sizeof(double)));
-      const unsigned nig = Blocks(nslots);
-      double* d_part = static_cast<double*>(alloc(static_cast<size_t>(nig) * 3 * sizeof(double)));
-      if (nterms) ADJ_CHECK(hipMemcpyAsync(d_terms, terms.data(), terms.size() * sizeof(DragTerm), hipMemcpyHostToDevice, stream));
+      const unsigned ncb = Blocks(ncells);
+      double* d_part = static_cast<double*>(alloc(static_cast<size_t>(ncb) * 3 * sizeof(double)));
+      if (nterms) {
+        ADJ_CHECK(hipMemcpyAsync(d_terms, terms.data(), terms.size() * sizeof(DragTerm), hipMemcpyHostToDevice, stream));
+        ADJ_CHECK(hipMemcpyAsync(d_seed_cell, seed_cell.data(), seed_cell.size() * sizeof(int), hipMemcpyHostToDevice,
+                                 stream));
+        ADJ_CHECK(hipMemcpyAsync(d_seed_w, seed_w.data(), seed_w.size() * sizeof(double), hipMemcpyHostToDevice,
+                                 stream));
+      }
       ADJ_CHECK(hipMemsetAsync(d_jv, 0, static_cast<size_t>(steps + 1) * sizeof(double), stream));
       ADJ_CHECK(hipMemsetAsync(d_dq, 0, static_cast<size_t>(nlinks > 0 ? nlinks : 1) * sizeof(double), stream));
       hipEvent_t ev[3];
       for (auto& e : ev) ADJ_CHECK(hipEventCreate(&e));
       ADJ_CHECK(hipEventRecord(ev[0], stream));
-      // forward with checkpoints
-      hipLaunchKernelGGL(KAdjInit, dim3(nig), dim3(kThreads), 0, stream, d_x, d_kslot, nslots, rho0, u0[0], u0[1],
-                         u0[2], mask);
+      // ---- forward: x_s and y_{s+1} = C(x_s) per step, checkpoints x_{cK}
+      hipLaunchKernelGGL(KAdjInit, dim3(ncb), dim3(kThreads), 0, stream, d_x, d_cells, d_ckind, ncells, rho0, u0[0],
+                         u0[1], u0[2], mask);
+      hipLaunchKernelGGL(KAdjCollide, dim3(ncb), dim3(kThreads), 0, stream, d_x, d_y[0], d_cells, d_ckind, ncells,
+                         omega);
+      int cur = 0;
       for (int s = 1; s <= steps; ++s) {
         if ((s - 1) % K == 0)
           ADJ_CHECK(hipMemcpyAsync(ck + static_cast<size_t>((s - 1) / K) * st, d_x, st * sizeof(double),
                                    hipMemcpyDeviceToDevice, stream));
-        Step(d_x, d_y, d_lx);  // the adjoint buffer is free until the reverse sweep
-        std::swap(d_x, d_lx);
+        hipLaunchKernelGGL(KAdjStep, dim3(ncb), dim3(kThreads), 0, stream, d_y[cur], d_x, d_y[cur ^ 1], d_src,
+                           d_links, d_cells, d_ckind, ncells, omega);
         if (s > w0 && s <= w1 && nterms)
-          hipLaunchKernelGGL(KAdjDrag, dim3(1), dim3(kThreads), 0, stream, d_x, d_y, d_terms, nterms, d_jv + s);
+          hipLaunchKernelGGL(KAdjDrag, dim3(1), dim3(kThreads), 0, stream, d_x, d_y[cur], d_terms, nterms, d_jv + s);
+        cur ^= 1;
       }
       ADJ_CHECK(hipMemcpyAsync(d_final, d_x, st * sizeof(double), hipMemcpyDeviceToDevice, stream));
       ADJ_CHECK(hipEventRecord(ev[1], stream));
-      // reverse sweep, one checkpoint segment at a time
+      // ---- reverse, one checkpoint segment at a time
       const double scale = 1.0 / static_cast<double>(w1 - w0);
       ADJ_CHECK(hipMemsetAsync(d_lx, 0, st * sizeof(double), stream));
       for (int c = nck - 1; c >= 0; --c) {
         const int s0 = c * K, s1 = std::min(steps, s0 + K);
         ADJ_CHECK(hipMemcpyAsync(seg, ck + static_cast<size_t>(c) * st, st * sizeof(double), hipMemcpyDeviceToDevice,
                                  stream));
-        for (int j = 1; j < s1 - s0; ++j) Step(seg + (j - 1) * st, d_y, seg + j * st);
+        if (s1 - s0 > 1)
+          hipLaunchKernelGGL(KAdjCollide, dim3(ncb), dim3(kThreads), 0, stream, seg, d_y[0], d_cells, d_ckind, ncells,
+                             omega);
+        int yc = 0;
+        for (int j = 1; j < s1 - s0; ++j) {
+          hipLaunchKernelGGL(KAdjStep, dim3(ncb), dim3(kThreads), 0, stream, d_y[yc], seg + j * st, d_y[yc ^ 1], d_src,
+                             d_links, d_cells, d_ckind, ncells, omega);
+          yc ^= 1;
+        }
         for (int s = s1; s > s0; --s) {
           const double* xp = seg + static_cast<size_t>(s - 1 - s0) * st;
           const bool win = s > w0 && s <= w1 && nterms;
-          if (win) hipLaunchKernelGGL(KAdjSeed, dim3(Blocks(nterms)), dim3(kThreads), 0, stream, d_lx, d_terms, nterms, scale, 1);
-          hipLaunchKernelGGL(KAdjStreamT, dim3(Blocks(ntot)), dim3(kThreads), 0, stream, d_lx, d_ly, d_off, d_ent,
-                             d_links, ntot);
+          if (win)
+            hipLaunchKernelGGL(KAdjSeedLive, dim3(Blocks(nterms)), dim3(kThreads), 0, stream, d_lx, d_terms, nterms,
+                               scale);
           if (nlinks)
-            hipLaunchKernelGGL(KAdjLinkGrad, dim3(Blocks(nlinks)), dim3(kThreads), 0, stream, d_lx, xp, d_kslot,
-                               d_links, nlinks, omega, d_dq);
-          if (win) hipLaunchKernelGGL(KAdjSeed, dim3(Blocks(nterms)), dim3(kThreads), 0, stream, d_ly, d_terms, nterms, scale, 0);
-          hipLaunchKernelGGL(KAdjCollideT, dim3(nig), dim3(kThreads), 0, stream, xp, d_ly, d_lx, d_kslot, nslots, omega);
+            hipLaunchKernelGGL(KAdjLinkGrad, dim3(Blocks(nlinks)), dim3(kThreads), 0, stream, d_lx, xp, d_cells,
+                               d_ckind, d_links, nlinks, omega, d_dq);
+          hipLaunchKernelGGL(KAdjReverse, dim3(ncb), dim3(kThreads), 0, stream, d_lx, d_ln, xp, d_rd, d_ovo, d_ove,
+                             d_links, d_cells, d_ckind, ncells, omega);
+          if (win)
+            hipLaunchKernelGGL(KAdjSnapSeed, dim3(Blocks(nseed)), dim3(kThreads), 0, stream, d_ln, xp, d_seed_cell,
+                               d_seed_w, nseed, d_cells, omega, scale);
+          std::swap(d_lx, d_ln);
         }
       }
-      hipLaunchKernelGGL(KAdjInitGrad, dim3(nig), dim3(kThreads), 0, stream, d_lx, d_kslot, nslots, rho0, u0[0],
-                         u0[1], u0[2], mask, d_part);
+      hipLaunchKernelGGL(KAdjInitGrad, dim3(ncb), dim3(kThreads), 0, stream, d_lx, d_cells, d_ckind, ncells, rho0,
+                         u0[0], u0[1], u0[2], mask, d_part);
       ADJ_CHECK(hipGetLastError());
       ADJ_CHECK(hipEventRecord(ev[2], stream));
-      std::vector<double> jv(static_cast<size_t>(steps + 1)), part(static_cast<size_t>(nig) * 3);
+      std::vector<double> jv(static_cast<size_t>(steps + 1)), part(static_cast<size_t>(ncb) * 3);
       ADJ_CHECK(hipMemcpyAsync(jv.data(), d_jv, jv.size() * sizeof(double), hipMemcpyDeviceToHost, stream));
       ADJ_CHECK(hipMemcpyAsync(part.data(), d_part, part.size() * sizeof(double), hipMemcpyDeviceToHost, stream));
       ADJ_CHECK(hipStreamSynchronize(stream));
@@ -574,7 +758,7 @@ struct octlb_adj {
       for (int s = w0 + 1; s <= w1; ++s) js += jv[s];
       *J = js * scale;
       du0[0] = du0[1] = du0[2] = 0.0;
-      for (unsigned g = 0; g < nig; ++g)
+      for (unsigned g = 0; g < ncb; ++g)
         for (int d = 0; d < 3; ++d) du0[d] += part[3 * g + d];
     } catch (...) {
       (void)hipStreamSynchronize(stream);
@@ -664,10 +848,9 @@ int octlb_adj_final_core(octlb_adj* adj, double* out) {
     ADJ_CHECK(hipMemcpy(s.data(), adj->d_final, s.size() * sizeof(double), hipMemcpyDeviceToHost));
     for (int b = 0; b < adj->nb; ++b)
       for (int c = 0; c < kNC; ++c) {
-        const int ix = c / (kN * kN), iy = (c / kN) % kN, iz = c % kN;
-        const int slot = ((ix + 1) * kNP + iy + 1) * kNP + iz + 1;
+        const Cell cl = adj->cells[adj->core_cell[static_cast<size_t>(b) * kNC + c]];
         for (int i = 0; i < kQ; ++i)
-          out[(static_cast<size_t>(b) * kNC + c) * kQ + i] = s[(static_cast<size_t>(b) * kQ + i) * kNS + slot];
+          out[(static_cast<size_t>(b) * kNC + c) * kQ + i] = s[static_cast<size_t>(cl.base) + static_cast<size_t>(i) * cl.stride];
       }
   });
 }
