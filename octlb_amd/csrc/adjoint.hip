@@ -57,6 +57,25 @@ namespace {
                                " (adjoint.hip:" + std::to_string(__LINE__) + ")");            \
   } while (0)
 
+// A launch's read-only inputs seen through the constant address space: their
+// loads cannot alias its stores, so the compiler batches them.
+#if defined(__HIP_DEVICE_COMPILE__)
+#define ADJ_RO __attribute__((address_space(4)))
+#else
+#define ADJ_RO
+#endif
+template <typename P>
+__device__ __forceinline__ const ADJ_RO P* Ro(const P* p) {
+  return (const ADJ_RO P*)p;
+}
+
+// Consecutive workgroups (consecutive cells of a block and its neighbours
+// in the cell order) go to the same XCD, whose L2 then serves the gathers.
+__device__ __forceinline__ int XcdWg(int w, int n) {
+  const int q8 = n >> 3;
+  return (w < (q8 << 3)) ? (w & 7) * q8 + (w >> 3) : w;
+}
+
 constexpr int kN = 8, kNP = 10, kNS = kNP * kNP * kNP, kNC = kN * kN * kN;
 constexpr int kThreads = 256;
 constexpr uint8_t kHalo = 255;  // kind of a halo slot: carried, never collided
@@ -81,7 +100,8 @@ struct Cell {
   int stride;  // live slots of the block (population plane length)
 };
 
-__device__ __forceinline__ void LoadCell(const double* s, Cell c, double* f) {
+template <typename Ptr>
+__device__ __forceinline__ void LoadCell(Ptr s, Cell c, double* f) {
 #pragma unroll
   for (int i = 0; i < kQ; ++i) f[i] = s[c.base + i * c.stride];
 }
@@ -152,41 +172,46 @@ __global__ __launch_bounds__(kThreads) void KAdjCollide(const double* x, double*
   for (int i = 0; i < kQ; ++i) y[c.base + i * c.stride] = f[i];
 }
 
-// x' = M y for this cell's entries, stored, then collided into the next y
+// entry e of x' = M y
+template <typename PY, typename PS, typename PL>
+__device__ __forceinline__ double RowValue(PY y, PS src, PL links, int e) {
+  const int s = src[e];
+  if (s >= 0) return y[s];
+  const AdjLink L = links[-1 - s];
+  return L.upper ? BouzidiPull(y[L.bb], y[L.other], y[L.other], L.q) : BouzidiPull(y[L.bb], y[L.bb], y[L.other], L.q);
+}
+
+// x' = M y for this cell's entries, stored when x is given, then collided
+// into the next y
 __global__ __launch_bounds__(kThreads) void KAdjStep(const double* y, double* x, double* yn, const int* src,
                                                      const AdjLink* links, const Cell* cells, const uint8_t* ckind,
                                                      int ncells, double omega) {
-  const int t = blockIdx.x * kThreads + threadIdx.x;
+  const int t = XcdWg(blockIdx.x, gridDim.x) * kThreads + threadIdx.x;
   if (t >= ncells) return;
-  const Cell c = cells[t];
+  const Cell c = Ro(cells)[t];
   double f[kQ];
 #pragma unroll
   for (int i = 0; i < kQ; ++i) {
     const int e = c.base + i * c.stride;
-    const int s = src[e];
-    double v;
-    if (s >= 0) {
-      v = y[s];
-    } else {
-      const AdjLink L = links[-1 - s];
-      v = L.upper ? BouzidiPull(y[L.bb], y[L.other], y[L.other], L.q) : BouzidiPull(y[L.bb], y[L.bb], y[L.other], L.q);
-    }
-    f[i] = v;
-    x[e] = v;
+    f[i] = RowValue(Ro(y), Ro(src), Ro(links), e);
+    if (x) x[e] = f[i];
   }
-  Collide(f, ckind[t], omega);
+  Collide(f, Ro(ckind)[t], omega);
 #pragma unroll
   for (int i = 0; i < kQ; ++i) yn[c.base + i * c.stride] = f[i];
 }
 
-// sum of w (snap + live) over the drag terms, fixed order (one workgroup)
-__global__ __launch_bounds__(kThreads) void KAdjDrag(const double* x, const double* y, const DragTerm* terms,
-                                                     int nterms, double* out) {
+// sum of w (snap + live) over the drag terms: one partial per workgroup of
+// consecutive terms (tree order fixed), summed on the host in order; the
+// live value is the stream row of the post-collide state
+__global__ __launch_bounds__(kThreads) void KAdjDrag(const double* y, const int* src, const AdjLink* links,
+                                                     const DragTerm* terms, int nterms, double* out) {
   __shared__ double red[kThreads];
+  const int t = blockIdx.x * kThreads + threadIdx.x;
   double acc = 0.0;
-  for (int t = threadIdx.x; t < nterms; t += kThreads) {
+  if (t < nterms) {
     const DragTerm d = terms[t];
-    acc += d.w * (y[d.snap] + x[d.live]);
+    acc = d.w * (y[d.snap] + RowValue(y, src, links, d.live));
   }
   red[threadIdx.x] = acc;
   __syncthreads();
@@ -194,7 +219,7 @@ __global__ __launch_bounds__(kThreads) void KAdjDrag(const double* x, const doub
     if (static_cast<int>(threadIdx.x) < w) red[threadIdx.x] += red[threadIdx.x + w];
     __syncthreads();
   }
-  if (threadIdx.x == 0) *out = red[0];
+  if (threadIdx.x == 0) out[blockIdx.x] = red[0];
 }
 
 // live-state drag seeds: every term owns its entry (distinct pops)
@@ -205,7 +230,7 @@ __global__ __launch_bounds__(kThreads) void KAdjSeedLive(double* lx, const DragT
   lx[d.live] += d.w * scale;
 }
 
-__device__ __forceinline__ double ReaderCoef(const AdjLink& L, int which) {
+__device__ __forceinline__ double ReaderCoef(AdjLink L, int which) {
   if (L.upper) {
     const double a = 1.0 / (2.0 * L.q);
     return which ? 1.0 - a : a;
@@ -220,29 +245,29 @@ __global__ __launch_bounds__(kThreads) void KAdjReverse(const double* lx, double
                                                         const int* rd, const int* ovo, const int2* ove,
                                                         const AdjLink* links, const Cell* cells,
                                                         const uint8_t* ckind, int ncells, double omega) {
-  const int t = blockIdx.x * kThreads + threadIdx.x;
+  const int t = XcdWg(blockIdx.x, gridDim.x) * kThreads + threadIdx.x;
   if (t >= ncells) return;
-  const Cell c = cells[t];
+  const Cell c = Ro(cells)[t];
   double l[kQ];
 #pragma unroll
   for (int i = 0; i < kQ; ++i) {
-    const int r = rd[c.base + i * c.stride];
+    const int r = Ro(rd)[c.base + i * c.stride];
     double v = 0.0;
     if (r >= 0) {
-      v = lx[r];
+      v = Ro(lx)[r];
     } else if (r < -1) {
       const int k = -2 - r;
-      for (int j = ovo[k]; j < ovo[k + 1]; ++j) {
-        const int2 o = ove[j];
-        v += (o.y < 0 ? 1.0 : ReaderCoef(links[o.y >> 1], o.y & 1)) * lx[o.x];
+      for (int j = Ro(ovo)[k]; j < Ro(ovo)[k + 1]; ++j) {
+        const int2 o = Ro(ove)[j];
+        v += (o.y < 0 ? 1.0 : ReaderCoef(Ro(links)[o.y >> 1], o.y & 1)) * Ro(lx)[o.x];
       }
     }
     l[i] = v;
   }
-  const uint8_t k = ckind[t];
+  const uint8_t k = Ro(ckind)[t];
   if (k == kBulk) {
     double f[kQ];
-    LoadCell(xp, c, f);
+    LoadCell(Ro(xp), c, f);
     BgkVjp(f, l, omega);
   } else if (k == kBounceBack) {
     double tmp[kQ];
@@ -354,7 +379,6 @@ struct octlb_adj {
   int2* d_ove = nullptr;
   AdjLink* d_links = nullptr;
   double* d_dq = nullptr;
-  double* d_x = nullptr;
   double* d_y[2] = {nullptr, nullptr};
   double* d_lx = nullptr;
   double* d_ln = nullptr;
@@ -597,7 +621,6 @@ struct octlb_adj {
     d_links = Alloc<AdjLink>(links.size());
     d_dq = Alloc<double>(links.size());
     const size_t st = static_cast<size_t>(ntot);
-    d_x = Alloc<double>(st);
     d_y[0] = Alloc<double>(st);
     d_y[1] = Alloc<double>(st);
     d_lx = Alloc<double>(st);
@@ -672,7 +695,8 @@ struct octlb_adj {
       DragTerm* d_terms = static_cast<DragTerm*>(alloc(terms.size() * sizeof(DragTerm)));
       int* d_seed_cell = static_cast<int*>(alloc(seed_cell.size() * sizeof(int)));
       double* d_seed_w = static_cast<double*>(alloc(seed_w.size() * sizeof(double)));
-      double* d_jv = static_cast<double*>(alloc(static_cast<size_t>(steps + 1) * sizeof(double)));
+      const int ndg = std::max(1, static_cast<int>(Blocks(nterms)));  // drag partials per step
+      double* d_jv = static_cast<double*>(alloc(static_cast<size_t>(steps + 1) * ndg * sizeof(double)));
       double* ck = static_cast<double*>(alloc(static_cast<size_t>(nck) * st * sizeof(double)));
       double* seg = static_cast<double*>(alloc(static_cast<size_t>(K) * st * sizeof(double)));
       const unsigned ncb = Blocks(ncells);
@@ -684,47 +708,55 @@ struct octlb_adj {
         ADJ_CHECK(hipMemcpyAsync(d_seed_w, seed_w.data(), seed_w.size() * sizeof(double), hipMemcpyHostToDevice,
                                  stream));
       }
-      ADJ_CHECK(hipMemsetAsync(d_jv, 0, static_cast<size_t>(steps + 1) * sizeof(double), stream));
+      ADJ_CHECK(hipMemsetAsync(d_jv, 0, static_cast<size_t>(steps + 1) * ndg * sizeof(double), stream));
       ADJ_CHECK(hipMemsetAsync(d_dq, 0, static_cast<size_t>(nlinks > 0 ? nlinks : 1) * sizeof(double), stream));
       hipEvent_t ev[3];
       for (auto& e : ev) ADJ_CHECK(hipEventCreate(&e));
       ADJ_CHECK(hipEventRecord(ev[0], stream));
-      // ---- forward: x_s and y_{s+1} = C(x_s) per step, checkpoints x_{cK}
-      hipLaunchKernelGGL(KAdjInit, dim3(ncb), dim3(kThreads), 0, stream, d_x, d_cells, d_ckind, ncells, rho0, u0[0],
-                         u0[1], u0[2], mask);
-      hipLaunchKernelGGL(KAdjCollide, dim3(ncb), dim3(kThreads), 0, stream, d_x, d_y[0], d_cells, d_ckind, ncells,
-                         omega);
+      // ---- forward: y_{s+1} = C(M y_s); x_s is stored only where the
+      // reverse sweep starts from it: the checkpoints x_{cK} and every state
+      // of the last segment (which is then not recomputed)
+      const int s_last = (nck - 1) * K;
+      auto xdest = [&](int s) -> double* {
+        if (s == steps) return d_final;
+        if (s >= s_last) return seg + static_cast<size_t>(s - s_last) * st;
+        return s % K == 0 ? ck + static_cast<size_t>(s / K) * st : nullptr;
+      };
+      hipLaunchKernelGGL(KAdjInit, dim3(ncb), dim3(kThreads), 0, stream, xdest(0), d_cells, d_ckind, ncells, rho0,
+                         u0[0], u0[1], u0[2], mask);
+      hipLaunchKernelGGL(KAdjCollide, dim3(ncb), dim3(kThreads), 0, stream, xdest(0), d_y[0], d_cells, d_ckind,
+                         ncells, omega);
       int cur = 0;
       for (int s = 1; s <= steps; ++s) {
-        if ((s - 1) % K == 0)
-          ADJ_CHECK(hipMemcpyAsync(ck + static_cast<size_t>((s - 1) / K) * st, d_x, st * sizeof(double),
-                                   hipMemcpyDeviceToDevice, stream));
-        hipLaunchKernelGGL(KAdjStep, dim3(ncb), dim3(kThreads), 0, stream, d_y[cur], d_x, d_y[cur ^ 1], d_src,
+        hipLaunchKernelGGL(KAdjStep, dim3(ncb), dim3(kThreads), 0, stream, d_y[cur], xdest(s), d_y[cur ^ 1], d_src,
                            d_links, d_cells, d_ckind, ncells, omega);
         if (s > w0 && s <= w1 && nterms)
-          hipLaunchKernelGGL(KAdjDrag, dim3(1), dim3(kThreads), 0, stream, d_x, d_y[cur], d_terms, nterms, d_jv + s);
+          hipLaunchKernelGGL(KAdjDrag, dim3(ndg), dim3(kThreads), 0, stream, d_y[cur], d_src, d_links, d_terms, nterms,
+                             d_jv + static_cast<size_t>(s) * ndg);
         cur ^= 1;
       }
-      ADJ_CHECK(hipMemcpyAsync(d_final, d_x, st * sizeof(double), hipMemcpyDeviceToDevice, stream));
       ADJ_CHECK(hipEventRecord(ev[1], stream));
       // ---- reverse, one checkpoint segment at a time
       const double scale = 1.0 / static_cast<double>(w1 - w0);
       ADJ_CHECK(hipMemsetAsync(d_lx, 0, st * sizeof(double), stream));
       for (int c = nck - 1; c >= 0; --c) {
         const int s0 = c * K, s1 = std::min(steps, s0 + K);
-        ADJ_CHECK(hipMemcpyAsync(seg, ck + static_cast<size_t>(c) * st, st * sizeof(double), hipMemcpyDeviceToDevice,
-                                 stream));
-        if (s1 - s0 > 1)
-          hipLaunchKernelGGL(KAdjCollide, dim3(ncb), dim3(kThreads), 0, stream, seg, d_y[0], d_cells, d_ckind, ncells,
+        // segment states: x_{s0} in its checkpoint (or seg for the last
+        // segment), x_{s0+j} in seg[j], recomputed unless the forward kept them
+        double* x0 = c == nck - 1 ? seg : ck + static_cast<size_t>(c) * st;
+        auto xs = [&](int j) { return j == 0 ? x0 : seg + static_cast<size_t>(j) * st; };
+        if (c != nck - 1 && s1 - s0 > 1) {
+          hipLaunchKernelGGL(KAdjCollide, dim3(ncb), dim3(kThreads), 0, stream, x0, d_y[0], d_cells, d_ckind, ncells,
                              omega);
-        int yc = 0;
-        for (int j = 1; j < s1 - s0; ++j) {
-          hipLaunchKernelGGL(KAdjStep, dim3(ncb), dim3(kThreads), 0, stream, d_y[yc], seg + j * st, d_y[yc ^ 1], d_src,
-                             d_links, d_cells, d_ckind, ncells, omega);
-          yc ^= 1;
+          int yc = 0;
+          for (int j = 1; j < s1 - s0; ++j) {
+            hipLaunchKernelGGL(KAdjStep, dim3(ncb), dim3(kThreads), 0, stream, d_y[yc], xs(j), d_y[yc ^ 1], d_src,
+                               d_links, d_cells, d_ckind, ncells, omega);
+            yc ^= 1;
+          }
         }
         for (int s = s1; s > s0; --s) {
-          const double* xp = seg + static_cast<size_t>(s - 1 - s0) * st;
+          const double* xp = xs(s - 1 - s0);
           const bool win = s > w0 && s <= w1 && nterms;
           if (win)
             hipLaunchKernelGGL(KAdjSeedLive, dim3(Blocks(nterms)), dim3(kThreads), 0, stream, d_lx, d_terms, nterms,
@@ -744,7 +776,7 @@ struct octlb_adj {
                          u0[0], u0[1], u0[2], mask, d_part);
       ADJ_CHECK(hipGetLastError());
       ADJ_CHECK(hipEventRecord(ev[2], stream));
-      std::vector<double> jv(static_cast<size_t>(steps + 1)), part(static_cast<size_t>(ncb) * 3);
+      std::vector<double> jv(static_cast<size_t>(steps + 1) * ndg), part(static_cast<size_t>(ncb) * 3);
       ADJ_CHECK(hipMemcpyAsync(jv.data(), d_jv, jv.size() * sizeof(double), hipMemcpyDeviceToHost, stream));
       ADJ_CHECK(hipMemcpyAsync(part.data(), d_part, part.size() * sizeof(double), hipMemcpyDeviceToHost, stream));
       ADJ_CHECK(hipStreamSynchronize(stream));
@@ -755,7 +787,11 @@ struct octlb_adj {
       bwd_ms = b;
       for (auto& e : ev) ADJ_CHECK(hipEventDestroy(e));
       double js = 0.0;
-      for (int s = w0 + 1; s <= w1; ++s) js += jv[s];
+      for (int s = w0 + 1; s <= w1; ++s) {
+        double step_sum = 0.0;
+        for (int g = 0; g < ndg; ++g) step_sum += jv[static_cast<size_t>(s) * ndg + g];
+        js += step_sum;
+      }
       *J = js * scale;
       du0[0] = du0[1] = du0[2] = 0.0;
       for (unsigned g = 0; g < ncb; ++g)
