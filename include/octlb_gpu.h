@@ -37,6 +37,7 @@ extern "C" {
 #define OCTLB_ENOMEM (-2)  /* host or device allocation failed            */
 #define OCTLB_EHIP (-3)    /* HIP runtime error (message has the details) */
 #define OCTLB_ESTATE (-4)  /* call not valid in the current state         */
+#define OCTLB_EIO (-5)     /* output file could not be written            */
 
 /* FaceDir (reference src/common/types.h:15-22) */
 enum { OCTLB_XMIN = 0, OCTLB_XMAX, OCTLB_YMIN, OCTLB_YMAX, OCTLB_ZMIN, OCTLB_ZMAX };
@@ -271,6 +272,34 @@ int octlb_dev_member_average_rho(octlb_dev* dev, int32_t member, double* out);
 int octlb_build_links(int32_t num_blocks, int32_t n, const double* bounds, const uint8_t* material,
                       int64_t num_triangles, const double* triangles, int64_t capacity,
                       int64_t* num_links, int32_t* links, double* q);
+
+/* ------------------------------------------------------------------ */
+/* Output from device buffers in the reference's VTK formats           */
+/* ------------------------------------------------------------------ */
+/* Density and velocity of every core cell from the device populations,
+ * CellProxy::computeRhoU (block_lattice.h:113-125) as the reference's VTK
+ * samplers take them (vtk_lbm_fields.h:42-80), in VTK cell order:
+ * rho[block][k][j][i], u[block][k][j][i][3] (x fastest).  4 doubles per cell
+ * cross to the host instead of 19 populations. */
+int octlb_dev_macro_fields(octlb_dev* dev, double* rho, double* u);
+/* Cylinder3dCase::write_vtk_timestep (cylinder3d_case.h:878-909): per block
+ * output_dir/base_block<id>_T<iT>.vts with CellData "velocity" (3) and
+ * "pressure" (rho/3), the index output_dir/base_T<iT as %05d>.vtm, and a
+ * DataSet entry base_T<iT>.vtm appended to output_dir/base.pvd (the
+ * reference's names, including the unpadded time step in the .pvd entry).
+ * output_dir must exist. */
+int octlb_dev_write_vtk(octlb_dev* dev, int32_t iT, const char* output_dir, const char* base_name);
+/* WriteStructuredGridVts (structured_grid_writer.cpp:61-117): one block of
+ * nx*ny*nz cells over bounds (x_min, y_min, z_min, x_max, y_max, z_max);
+ * field f has components[f] (1..3) doubles per cell in VTK cell order. */
+int octlb_vtk_write_vts(const char* path, int32_t nx, int32_t ny, int32_t nz, const double bounds[6],
+                        int32_t num_fields, const char* const* names, const int32_t* components,
+                        const double* const* data);
+/* WriteVtmIndex (parallel_vtk_index.cpp:132-158), one rank. */
+int octlb_vtk_write_vtm(const char* path, int32_t count, const char* const* relative_paths);
+/* AppendPvdTimestep (parallel_vtk_index.cpp:160-186): creates the
+ * collection if absent, then appends one DataSet. */
+int octlb_vtk_append_pvd(const char* pvd_path, int32_t iT, const char* vtm_relative_path);
 
 /* ------------------------------------------------------------------ */
 /* Discrete adjoint (no counterpart in the reference, which has none): */

@@ -187,6 +187,12 @@ _SIGNATURES = {
     "octlb_dev_member_average_rho": ([_VP, C.c_int32, _F64P], C.c_int),
     "octlb_build_links": ([C.c_int32, C.c_int32, _F64P, _U8P, C.c_int64, _F64P, C.c_int64,
                            C.POINTER(C.c_int64), _I32P, _F64P], C.c_int),
+    "octlb_dev_macro_fields": ([_VP, _F64P, _F64P], C.c_int),
+    "octlb_dev_write_vtk": ([_VP, C.c_int32, C.c_char_p, C.c_char_p], C.c_int),
+    "octlb_vtk_write_vts": ([C.c_char_p, C.c_int32, C.c_int32, C.c_int32, _F64P, C.c_int32,
+                             C.POINTER(C.c_char_p), _I32P, C.POINTER(_F64P)], C.c_int),
+    "octlb_vtk_write_vtm": ([C.c_char_p, C.c_int32, C.POINTER(C.c_char_p)], C.c_int),
+    "octlb_vtk_append_pvd": ([C.c_char_p, C.c_int32, C.c_char_p], C.c_int),
     "octlb_adj_create": ([_VP, C.POINTER(C.c_void_p)], C.c_int),
     "octlb_adj_destroy": ([_VP], C.c_int),
     "octlb_adj_run": ([_VP, C.c_int32, C.c_int32, C.c_int32, C.c_int32, _F64P, _F64P, C.c_double, _F64P,

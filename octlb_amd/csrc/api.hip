@@ -7,6 +7,7 @@
 #include <algorithm>
 #include <array>
 #include <cmath>
+#include <cstdio>
 #include <cstring>
 #include <map>
 #include <stdexcept>
@@ -1219,6 +1220,8 @@ int Guard(F&& f) {
     return SetError(OCTLB_ESTATE, e.what());
   } catch (const HipError& e) {
     return SetError(OCTLB_EHIP, e.what());
+  } catch (const std::runtime_error& e) {  // output files (vtk_out.cpp)
+    return SetError(OCTLB_EIO, e.what());
   } catch (const std::bad_alloc&) {
     return SetError(OCTLB_ENOMEM, "host allocation failed");
   } catch (const std::exception& e) {
@@ -1365,6 +1368,26 @@ void Mass(octlb_dev* d, double* out) {
   HIP_CHECK(hipGetLastError());
   HIP_CHECK(hipMemcpyAsync(out, d->d_out + 3, sizeof(double), hipMemcpyDeviceToHost, d->stream));
   d->Sync();
+}
+
+// rho [nb][ncore], u [nb][ncore][3] in VTK cell order, on the host
+template <typename T>
+void MacroFields(octlb_dev* d, double* rho, double* u) {
+  const size_t cells = static_cast<size_t>(d->nb) * d->ncore;
+  double* buf = DevAlloc<double>(4 * cells);
+  try {
+    LaunchMacro<T>(static_cast<const T*>(d->pops[0]), static_cast<const T*>(d->pops[1]), d->Par(), d->d_bt, d->sc,
+                   d->nb, buf, buf + cells, d->stream);
+    HIP_CHECK(hipGetLastError());
+    HIP_CHECK(hipMemcpyAsync(rho, buf, cells * sizeof(double), hipMemcpyDeviceToHost, d->stream));
+    HIP_CHECK(hipMemcpyAsync(u, buf + cells, 3 * cells * sizeof(double), hipMemcpyDeviceToHost, d->stream));
+    d->Sync();
+  } catch (...) {
+    (void)hipStreamSynchronize(d->stream);
+    (void)hipFree(buf);
+    throw;
+  }
+  HIP_CHECK(hipFree(buf));
 }
 
 void CheckDev(const octlb_dev* d) {
@@ -1699,6 +1722,52 @@ int octlb_dev_mass(octlb_dev* d, double* out) {
     } else {
       Mass<float>(d, out);
     }
+  });
+}
+
+int octlb_dev_macro_fields(octlb_dev* d, double* rho, double* u) {
+  return Guard([&] {
+    CheckDev(d);
+    if (!rho || !u) throw ArgError("null argument");
+    d->Sync();
+    if (d->prec == 64) {
+      MacroFields<double>(d, rho, u);
+    } else {
+      MacroFields<float>(d, rho, u);
+    }
+  });
+}
+
+int octlb_dev_write_vtk(octlb_dev* d, int32_t iT, const char* output_dir, const char* base_name) {
+  return Guard([&] {
+    CheckDev(d);
+    if (!output_dir || !base_name) throw ArgError("null argument");
+    const size_t nc = static_cast<size_t>(d->ncore);
+    std::vector<double> rho(static_cast<size_t>(d->nb) * nc), u(3 * rho.size()), p(nc);
+    d->Sync();
+    if (d->prec == 64) {
+      MacroFields<double>(d, rho.data(), u.data());
+    } else {
+      MacroFields<float>(d, rho.data(), u.data());
+    }
+    const std::string dir(output_dir), base(base_name), step = std::to_string(iT);
+    std::vector<std::string> rel(static_cast<size_t>(d->nb));
+    const double cs2 = 1.0 / 3.0;  // T{1} / invCs2 (vtk_lbm_fields.h:90-101)
+    for (int b = 0; b < d->nb; ++b) {
+      for (size_t c = 0; c < nc; ++c) p[c] = rho[b * nc + c] * cs2;
+      rel[b] = base + "_block" + std::to_string(b) + "_T" + step + ".vts";
+      const char* names[2] = {"velocity", "pressure"};
+      const int32_t comps[2] = {3, 1};
+      const double* data[2] = {u.data() + 3 * b * nc, p.data()};
+      WriteVts(dir + "/" + rel[b], d->n, d->n, d->n, d->bounds.data() + 6 * static_cast<size_t>(b), 2, names, comps,
+               data);
+    }
+    std::vector<const char*> relp(rel.size());
+    for (size_t b = 0; b < rel.size(); ++b) relp[b] = rel[b].c_str();
+    char tag[16];
+    std::snprintf(tag, sizeof(tag), "%05d", iT);
+    WriteVtm(dir + "/" + base + "_T" + tag + ".vtm", d->nb, relp.data());
+    AppendPvd(dir + "/" + base + ".pvd", iT, base + "_T" + step + ".vtm");
   });
 }
 

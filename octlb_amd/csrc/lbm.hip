@@ -1644,6 +1644,33 @@ void LaunchMass(const T* p0, const T* p1, const ParArray& par, const BlockTab* b
   hipLaunchKernelGGL(KSum1, dim3(1), dim3(1), 0, s, partial, nb, out);
 }
 
+// Density and velocity per core cell for output (CellProxy::computeRhoU,
+// block_lattice.h:113-125), written in VTK cell order: x fastest.
+template <typename T>
+__global__ __launch_bounds__(256) void KMacro(const T* pops0, const T* pops1, ParArray par, const BlockTab* bt,
+                                              SimConst sc, double* rho, double* u) {
+  const int b = blockIdx.x;
+  const T* base = (par.v[bt[b].level] ? pops1 : pops0) + static_cast<size_t>(b) * kQ * sc.plane;
+  const int n = sc.n;
+  for (int t = threadIdx.x; t < sc.ncore; t += blockDim.x) {
+    const int i = t % n, j = (t / n) % n, k = t / (n * n);  // VTK order
+    const int sl = CoreSlot(sc, (i * n + j) * n + k);
+    T f[kQ];
+    for (int q = 0; q < kQ; ++q) f[q] = base[static_cast<size_t>(q) * sc.plane + sl];
+    T r, v[3];
+    ProxyRhoU(f, r, v);
+    const size_t o = static_cast<size_t>(b) * sc.ncore + t;
+    rho[o] = static_cast<double>(r);
+    for (int d = 0; d < 3; ++d) u[3 * o + d] = static_cast<double>(v[d]);
+  }
+}
+
+template <typename T>
+void LaunchMacro(const T* p0, const T* p1, const ParArray& par, const BlockTab* bt, const SimConst& sc, int nb,
+                 double* rho, double* u, hipStream_t s) {
+  hipLaunchKernelGGL(KMacro<T>, dim3(nb), dim3(256), 0, s, p0, p1, par, bt, sc, rho, u);
+}
+
 #define OCTLB_INSTANTIATE(T)                                                                      \
   template void LaunchTile<T>(const TileArgs<T>&, hipStream_t);                                   \
   template void LaunchStatsFinalize<T>(const double*, const int*, int, double*, hipStream_t);     \
@@ -1661,7 +1688,9 @@ void LaunchMass(const T* p0, const T* p1, const ParArray& par, const BlockTab* b
   template void LaunchDrag<T>(const DragArgs<T>&, int, const int*, int, double*, double*,         \
                               hipStream_t);                                                       \
   template void LaunchMass<T>(const T*, const T*, const ParArray&, const BlockTab*,               \
-                              const uint8_t*, const SimConst&, int, double*, double*, hipStream_t);
+                              const uint8_t*, const SimConst&, int, double*, double*, hipStream_t); \
+  template void LaunchMacro<T>(const T*, const T*, const ParArray&, const BlockTab*, const SimConst&, \
+                               int, double*, double*, hipStream_t);
 
 OCTLB_INSTANTIATE(double)
 OCTLB_INSTANTIATE(float)

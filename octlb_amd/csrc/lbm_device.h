@@ -447,6 +447,9 @@ void LaunchDrag(const DragArgs<T>& a, int nlinks, const int* block_begin, int nb
 template <typename T>
 void LaunchMass(const T* p0, const T* p1, const ParArray& par, const BlockTab* bt, const uint8_t* kind,
                 const SimConst& sc, int nb, double* partial, double* out, hipStream_t s);
+template <typename T>
+void LaunchMacro(const T* p0, const T* p1, const ParArray& par, const BlockTab* bt, const SimConst& sc, int nb,
+                 double* rho, double* u, hipStream_t s);
 
 }  // namespace octlb
 

@@ -26,6 +26,12 @@ struct AdjTopo {
   std::vector<int> halo_src;      // [nb][1000] exchanged halo slot -> source block*1000 + slot, or -1
 };
 
+// VTK writers (vtk_out.cpp); throw on bad arguments or file errors.
+void WriteVts(const std::string& path, int nx, int ny, int nz, const double* bounds, int nfields,
+              const char* const* names, const int32_t* comps, const double* const* data);
+void WriteVtm(const std::string& path, int count, const char* const* rel);
+void AppendPvd(const std::string& path, int iT, const std::string& vtm_rel);
+
 // Fills t from dev; throws std::invalid_argument when the handle is outside
 // what the adjoint differentiates.
 void ExportAdjTopo(const struct ::octlb_dev* dev, AdjTopo* t);

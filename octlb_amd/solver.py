@@ -107,6 +107,21 @@ class GpuSolver:
         abi.check(self._lib.octlb_dev_download_core(self._h, abi.f64p(a)), "download_core")
         return a.reshape(self.nb, self.n, self.n, self.n, abi.Q)
 
+    # ---- output (the reference's VTK samplers and writers)
+    def macro_fields(self):
+        """(rho [nb, n, n, n], u [nb, n, n, n, 3]) per core cell in VTK cell
+        order [block][k][j][i] (x fastest)."""
+        rho = np.empty(self.nb * self.ncore)
+        u = np.empty(self.nb * self.ncore * 3)
+        abi.check(self._lib.octlb_dev_macro_fields(self._h, abi.f64p(rho), abi.f64p(u)), "macro_fields")
+        return rho.reshape(self.nb, self.n, self.n, self.n), u.reshape(self.nb, self.n, self.n, self.n, 3)
+
+    def write_vtk(self, it, output_dir, base_name="cylinder3d"):
+        """Cylinder3dCase::write_vtk_timestep: one .vts per block (velocity,
+        pressure), the .vtm index and an entry in base_name.pvd."""
+        abi.check(self._lib.octlb_dev_write_vtk(self._h, int(it), str(output_dir).encode(), base_name.encode()),
+                  "write_vtk")
+
     # ---- stepping
     def advance(self, steps=1):
         abi.check(self._lib.octlb_dev_advance(self._h, int(steps)), "advance")
