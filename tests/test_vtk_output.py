@@ -128,3 +128,15 @@ def test_writer_errors_are_reported(tmp_path):
     assert rc == -1  # OCTLB_EINVAL: non-positive extent
     rc = lib.octlb_vtk_write_vtm(str(tmp_path / "missing" / "x.vtm").encode(), 0, None)
     assert rc == -5 and b"failed to open" in lib.octlb_last_error()  # OCTLB_EIO
+
+
+def test_cd_csv_format(tmp_path):
+    from octlb_amd.output import CdCsv, drag_coefficient
+
+    log = CdCsv(tmp_path)
+    # default C++ stream output of these doubles (precision 6, %g)
+    for step, v in ((100, 6.3612345), (200, 1e-7), (300, 123456789.0), (400, -0.5)):
+        log.append(step, v)
+    assert log.path.read_text() == "iT,Cd\n100,6.36123\n200,1e-07\n300,1.23457e+08\n400,-0.5\n"
+    # Cylinder3dCase::drag_coefficient: 2 (-F . dir) / (rho0 u^2 A)
+    assert drag_coefficient((-0.5, 0.1, 0.0), 1.0, 0.05, 64.0) == 2.0 * 0.5 / (0.05 * 0.05 * 64.0)
