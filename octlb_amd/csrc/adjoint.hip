@@ -172,6 +172,21 @@ __global__ __launch_bounds__(kThreads) void KAdjCollide(const double* x, double*
   for (int i = 0; i < kQ; ++i) y[c.base + i * c.stride] = f[i];
 }
 
+// Regular cells (every population pulled plainly from its geometric
+// neighbour in this block or an exchanged neighbour block): the entry of
+// population i at the cell offset by (dx, dy, dz) is computed from the
+// block's 27 neighbour (base, plane length) pairs -- core cells come first in
+// every block, in (x, y, z) order -- instead of loaded from the index tables.
+// The host sets the flag only where this reproduces the tables entry by
+// entry.
+__device__ __forceinline__ int RegularEntry(const ADJ_RO int2* nbase, int ix, int iy, int iz, int i) {
+  const int ox = ix < 0 ? 0 : (ix > kN - 1 ? 2 : 1);
+  const int oy = iy < 0 ? 0 : (iy > kN - 1 ? 2 : 1);
+  const int oz = iz < 0 ? 0 : (iz > kN - 1 ? 2 : 1);
+  const int2 nb = nbase[(ox * 3 + oy) * 3 + oz];
+  return nb.x + i * nb.y + (((ix & 7) * kN + (iy & 7)) * kN + (iz & 7));
+}
+
 // entry e of x' = M y
 template <typename PY, typename PS, typename PL>
 __device__ __forceinline__ double RowValue(PY y, PS src, PL links, int e) {
@@ -185,16 +200,28 @@ __device__ __forceinline__ double RowValue(PY y, PS src, PL links, int e) {
 // into the next y
 __global__ __launch_bounds__(kThreads) void KAdjStep(const double* y, double* x, double* yn, const int* src,
                                                      const AdjLink* links, const Cell* cells, const uint8_t* ckind,
+                                                     const int* cblock, const uint8_t* cflag, const int2* nbase,
                                                      int ncells, double omega) {
   const int t = XcdWg(blockIdx.x, gridDim.x) * kThreads + threadIdx.x;
   if (t >= ncells) return;
   const Cell c = Ro(cells)[t];
   double f[kQ];
+  if (Ro(cflag)[t] & 1) {
+    const ADJ_RO int2* nb = Ro(nbase) + 27 * Ro(cblock)[t];
+    const int r = c.base - nb[13].x, ix = r >> 6, iy = (r >> 3) & 7, iz = r & 7;
 #pragma unroll
-  for (int i = 0; i < kQ; ++i) {
-    const int e = c.base + i * c.stride;
-    f[i] = RowValue(Ro(y), Ro(src), Ro(links), e);
-    if (x) x[e] = f[i];
+    for (int i = 0; i < kQ; ++i) f[i] = Ro(y)[RegularEntry(nb, ix - Cx(i), iy - Cy(i), iz - Cz(i), i)];
+    if (x) {
+#pragma unroll
+      for (int i = 0; i < kQ; ++i) x[c.base + i * c.stride] = f[i];
+    }
+  } else {
+#pragma unroll
+    for (int i = 0; i < kQ; ++i) {
+      const int e = c.base + i * c.stride;
+      f[i] = RowValue(Ro(y), Ro(src), Ro(links), e);
+      if (x) x[e] = f[i];
+    }
   }
   Collide(f, Ro(ckind)[t], omega);
 #pragma unroll
@@ -244,25 +271,33 @@ __device__ __forceinline__ double ReaderCoef(AdjLink L, int which) {
 __global__ __launch_bounds__(kThreads) void KAdjReverse(const double* lx, double* lxn, const double* xp,
                                                         const int* rd, const int* ovo, const int2* ove,
                                                         const AdjLink* links, const Cell* cells,
-                                                        const uint8_t* ckind, int ncells, double omega) {
+                                                        const uint8_t* ckind, const int* cblock, const uint8_t* cflag,
+                                                        const int2* nbase, int ncells, double omega) {
   const int t = XcdWg(blockIdx.x, gridDim.x) * kThreads + threadIdx.x;
   if (t >= ncells) return;
   const Cell c = Ro(cells)[t];
   double l[kQ];
+  if (Ro(cflag)[t] & 2) {
+    const ADJ_RO int2* nb = Ro(nbase) + 27 * Ro(cblock)[t];
+    const int r = c.base - nb[13].x, ix = r >> 6, iy = (r >> 3) & 7, iz = r & 7;
 #pragma unroll
-  for (int i = 0; i < kQ; ++i) {
-    const int r = Ro(rd)[c.base + i * c.stride];
-    double v = 0.0;
-    if (r >= 0) {
-      v = Ro(lx)[r];
-    } else if (r < -1) {
-      const int k = -2 - r;
-      for (int j = Ro(ovo)[k]; j < Ro(ovo)[k + 1]; ++j) {
-        const int2 o = Ro(ove)[j];
-        v += (o.y < 0 ? 1.0 : ReaderCoef(Ro(links)[o.y >> 1], o.y & 1)) * Ro(lx)[o.x];
+    for (int i = 0; i < kQ; ++i) l[i] = Ro(lx)[RegularEntry(nb, ix + Cx(i), iy + Cy(i), iz + Cz(i), i)];
+  } else {
+#pragma unroll
+    for (int i = 0; i < kQ; ++i) {
+      const int r = Ro(rd)[c.base + i * c.stride];
+      double v = 0.0;
+      if (r >= 0) {
+        v = Ro(lx)[r];
+      } else if (r < -1) {
+        const int k = -2 - r;
+        for (int j = Ro(ovo)[k]; j < Ro(ovo)[k + 1]; ++j) {
+          const int2 o = Ro(ove)[j];
+          v += (o.y < 0 ? 1.0 : ReaderCoef(Ro(links)[o.y >> 1], o.y & 1)) * Ro(lx)[o.x];
+        }
       }
+      l[i] = v;
     }
-    l[i] = v;
   }
   const uint8_t k = Ro(ckind)[t];
   if (k == kBulk) {
@@ -368,10 +403,16 @@ struct octlb_adj {
   std::vector<uint8_t> ckind;
   std::vector<int> cell_block, cell_slot;
   std::vector<int> core_cell;  // [nb][512] -> cell
+  std::vector<int> cblock;     // block of each cell
+  std::vector<uint8_t> cflag;  // bit 0: forward rows regular, bit 1: readers regular
+  std::vector<int2> nbase;     // [nb][27] (entry base, plane length) of the block at (dx, dy, dz), or -1
   std::vector<double> origin_cw;
   std::unordered_map<int64_t, int> link_index;  // (block*512 + cell)*19 + pop -> Bouzidi row
   hipStream_t stream = nullptr;
   Cell* d_cells = nullptr;
+  int* d_cblock = nullptr;
+  uint8_t* d_cflag = nullptr;
+  int2* d_nbase = nullptr;
   uint8_t* d_ckind = nullptr;
   int* d_src = nullptr;
   int* d_rd = nullptr;
@@ -526,16 +567,23 @@ struct octlb_adj {
     cell_slot.clear();
     core_cell.assign(static_cast<size_t>(nb) * kNC, -1);
     for (int b = 0; b < nb; ++b) {
+      // core cells first in (x, y, z) order, then the live halo slots
+      std::vector<int> order;
+      for (int c = 0; c < kNC; ++c)
+        order.push_back(((c / (kN * kN) + 1) * kNP + (c / kN) % kN + 1) * kNP + c % kN + 1);
+      for (int slot = 0; slot < kNS; ++slot) {
+        const int px = slot / (kNP * kNP), py = (slot / kNP) % kNP, pz = slot % kNP;
+        if (live[static_cast<size_t>(b) * kNS + slot] && !(in_core(px - 1) && in_core(py - 1) && in_core(pz - 1)))
+          order.push_back(slot);
+      }
       int r = 0;
-      for (int slot = 0; slot < kNS; ++slot)
-        if (live[static_cast<size_t>(b) * kNS + slot]) rank[static_cast<size_t>(b) * kNS + slot] = r++;
+      for (int slot : order) rank[static_cast<size_t>(b) * kNS + slot] = r++;
       cnt[b] = r;
       boff[b] = total;
       total += kQ * r;
       first_cell[b] = static_cast<int>(cells.size());
-      for (int slot = 0; slot < kNS; ++slot) {
+      for (int slot : order) {
         const int rk = rank[static_cast<size_t>(b) * kNS + slot];
-        if (rk < 0) continue;
         const int px = slot / (kNP * kNP), py = (slot / kNP) % kNP, pz = slot % kNP;
         const bool core = in_core(px - 1) && in_core(py - 1) && in_core(pz - 1);
         if (core) core_cell[static_cast<size_t>(b) * kNC + ((px - 1) * kN + py - 1) * kN + pz - 1] =
@@ -611,8 +659,56 @@ struct octlb_adj {
         ove[ofill[ovidx[L.other]]++] = make_int2(e, 2 * lk + 1);
       }
     }
+    // ---- regular cells: entries computable from 27 neighbour-block bases
+    nbase.assign(static_cast<size_t>(nb) * 27, make_int2(-1, 0));
+    for (int b = 0; b < nb; ++b) {
+      for (int o = 0; o < 27; ++o) {
+        const int dx = o / 9 - 1, dy = (o / 3) % 3 - 1, dz = o % 3 - 1;
+        if (dx == 0 && dy == 0 && dz == 0) {
+          nbase[27 * static_cast<size_t>(b) + o] = make_int2(boff[b], cnt[b]);
+          continue;
+        }
+        // the block that supplies this block's halo slot in that direction
+        const int hx = dx < 0 ? 0 : (dx > 0 ? kNP - 1 : 1), hy = dy < 0 ? 0 : (dy > 0 ? kNP - 1 : 1),
+                  hz = dz < 0 ? 0 : (dz > 0 ? kNP - 1 : 1);
+        const int hs = t.halo_src[static_cast<size_t>(b) * kNS + (hx * kNP + hy) * kNP + hz];
+        if (hs < 0) continue;
+        nbase[27 * static_cast<size_t>(b) + o] = make_int2(boff[hs / kNS], cnt[hs / kNS]);
+      }
+    }
+    auto regular_entry = [&](int b, int ix, int iy, int iz, int i) {
+      const int ox = ix < 0 ? 0 : (ix > kN - 1 ? 2 : 1), oy = iy < 0 ? 0 : (iy > kN - 1 ? 2 : 1),
+                oz = iz < 0 ? 0 : (iz > kN - 1 ? 2 : 1);
+      const int2 nbb = nbase[27 * static_cast<size_t>(b) + (ox * 3 + oy) * 3 + oz];
+      if (nbb.x < 0) return -1;
+      return nbb.x + i * nbb.y + (((ix & 7) * kN + (iy & 7)) * kN + (iz & 7));
+    };
+    cblock.assign(cells.size(), 0);
+    cflag.assign(cells.size(), 0);
+    for (size_t cl = 0; cl < cells.size(); ++cl) {
+      const int b = cell_block[cl];
+      cblock[cl] = b;
+      const int r = cells[cl].base - boff[b];
+      if (r >= kNC) continue;  // halo slot
+      const int ix = r >> 6, iy = (r >> 3) & 7, iz = r & 7;
+      bool fwd = true, rev = true;
+      for (int i = 0; i < kQ; ++i) {
+        const int e = cells[cl].base + i * cells[cl].stride;
+        const int fs = regular_entry(b, ix - Cx(i), iy - Cy(i), iz - Cz(i), i);
+        fwd = fwd && fs >= 0 && src[e] == fs;
+        const int rs = regular_entry(b, ix + Cx(i), iy + Cy(i), iz + Cz(i), i);
+        rev = rev && rs >= 0 && rd[e] == rs;
+      }
+      cflag[cl] = static_cast<uint8_t>((fwd ? 1 : 0) | (rev ? 2 : 0));
+    }
     ADJ_CHECK(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
     d_cells = Alloc<Cell>(cells.size());
+    d_cblock = Alloc<int>(cblock.size());
+    d_cflag = Alloc<uint8_t>(cflag.size());
+    d_nbase = Alloc<int2>(nbase.size());
+    ADJ_CHECK(hipMemcpy(d_cblock, cblock.data(), cblock.size() * sizeof(int), hipMemcpyHostToDevice));
+    ADJ_CHECK(hipMemcpy(d_cflag, cflag.data(), cflag.size(), hipMemcpyHostToDevice));
+    ADJ_CHECK(hipMemcpy(d_nbase, nbase.data(), nbase.size() * sizeof(int2), hipMemcpyHostToDevice));
     d_ckind = Alloc<uint8_t>(ckind.size());
     d_src = Alloc<int>(src.size());
     d_rd = Alloc<int>(rd.size());
@@ -729,7 +825,7 @@ struct octlb_adj {
       int cur = 0;
       for (int s = 1; s <= steps; ++s) {
         hipLaunchKernelGGL(KAdjStep, dim3(ncb), dim3(kThreads), 0, stream, d_y[cur], xdest(s), d_y[cur ^ 1], d_src,
-                           d_links, d_cells, d_ckind, ncells, omega);
+                           d_links, d_cells, d_ckind, d_cblock, d_cflag, d_nbase, ncells, omega);
         if (s > w0 && s <= w1 && nterms)
           hipLaunchKernelGGL(KAdjDrag, dim3(ndg), dim3(kThreads), 0, stream, d_y[cur], d_src, d_links, d_terms, nterms,
                              d_jv + static_cast<size_t>(s) * ndg);
@@ -751,7 +847,7 @@ struct octlb_adj {
           int yc = 0;
           for (int j = 1; j < s1 - s0; ++j) {
             hipLaunchKernelGGL(KAdjStep, dim3(ncb), dim3(kThreads), 0, stream, d_y[yc], xs(j), d_y[yc ^ 1], d_src,
-                               d_links, d_cells, d_ckind, ncells, omega);
+                               d_links, d_cells, d_ckind, d_cblock, d_cflag, d_nbase, ncells, omega);
             yc ^= 1;
           }
         }
@@ -765,7 +861,7 @@ struct octlb_adj {
             hipLaunchKernelGGL(KAdjLinkGrad, dim3(Blocks(nlinks)), dim3(kThreads), 0, stream, d_lx, xp, d_cells,
                                d_ckind, d_links, nlinks, omega, d_dq);
           hipLaunchKernelGGL(KAdjReverse, dim3(ncb), dim3(kThreads), 0, stream, d_lx, d_ln, xp, d_rd, d_ovo, d_ove,
-                             d_links, d_cells, d_ckind, ncells, omega);
+                             d_links, d_cells, d_ckind, d_cblock, d_cflag, d_nbase, ncells, omega);
           if (win)
             hipLaunchKernelGGL(KAdjSnapSeed, dim3(Blocks(nseed)), dim3(kThreads), 0, stream, d_ln, xp, d_seed_cell,
                                d_seed_w, nseed, d_cells, omega, scale);
