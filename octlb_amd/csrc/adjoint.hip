@@ -26,8 +26,8 @@
 //            reader index per entry (the few entries read more than once
 //            or by a Bouzidi row go through an overflow list), then the BGK
 //            vector-Jacobian product at the stored pre-collide state;
-//            KAdjLinkGrad adds d out / d q on the Bouzidi rows and
-//            KAdjSnapSeed the drag's post-collide seeds, both tiny
+//            the drag seeds of the step ride along; KAdjLinkGrad adds
+//            d out / d q on the Bouzidi rows (tiny)
 // The forward keeps a checkpoint every K steps; the reverse sweep recomputes
 // each segment of K states from its checkpoint, so memory is (steps/K + K)
 // states.  Objective values, dJ/du0 and dJ/dq are accumulated in FP64 in a
@@ -272,7 +272,9 @@ __global__ __launch_bounds__(kThreads) void KAdjReverse(const double* lx, double
                                                         const int* rd, const int* ovo, const int2* ove,
                                                         const AdjLink* links, const Cell* cells,
                                                         const uint8_t* ckind, const int* cblock, const uint8_t* cflag,
-                                                        const int2* nbase, int ncells, double omega) {
+                                                        const int2* nbase, const int* cseed, const double* seed_w,
+                                                        double snap_scale, double live_scale, int ncells,
+                                                        double omega) {
   const int t = XcdWg(blockIdx.x, gridDim.x) * kThreads + threadIdx.x;
   if (t >= ncells) return;
   const Cell c = Ro(cells)[t];
@@ -299,6 +301,13 @@ __global__ __launch_bounds__(kThreads) void KAdjReverse(const double* lx, double
       l[i] = v;
     }
   }
+  // drag cells: the step's post-collide seed enters before the collide
+  // transpose, the previous step's live seed after it
+  const int sk = Ro(cseed)[t];
+  if (sk >= 0 && snap_scale != 0.0) {
+#pragma unroll
+    for (int i = 0; i < kQ; ++i) l[i] += snap_scale * Ro(seed_w)[static_cast<size_t>(sk) * 2 * kQ + i];
+  }
   const uint8_t k = Ro(ckind)[t];
   if (k == kBulk) {
     double f[kQ];
@@ -311,25 +320,12 @@ __global__ __launch_bounds__(kThreads) void KAdjReverse(const double* lx, double
 #pragma unroll
     for (int i = 0; i < kQ; ++i) l[i] = tmp[Opp(i)];
   }
+  if (sk >= 0 && live_scale != 0.0) {
+#pragma unroll
+    for (int i = 0; i < kQ; ++i) l[i] += live_scale * Ro(seed_w)[static_cast<size_t>(sk) * 2 * kQ + kQ + i];
+  }
 #pragma unroll
   for (int i = 0; i < kQ; ++i) lxn[c.base + i * c.stride] = l[i];
-}
-
-// post-collide drag seeds of the step: lxn += C^T s for the seeded (bulk)
-// cells, s = scale * per-population weights
-__global__ __launch_bounds__(kThreads) void KAdjSnapSeed(double* lxn, const double* xp, const int* seed_cell,
-                                                         const double* seed_w, int nseed, const Cell* cells,
-                                                         double omega, double scale) {
-  const int t = blockIdx.x * kThreads + threadIdx.x;
-  if (t >= nseed) return;
-  const Cell c = cells[seed_cell[t]];
-  double f[kQ], l[kQ];
-  LoadCell(xp, c, f);
-#pragma unroll
-  for (int i = 0; i < kQ; ++i) l[i] = scale * seed_w[static_cast<size_t>(t) * kQ + i];
-  BgkVjp(f, l, omega);
-#pragma unroll
-  for (int i = 0; i < kQ; ++i) lxn[c.base + i * c.stride] += l[i];
 }
 
 __device__ double PostCollide(const double* x, const Cell* cells, const uint8_t* ckind, int cell, int pop,
@@ -760,12 +756,13 @@ struct octlb_adj {
               d.live = c.base + Opp(i) * c.stride;
               d.w = -(dir[0] * Cx(i) + dir[1] * Cy(i) + dir[2] * Cz(i));
               terms->push_back(d);
-              if (!seeded) {
+              if (!seeded) {  // per drag cell: post-collide weights, then live weights
                 seed_cell->push_back(cell);
-                seed_w->resize(seed_w->size() + kQ, 0.0);
+                seed_w->resize(seed_w->size() + 2 * kQ, 0.0);
                 seeded = true;
               }
-              (*seed_w)[seed_w->size() - kQ + i] += d.w;
+              (*seed_w)[seed_w->size() - 2 * kQ + i] += d.w;
+              (*seed_w)[seed_w->size() - kQ + Opp(i)] += d.w;
             }
           }
     }
@@ -789,7 +786,9 @@ struct octlb_adj {
     };
     try {
       DragTerm* d_terms = static_cast<DragTerm*>(alloc(terms.size() * sizeof(DragTerm)));
-      int* d_seed_cell = static_cast<int*>(alloc(seed_cell.size() * sizeof(int)));
+      std::vector<int> cseed(static_cast<size_t>(ncells), -1);
+      for (int k = 0; k < nseed; ++k) cseed[seed_cell[k]] = k;
+      int* d_cseed = static_cast<int*>(alloc(cseed.size() * sizeof(int)));
       double* d_seed_w = static_cast<double*>(alloc(seed_w.size() * sizeof(double)));
       const int ndg = std::max(1, static_cast<int>(Blocks(nterms)));  // drag partials per step
       double* d_jv = static_cast<double*>(alloc(static_cast<size_t>(steps + 1) * ndg * sizeof(double)));
@@ -799,8 +798,7 @@ struct octlb_adj {
       double* d_part = static_cast<double*>(alloc(static_cast<size_t>(ncb) * 3 * sizeof(double)));
       if (nterms) {
         ADJ_CHECK(hipMemcpyAsync(d_terms, terms.data(), terms.size() * sizeof(DragTerm), hipMemcpyHostToDevice, stream));
-        ADJ_CHECK(hipMemcpyAsync(d_seed_cell, seed_cell.data(), seed_cell.size() * sizeof(int), hipMemcpyHostToDevice,
-                                 stream));
+        ADJ_CHECK(hipMemcpyAsync(d_cseed, cseed.data(), cseed.size() * sizeof(int), hipMemcpyHostToDevice, stream));
         ADJ_CHECK(hipMemcpyAsync(d_seed_w, seed_w.data(), seed_w.size() * sizeof(double), hipMemcpyHostToDevice,
                                  stream));
       }
@@ -835,6 +833,8 @@ struct octlb_adj {
       // ---- reverse, one checkpoint segment at a time
       const double scale = 1.0 / static_cast<double>(w1 - w0);
       ADJ_CHECK(hipMemsetAsync(d_lx, 0, st * sizeof(double), stream));
+      if (steps > w0 && steps <= w1 && nterms)  // live seed of the last step; the others ride in KAdjReverse
+        hipLaunchKernelGGL(KAdjSeedLive, dim3(Blocks(nterms)), dim3(kThreads), 0, stream, d_lx, d_terms, nterms, scale);
       for (int c = nck - 1; c >= 0; --c) {
         const int s0 = c * K, s1 = std::min(steps, s0 + K);
         // segment states: x_{s0} in its checkpoint (or seg for the last
@@ -853,18 +853,13 @@ struct octlb_adj {
         }
         for (int s = s1; s > s0; --s) {
           const double* xp = xs(s - 1 - s0);
-          const bool win = s > w0 && s <= w1 && nterms;
-          if (win)
-            hipLaunchKernelGGL(KAdjSeedLive, dim3(Blocks(nterms)), dim3(kThreads), 0, stream, d_lx, d_terms, nterms,
-                               scale);
+          auto win = [&](int st) { return st > w0 && st <= w1 && nterms > 0; };
           if (nlinks)
             hipLaunchKernelGGL(KAdjLinkGrad, dim3(Blocks(nlinks)), dim3(kThreads), 0, stream, d_lx, xp, d_cells,
                                d_ckind, d_links, nlinks, omega, d_dq);
           hipLaunchKernelGGL(KAdjReverse, dim3(ncb), dim3(kThreads), 0, stream, d_lx, d_ln, xp, d_rd, d_ovo, d_ove,
-                             d_links, d_cells, d_ckind, d_cblock, d_cflag, d_nbase, ncells, omega);
-          if (win)
-            hipLaunchKernelGGL(KAdjSnapSeed, dim3(Blocks(nseed)), dim3(kThreads), 0, stream, d_ln, xp, d_seed_cell,
-                               d_seed_w, nseed, d_cells, omega, scale);
+                             d_links, d_cells, d_ckind, d_cblock, d_cflag, d_nbase, d_cseed, d_seed_w,
+                             win(s) ? scale : 0.0, win(s - 1) ? scale : 0.0, ncells, omega);
           std::swap(d_lx, d_ln);
         }
       }
