@@ -293,8 +293,9 @@ def main():
     ap.add_argument("--ensemble-steps", type=int, default=40)
     ap.add_argument("--no-adjoint", action="store_true",
                     help="skip the adjoint/forward cost measurement")
-    ap.add_argument("--adjoint-steps", type=int, default=200)
-    ap.add_argument("--adjoint-checkpoint", type=int, default=20)
+    ap.add_argument("--adjoint-steps", type=int, default=4000,
+                    help="adjoint run length; J averages the drag over its second half")
+    ap.add_argument("--adjoint-checkpoint", type=int, default=100)
     ap.add_argument("--precision", default="both", choices=["both", "32", "64"])
     ap.add_argument("--calibrate-copy-mb", type=int, default=0,
                     help="run one device-to-device copy of this many MiB first (profiler "
