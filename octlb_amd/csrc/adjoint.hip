@@ -139,10 +139,12 @@ __device__ __forceinline__ void BgkVjp(const double* f, double* l, double omega)
     G1 += g * (3.0 * Cy(i) + 9.0 * cu * Cy(i) - 3.0 * u[1]);
     G2 += g * (3.0 * Cz(i) + 9.0 * cu * Cz(i) - 3.0 * u[2]);
   }
-  const double base_g = A - (G0 * u[0] + G1 * u[1] + G2 * u[2]) / rho;
+  const double inv_rho = 1.0 / rho;  // one division per cell
+  const double h0 = G0 * inv_rho, h1 = G1 * inv_rho, h2 = G2 * inv_rho;
+  const double base_g = A - (h0 * u[0] + h1 * u[1] + h2 * u[2]);
 #pragma unroll
   for (int j = 0; j < kQ; ++j) {
-    const double g = base_g + (G0 * Cx(j) + G1 * Cy(j) + G2 * Cz(j)) / rho;
+    const double g = base_g + (h0 * Cx(j) + h1 * Cy(j) + h2 * Cz(j));
     l[j] = (1.0 - omega) * l[j] + omega * g;
   }
 }
