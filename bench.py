@@ -162,6 +162,10 @@ def run_adjoint(steps, checkpoint):
     J, du, dq = adj.evaluate()
     wall = time.perf_counter() - t0
     fw, bw = adj.timing
+    # FP32 forward states, FP64 adjoint (the C5 precision mode)
+    adj32 = DragAdjoint(case, steps, win, checkpoint=checkpoint, precision=32)
+    J32, du32, dq32 = adj32.evaluate()
+    fw32, bw32 = adj32.timing
     per_coarse = 2 ** int(case.topo.level.max())  # updates of the one level per coarse step
     coarse = max(1, steps // per_coarse)
     fwd = {}
@@ -185,7 +189,15 @@ def run_adjoint(steps, checkpoint):
             "adjoint_wall_ms": round(1e3 * wall, 3),
             "forward_fp64_ms_per_update": round(fwd[64], 5), "forward_fp32_ms_per_update": round(fwd[32], 5),
             "adjoint_over_forward_fp64": round((fw + bw) / (steps * fwd[64]), 3),
-            "adjoint_over_forward_fp32": round((fw + bw) / (steps * fwd[32]), 3)}
+            "adjoint_over_forward_fp32": round((fw + bw) / (steps * fwd[32]), 3),
+            "fp32_forward": {"adjoint_ms": round(fw32 + bw32, 3), "forward_sweep_ms": round(fw32, 3),
+                             "reverse_sweep_ms": round(bw32, 3),
+                             "over_forward_fp64": round((fw32 + bw32) / (steps * fwd[64]), 3),
+                             "over_forward_fp32": round((fw32 + bw32) / (steps * fwd[32]), 3),
+                             "rel_err_vs_fp64_adjoint": {
+                                 "J": abs(J32 - J) / abs(J),
+                                 "dJ_du0": abs(du32[0] - du[0]) / abs(du[0]),
+                                 "dJ_dq_max": float(np.abs(dq32 - dq).max() / np.abs(dq).max())}}}
 
 
 def cpu_baseline(case, budget_s):
@@ -380,6 +392,8 @@ def main():
     if not args.no_adjoint and args.workload == "cylinder":
         log("adjoint")
         adjo = run_adjoint(args.adjoint_steps, args.adjoint_checkpoint)
+        log(f"  FP32 forward / FP64 adjoint: {adjo['fp32_forward']['adjoint_ms']} ms -> "
+            f"{adjo['fp32_forward']['over_forward_fp64']}x FP64 forward, errors {adjo['fp32_forward']['rel_err_vs_fp64_adjoint']}")
         log(f"  {adjo['steps']} steps: adjoint {adjo['adjoint_ms']} ms (forward sweep {adjo['forward_sweep_ms']}), "
             f"forward FP64 {adjo['forward_fp64_ms_per_update']} ms/update -> ratio {adjo['adjoint_over_forward_fp64']}")
     cpu = None

@@ -323,8 +323,12 @@ int octlb_adj_destroy(octlb_adj* adj);
  * `checkpoint` steps, and return J = mean over steps win_begin+1 .. win_end
  * of dir . drag(bbox) (octlb_dev_drag's sum), dJ/du0[3] and, for each of the
  * num_links (block, cell, pop) triples in link, dJ/dq of that link's wall
- * fraction (0 for a triple that is not a Bouzidi row).  FP64 throughout;
- * device memory ~ (steps/checkpoint + checkpoint + 6) x 152 kB per block. */
+ * fraction (0 for a triple that is not a Bouzidi row).  The forward sweep
+ * and its recompute run in the handle's precision (an FP32 handle: FP32
+ * states and collisions, the FP32 path's arithmetic); adjoint variables,
+ * vector-Jacobian products and all sums are FP64.  Device memory ~
+ * (steps/checkpoint + checkpoint + 3) states of 19 x live slots x 4 or 8 B
+ * plus 3 FP64 adjoint states. */
 int octlb_adj_run(octlb_adj* adj, int32_t steps, int32_t win_begin, int32_t win_end, int32_t checkpoint,
                   const double dir[3], const double bbox[6], double rho0, const double u0[3],
                   uint32_t kind_mask, int64_t num_links, const int32_t* link, double* J,

@@ -25,17 +25,18 @@ from .solver import GpuSolver
 
 class DragAdjoint:
     def __init__(self, case, steps: int, window: tuple, checkpoint: int = 100, direction=(1.0, 0.0, 0.0),
-                 rho0: float = 1.0):
+                 rho0: float = 1.0, precision: int = 64):
         self.case = case
         self.steps = int(steps)
         self.window = (int(window[0]), int(window[1]))
         self.checkpoint = int(checkpoint)
         self.direction = np.ascontiguousarray(direction, dtype=np.float64)
         self.rho0 = float(rho0)
+        self.precision = int(precision)  # forward states: 64, or 32 with the adjoint still in FP64
         self.timing = (0.0, 0.0)
 
     def _handles(self, q):
-        sol = GpuSolver(self.case.topo, self.case.params(64))
+        sol = GpuSolver(self.case.topo, self.case.params(self.precision))
         sol.set_bc_kinds(self.case.kinds)
         if len(self.case.link):
             sol.set_links(self.case.link, q)

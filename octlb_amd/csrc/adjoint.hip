@@ -30,8 +30,10 @@
 //            d out / d q on the Bouzidi rows (tiny)
 // The forward keeps a checkpoint every K steps; the reverse sweep recomputes
 // each segment of K states from its checkpoint, so memory is (steps/K + K)
-// states.  Objective values, dJ/du0 and dJ/dq are accumulated in FP64 in a
-// fixed order.
+// states.  The forward states take the handle's precision (FP32: the FP32
+// path's collide arithmetic, Bouzidi rows in FP64 rounded to FP32, as the
+// forward does); adjoint variables, vector-Jacobian products, objective
+// values, dJ/du0 and dJ/dq are FP64, accumulated in a fixed order.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -100,20 +102,22 @@ struct Cell {
   int stride;  // live slots of the block (population plane length)
 };
 
-template <typename Ptr>
-__device__ __forceinline__ void LoadCell(Ptr s, Cell c, double* f) {
+template <typename Ptr, typename V>
+__device__ __forceinline__ void LoadCell(Ptr s, Cell c, V* f) {
 #pragma unroll
   for (int i = 0; i < kQ; ++i) f[i] = s[c.base + i * c.stride];
 }
 
-// dispatch_collide for the kinds the adjoint admits (bc_dispatcher.h:32-81)
-__device__ __forceinline__ void Collide(double* f, uint8_t k, double omega) {
+// dispatch_collide for the kinds the adjoint admits (bc_dispatcher.h:32-81),
+// in the state's precision like the forward path
+template <typename S>
+__device__ __forceinline__ void Collide(S* f, uint8_t k, double omega) {
   if (k == kBulk) {
-    double rho, u[3];
+    S rho, u[3];
     ProxyRhoU(f, rho, u);
-    OlbBgk(f, rho, u, omega);
+    OlbBgk(f, rho, u, static_cast<S>(omega));
   } else if (k == kBounceBack) {
-    double t[kQ];
+    S t[kQ];
 #pragma unroll
     for (int i = 0; i < kQ; ++i) t[i] = f[i];
 #pragma unroll
@@ -149,7 +153,8 @@ __device__ __forceinline__ void BgkVjp(const double* f, double* l, double omega)
   }
 }
 
-__global__ __launch_bounds__(kThreads) void KAdjInit(double* x, const Cell* cells, const uint8_t* ckind, int ncells,
+template <typename S>
+__global__ __launch_bounds__(kThreads) void KAdjInit(S* x, const Cell* cells, const uint8_t* ckind, int ncells,
                                                      double rho0, double u0x, double u0y, double u0z, uint32_t mask) {
   const int t = blockIdx.x * kThreads + threadIdx.x;
   if (t >= ncells) return;
@@ -159,15 +164,16 @@ __global__ __launch_bounds__(kThreads) void KAdjInit(double* x, const Cell* cell
   const double u[3] = {u0x, u0y, u0z};
   const double usqr = NormSqr(u);
 #pragma unroll
-  for (int i = 0; i < kQ; ++i) x[c.base + i * c.stride] = on ? Feq(i, rho0, u, usqr) : 0.0;
+  for (int i = 0; i < kQ; ++i) x[c.base + i * c.stride] = on ? static_cast<S>(Feq<double>(i, rho0, u, usqr)) : S(0);
 }
 
-__global__ __launch_bounds__(kThreads) void KAdjCollide(const double* x, double* y, const Cell* cells,
+template <typename S>
+__global__ __launch_bounds__(kThreads) void KAdjCollide(const S* x, S* y, const Cell* cells,
                                                         const uint8_t* ckind, int ncells, double omega) {
   const int t = blockIdx.x * kThreads + threadIdx.x;
   if (t >= ncells) return;
   const Cell c = cells[t];
-  double f[kQ];
+  S f[kQ];
   LoadCell(x, c, f);
   Collide(f, ckind[t], omega);
 #pragma unroll
@@ -189,25 +195,28 @@ __device__ __forceinline__ int RegularEntry(const ADJ_RO int2* nbase, int ix, in
   return nb.x + i * nb.y + (((ix & 7) * kN + (iy & 7)) * kN + (iz & 7));
 }
 
-// entry e of x' = M y
-template <typename PY, typename PS, typename PL>
-__device__ __forceinline__ double RowValue(PY y, PS src, PL links, int e) {
+// entry e of x' = M y, rounded to the state's precision (Bouzidi rows are
+// evaluated in FP64 for every storage type, as the forward does)
+template <typename S, typename PY, typename PS, typename PL>
+__device__ __forceinline__ S RowValue(PY y, PS src, PL links, int e) {
   const int s = src[e];
   if (s >= 0) return y[s];
   const AdjLink L = links[-1 - s];
-  return L.upper ? BouzidiPull(y[L.bb], y[L.other], y[L.other], L.q) : BouzidiPull(y[L.bb], y[L.bb], y[L.other], L.q);
+  const double a = y[L.bb], o = y[L.other];
+  return static_cast<S>(L.upper ? BouzidiPull(a, o, o, L.q) : BouzidiPull(a, a, o, L.q));
 }
 
 // x' = M y for this cell's entries, stored when x is given, then collided
 // into the next y
-__global__ __launch_bounds__(kThreads) void KAdjStep(const double* y, double* x, double* yn, const int* src,
+template <typename S>
+__global__ __launch_bounds__(kThreads) void KAdjStep(const S* y, S* x, S* yn, const int* src,
                                                      const AdjLink* links, const Cell* cells, const uint8_t* ckind,
                                                      const int* cblock, const uint8_t* cflag, const int2* nbase,
                                                      int ncells, double omega) {
   const int t = XcdWg(blockIdx.x, gridDim.x) * kThreads + threadIdx.x;
   if (t >= ncells) return;
   const Cell c = Ro(cells)[t];
-  double f[kQ];
+  S f[kQ];
   if (Ro(cflag)[t] & 1) {
     const ADJ_RO int2* nb = Ro(nbase) + 27 * Ro(cblock)[t];
     const int r = c.base - nb[13].x, ix = r >> 6, iy = (r >> 3) & 7, iz = r & 7;
@@ -221,7 +230,7 @@ __global__ __launch_bounds__(kThreads) void KAdjStep(const double* y, double* x,
 #pragma unroll
     for (int i = 0; i < kQ; ++i) {
       const int e = c.base + i * c.stride;
-      f[i] = RowValue(Ro(y), Ro(src), Ro(links), e);
+      f[i] = RowValue<S>(Ro(y), Ro(src), Ro(links), e);
       if (x) x[e] = f[i];
     }
   }
@@ -233,14 +242,15 @@ __global__ __launch_bounds__(kThreads) void KAdjStep(const double* y, double* x,
 // sum of w (snap + live) over the drag terms: one partial per workgroup of
 // consecutive terms (tree order fixed), summed on the host in order; the
 // live value is the stream row of the post-collide state
-__global__ __launch_bounds__(kThreads) void KAdjDrag(const double* y, const int* src, const AdjLink* links,
+template <typename S>
+__global__ __launch_bounds__(kThreads) void KAdjDrag(const S* y, const int* src, const AdjLink* links,
                                                      const DragTerm* terms, int nterms, double* out) {
   __shared__ double red[kThreads];
   const int t = blockIdx.x * kThreads + threadIdx.x;
   double acc = 0.0;
   if (t < nterms) {
     const DragTerm d = terms[t];
-    acc = d.w * (y[d.snap] + RowValue(y, src, links, d.live));
+    acc = d.w * (static_cast<double>(y[d.snap]) + static_cast<double>(RowValue<S>(y, src, links, d.live)));
   }
   red[threadIdx.x] = acc;
   __syncthreads();
@@ -270,7 +280,8 @@ __device__ __forceinline__ double ReaderCoef(AdjLink L, int which) {
 // lxn = C^T (M^T lx) at the pre-collide state xp.  rd[e] >= 0: the one
 // output that copies e; -1: none; -2 - k: overflow list k (several readers
 // or a Bouzidi row) in ovo / ove (row, -1 for a copy or 2 link + which).
-__global__ __launch_bounds__(kThreads) void KAdjReverse(const double* lx, double* lxn, const double* xp,
+template <typename S>
+__global__ __launch_bounds__(kThreads) void KAdjReverse(const double* lx, double* lxn, const S* xp,
                                                         const int* rd, const int* ovo, const int2* ove,
                                                         const AdjLink* links, const Cell* cells,
                                                         const uint8_t* ckind, const int* cblock, const uint8_t* cflag,
@@ -330,20 +341,21 @@ __global__ __launch_bounds__(kThreads) void KAdjReverse(const double* lx, double
   for (int i = 0; i < kQ; ++i) lxn[c.base + i * c.stride] = l[i];
 }
 
-__device__ double PostCollide(const double* x, const Cell* cells, const uint8_t* ckind, int cell, int pop,
-                              double omega) {
-  double f[kQ];
+template <typename S>
+__device__ double PostCollide(const S* x, const Cell* cells, const uint8_t* ckind, int cell, int pop, double omega) {
+  S f[kQ];
   LoadCell(x, cells[cell], f);
   Collide(f, ckind[cell], omega);
   double v = 0.0;
 #pragma unroll
-  for (int i = 0; i < kQ; ++i) v = (i == pop) ? f[i] : v;
+  for (int i = 0; i < kQ; ++i) v = (i == pop) ? static_cast<double>(f[i]) : v;
   return v;
 }
 
 // dq += lx[row] * d BouzidiPull / d q, the pulled values recollided from the
 // step's pre-collide state
-__global__ __launch_bounds__(kThreads) void KAdjLinkGrad(const double* lx, const double* xp, const Cell* cells,
+template <typename S>
+__global__ __launch_bounds__(kThreads) void KAdjLinkGrad(const double* lx, const S* xp, const Cell* cells,
                                                          const uint8_t* ckind, const AdjLink* links, int nlinks,
                                                          double omega, double* dq) {
   const int k = blockIdx.x * kThreads + threadIdx.x;
@@ -418,10 +430,12 @@ struct octlb_adj {
   int2* d_ove = nullptr;
   AdjLink* d_links = nullptr;
   double* d_dq = nullptr;
-  double* d_y[2] = {nullptr, nullptr};
+  void* d_y[2] = {nullptr, nullptr};  // post-collide states (state precision)
   double* d_lx = nullptr;
   double* d_ln = nullptr;
-  double* d_final = nullptr;  // post-stream state after the last run's forward
+  void* d_final = nullptr;  // post-stream state after the last run's forward
+  int prec = 64;            // state precision: 64, or 32 (FP32 forward, FP64 adjoint)
+  size_t selem = 8;
   std::vector<void*> owned;
   double fwd_ms = 0.0, bwd_ms = 0.0;
 
@@ -442,6 +456,8 @@ struct octlb_adj {
   void Build(const AdjTopo& t) {
     nb = t.nb;
     omega = t.omega;
+    prec = t.prec;
+    selem = prec == 32 ? sizeof(float) : sizeof(double);
     const long long nfull = static_cast<long long>(nb) * kQ * kNS;
     if (nfull >= (1ll << 31)) throw std::invalid_argument("adjoint: too many blocks");
     origin_cw = t.origin_cw;
@@ -715,11 +731,11 @@ struct octlb_adj {
     d_links = Alloc<AdjLink>(links.size());
     d_dq = Alloc<double>(links.size());
     const size_t st = static_cast<size_t>(ntot);
-    d_y[0] = Alloc<double>(st);
-    d_y[1] = Alloc<double>(st);
+    d_y[0] = Alloc<unsigned char>(st * selem);
+    d_y[1] = Alloc<unsigned char>(st * selem);
     d_lx = Alloc<double>(st);
     d_ln = Alloc<double>(st);
-    d_final = Alloc<double>(st);
+    d_final = Alloc<unsigned char>(st * selem);
     ADJ_CHECK(hipMemcpy(d_cells, cells.data(), cells.size() * sizeof(Cell), hipMemcpyHostToDevice));
     ADJ_CHECK(hipMemcpy(d_ckind, ckind.data(), ckind.size(), hipMemcpyHostToDevice));
     ADJ_CHECK(hipMemcpy(d_src, src.data(), src.size() * sizeof(int), hipMemcpyHostToDevice));
@@ -772,6 +788,18 @@ struct octlb_adj {
 
   void Run(int steps, int w0, int w1, int K, const double* dir, const double* bbox, double rho0, const double* u0,
            uint32_t mask, double* J, double* du0) {
+    if (prec == 32) {
+      RunT<float>(steps, w0, w1, K, dir, bbox, rho0, u0, mask, J, du0);
+    } else {
+      RunT<double>(steps, w0, w1, K, dir, bbox, rho0, u0, mask, J, du0);
+    }
+  }
+
+  template <typename S>
+  void RunT(int steps, int w0, int w1, int K, const double* dir, const double* bbox, double rho0, const double* u0,
+            uint32_t mask, double* J, double* du0) {
+    S* ys[2] = {static_cast<S*>(d_y[0]), static_cast<S*>(d_y[1])};
+    S* xfinal = static_cast<S*>(d_final);
     std::vector<DragTerm> terms;
     std::vector<int> seed_cell;
     std::vector<double> seed_w;
@@ -794,8 +822,8 @@ struct octlb_adj {
       double* d_seed_w = static_cast<double*>(alloc(seed_w.size() * sizeof(double)));
       const int ndg = std::max(1, static_cast<int>(Blocks(nterms)));  // drag partials per step
       double* d_jv = static_cast<double*>(alloc(static_cast<size_t>(steps + 1) * ndg * sizeof(double)));
-      double* ck = static_cast<double*>(alloc(static_cast<size_t>(nck) * st * sizeof(double)));
-      double* seg = static_cast<double*>(alloc(static_cast<size_t>(K) * st * sizeof(double)));
+      S* ck = static_cast<S*>(alloc(static_cast<size_t>(nck) * st * sizeof(S)));
+      S* seg = static_cast<S*>(alloc(static_cast<size_t>(K) * st * sizeof(S)));
       const unsigned ncb = Blocks(ncells);
       double* d_part = static_cast<double*>(alloc(static_cast<size_t>(ncb) * 3 * sizeof(double)));
       if (nterms) {
@@ -813,21 +841,21 @@ struct octlb_adj {
       // reverse sweep starts from it: the checkpoints x_{cK} and every state
       // of the last segment (which is then not recomputed)
       const int s_last = (nck - 1) * K;
-      auto xdest = [&](int s) -> double* {
-        if (s == steps) return d_final;
+      auto xdest = [&](int s) -> S* {
+        if (s == steps) return xfinal;
         if (s >= s_last) return seg + static_cast<size_t>(s - s_last) * st;
         return s % K == 0 ? ck + static_cast<size_t>(s / K) * st : nullptr;
       };
-      hipLaunchKernelGGL(KAdjInit, dim3(ncb), dim3(kThreads), 0, stream, xdest(0), d_cells, d_ckind, ncells, rho0,
+      hipLaunchKernelGGL(KAdjInit<S>, dim3(ncb), dim3(kThreads), 0, stream, xdest(0), d_cells, d_ckind, ncells, rho0,
                          u0[0], u0[1], u0[2], mask);
-      hipLaunchKernelGGL(KAdjCollide, dim3(ncb), dim3(kThreads), 0, stream, xdest(0), d_y[0], d_cells, d_ckind,
+      hipLaunchKernelGGL(KAdjCollide<S>, dim3(ncb), dim3(kThreads), 0, stream, xdest(0), ys[0], d_cells, d_ckind,
                          ncells, omega);
       int cur = 0;
       for (int s = 1; s <= steps; ++s) {
-        hipLaunchKernelGGL(KAdjStep, dim3(ncb), dim3(kThreads), 0, stream, d_y[cur], xdest(s), d_y[cur ^ 1], d_src,
+        hipLaunchKernelGGL(KAdjStep<S>, dim3(ncb), dim3(kThreads), 0, stream, ys[cur], xdest(s), ys[cur ^ 1], d_src,
                            d_links, d_cells, d_ckind, d_cblock, d_cflag, d_nbase, ncells, omega);
         if (s > w0 && s <= w1 && nterms)
-          hipLaunchKernelGGL(KAdjDrag, dim3(ndg), dim3(kThreads), 0, stream, d_y[cur], d_src, d_links, d_terms, nterms,
+          hipLaunchKernelGGL(KAdjDrag<S>, dim3(ndg), dim3(kThreads), 0, stream, ys[cur], d_src, d_links, d_terms, nterms,
                              d_jv + static_cast<size_t>(s) * ndg);
         cur ^= 1;
       }
@@ -841,25 +869,25 @@ struct octlb_adj {
         const int s0 = c * K, s1 = std::min(steps, s0 + K);
         // segment states: x_{s0} in its checkpoint (or seg for the last
         // segment), x_{s0+j} in seg[j], recomputed unless the forward kept them
-        double* x0 = c == nck - 1 ? seg : ck + static_cast<size_t>(c) * st;
+        S* x0 = c == nck - 1 ? seg : ck + static_cast<size_t>(c) * st;
         auto xs = [&](int j) { return j == 0 ? x0 : seg + static_cast<size_t>(j) * st; };
         if (c != nck - 1 && s1 - s0 > 1) {
-          hipLaunchKernelGGL(KAdjCollide, dim3(ncb), dim3(kThreads), 0, stream, x0, d_y[0], d_cells, d_ckind, ncells,
+          hipLaunchKernelGGL(KAdjCollide<S>, dim3(ncb), dim3(kThreads), 0, stream, x0, ys[0], d_cells, d_ckind, ncells,
                              omega);
           int yc = 0;
           for (int j = 1; j < s1 - s0; ++j) {
-            hipLaunchKernelGGL(KAdjStep, dim3(ncb), dim3(kThreads), 0, stream, d_y[yc], xs(j), d_y[yc ^ 1], d_src,
+            hipLaunchKernelGGL(KAdjStep<S>, dim3(ncb), dim3(kThreads), 0, stream, ys[yc], xs(j), ys[yc ^ 1], d_src,
                                d_links, d_cells, d_ckind, d_cblock, d_cflag, d_nbase, ncells, omega);
             yc ^= 1;
           }
         }
         for (int s = s1; s > s0; --s) {
-          const double* xp = xs(s - 1 - s0);
+          const S* xp = xs(s - 1 - s0);
           auto win = [&](int st) { return st > w0 && st <= w1 && nterms > 0; };
           if (nlinks)
-            hipLaunchKernelGGL(KAdjLinkGrad, dim3(Blocks(nlinks)), dim3(kThreads), 0, stream, d_lx, xp, d_cells,
+            hipLaunchKernelGGL(KAdjLinkGrad<S>, dim3(Blocks(nlinks)), dim3(kThreads), 0, stream, d_lx, xp, d_cells,
                                d_ckind, d_links, nlinks, omega, d_dq);
-          hipLaunchKernelGGL(KAdjReverse, dim3(ncb), dim3(kThreads), 0, stream, d_lx, d_ln, xp, d_rd, d_ovo, d_ove,
+          hipLaunchKernelGGL(KAdjReverse<S>, dim3(ncb), dim3(kThreads), 0, stream, d_lx, d_ln, xp, d_rd, d_ovo, d_ove,
                              d_links, d_cells, d_ckind, d_cblock, d_cflag, d_nbase, d_cseed, d_seed_w,
                              win(s) ? scale : 0.0, win(s - 1) ? scale : 0.0, ncells, omega);
           std::swap(d_lx, d_ln);
@@ -974,7 +1002,13 @@ int octlb_adj_final_core(octlb_adj* adj, double* out) {
   return AdjGuard([&] {
     if (!adj || !out) throw std::invalid_argument("null argument");
     std::vector<double> s(static_cast<size_t>(adj->ntot));
-    ADJ_CHECK(hipMemcpy(s.data(), adj->d_final, s.size() * sizeof(double), hipMemcpyDeviceToHost));
+    if (adj->prec == 32) {
+      std::vector<float> f(s.size());
+      ADJ_CHECK(hipMemcpy(f.data(), adj->d_final, f.size() * sizeof(float), hipMemcpyDeviceToHost));
+      for (size_t k = 0; k < f.size(); ++k) s[k] = static_cast<double>(f[k]);
+    } else {
+      ADJ_CHECK(hipMemcpy(s.data(), adj->d_final, s.size() * sizeof(double), hipMemcpyDeviceToHost));
+    }
     for (int b = 0; b < adj->nb; ++b)
       for (int c = 0; c < kNC; ++c) {
         const Cell cl = adj->cells[adj->core_cell[static_cast<size_t>(b) * kNC + c]];

@@ -1411,6 +1411,7 @@ void ExportAdjTopo(const octlb_dev* d, AdjTopo* t) {
     if (IsFd(k)) throw std::invalid_argument("adjoint: FD boundary kinds are not differentiated");
   constexpr int NP = kTile + 2, NS = NP * NP * NP;
   t->nb = d->nb;
+  t->prec = d->prec;
   t->omega = d->sc.omega;
   t->kinds = d->kinds;
   t->link_key.clear();

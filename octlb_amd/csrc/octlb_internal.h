@@ -18,6 +18,7 @@ int SetError(int code, const std::string& msg);
 // plain BGK and Bouzidi streaming.
 struct AdjTopo {
   int nb = 0;
+  int prec = 64;                  // forward precision of the handle
   double omega = 0.0;
   std::vector<uint8_t> kinds;     // [nb][512] BcKind bytes
   std::vector<int64_t> link_key;  // (block*512 + cell)*19 + pop of every set link

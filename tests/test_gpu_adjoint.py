@@ -4,7 +4,9 @@
 run in another order); the adjoint's own forward sweep against
 octlb_dev_advance bit for bit; gradients independent of the checkpoint
 interval; the torch.autograd wrapper; central differences of the device
-objective; and the scope check of octlb_adj_create.
+objective; the scope check of octlb_adj_create; and the FP32-forward /
+FP64-adjoint mode: its forward sweep equals the FP32 production step bit for
+bit and its gradients stay within 1e-4 of the FP64 ones.
 """
 import ctypes as C
 
@@ -111,3 +113,21 @@ def test_create_rejects_what_it_does_not_differentiate():
     msg = abi.lib().octlb_last_error().decode()
     sol.close()
     assert rc == -1 and "FD" in msg  # OCTLB_EINVAL
+
+
+def test_fp32_forward_fp64_adjoint(box, device_grad):
+    J, du, dq, core = DragAdjoint(box, STEPS, WIN, checkpoint=7, precision=32).evaluate(final_core=True)
+    sol = GpuSolver(box.topo, box.params(32))
+    sol.set_bc_kinds(box.kinds)
+    sol.set_links(box.link, box.q)
+    sol.init_equilibrium(1.0, box.init_u0, box.init_mask)
+    sol.advance(STEPS)
+    ref = sol.download_core()
+    sol.close()
+    np.testing.assert_array_equal(core, ref.reshape(core.shape))
+    J64, du64, dq64, _ = device_grad
+    rel_u = abs(du[0] - du64[0]) / abs(du64[0])
+    rel_q = np.abs(dq - dq64).max() / np.abs(dq64).max()
+    print(f"FP32 forward vs FP64: J {abs(J - J64) / abs(J64):.2e}, dJ/du0 {rel_u:.2e}, dJ/dq {rel_q:.2e}", flush=True)
+    assert abs(J - J64) <= 1e-4 * abs(J64)
+    assert rel_u <= 1e-4 and rel_q <= 1e-4
