@@ -79,7 +79,7 @@ __device__ __forceinline__ int XcdWg(int w, int n) {
 }
 
 constexpr int kN = 8, kNP = 10, kNS = kNP * kNP * kNP, kNC = kN * kN * kN;
-constexpr int kThreads = 256;
+constexpr int kThreads = 128;
 constexpr uint8_t kHalo = 255;  // kind of a halo slot: carried, never collided
 
 struct AdjLink {
