@@ -42,6 +42,22 @@ def test_device_create_reports_errors_without_crashing():
     assert abi.lib().octlb_last_error()
 
 
+def test_adjoint_and_output_entry_points_reject_null_arguments():
+    # argument checks run before any device work
+    import ctypes as C
+    lib = abi.lib()
+    h = C.c_void_p()
+    assert lib.octlb_adj_create(None, C.byref(h)) == -1 and not h.value
+    z = np.zeros(6)
+    rc = lib.octlb_adj_run(None, 10, 0, 10, 5, abi.f64p(z), abi.f64p(z), 1.0, abi.f64p(z), 1, 0, None,
+                           abi.f64p(z), abi.f64p(z), None)
+    assert rc == -1 and b"null" in lib.octlb_last_error()
+    assert lib.octlb_adj_destroy(None) == 0
+    assert lib.octlb_dev_macro_fields(None, None, None) != 0
+    assert lib.octlb_dev_write_vtk(None, 0, b".", b"x") != 0
+    assert lib.octlb_vtk_append_pvd(None, 0, b"x.vtm") == -1
+
+
 def _leaves(topo):
     return topo.bounds
 

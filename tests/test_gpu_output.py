@@ -91,3 +91,21 @@ def test_fp32_macro_fields(stepped):
     np.testing.assert_allclose(rho, r.reshape(sim.nb, n, n, n).transpose(0, 3, 2, 1), rtol=1e-6)
     np.testing.assert_allclose(u, v.reshape(sim.nb, n, n, n, 3).transpose(0, 3, 2, 1, 4), rtol=0, atol=1e-6)
     sim.close()
+
+
+def test_amr_writer_cavity_style(tmp_path):
+    from octlb_amd.output import write_amr_timestep
+
+    case = cases.cavity_blocked(16, tau=0.6, u_lid=0.05, const_rho=0)
+    sim = GpuSolver(case.topo, case.params(64))
+    setup(sim, case)
+    sim.advance(5)
+    _, u = sim.macro_fields()
+    paths = write_amr_timestep(sim, 5, tmp_path, "cav", phys_scale=2.5)
+    assert [p.name for p in paths[:2]] == ["cav_r0_oct0_T00005.vts", "cav_r0_oct1_T00005.vts"]
+    xml = paths[1].read_text()
+    np.testing.assert_array_equal(_decode(xml, "velocity"), u[1].reshape(-1))
+    np.testing.assert_array_equal(_decode(xml, "physVelocity"), u[1].reshape(-1) * 2.5)
+    assert (tmp_path / "cav_T00005.vtm").read_text().count("<Block index=") == sim.nb
+    assert 'file="cav_T00005.vtm"' in (tmp_path / "cav.pvd").read_text()
+    sim.close()
