@@ -106,13 +106,17 @@ def unit_converter(resolution, tau, length, velocity, nu, rho=1.0):
 
 
 # ------------------------------------------------------------- cavity (C1)
-def cavity(n_lat: int = 64, tau: float = 0.6, u_lid: float = 0.05, halo: int = 3) -> Case:
+def cavity(n_lat: int = 64, tau: float = 0.6, u_lid: float = 0.05, halo: int = 3, nu: float = None) -> Case:
     """Lid-driven cavity in the reference's single-block form
     (cavity3d_case.h:139-194): one n_lat^3 block with OpenLB padding 3,
     ConstRhoBGK bulk, interpolated-velocity (Skordos FD) on all six faces,
-    lid = u on the y-max face, padded-block rotate streaming."""
+    lid = u on the y-max face, padded-block rotate streaming.  ``nu`` given:
+    UnitConverter(n_lat - 1, tau, 1, 1, nu) exactly (the lattice lid speed
+    follows from it, e.g. OpenLbCavity3dDefaults = (30, 0.509, 1, 1, 0.001),
+    unit_converter.cpp:28-31); otherwise nu is chosen so the lid is u_lid."""
     res = n_lat - 1
-    nu = (tau - 0.5) / 3.0 * (1.0 / res) / u_lid  # gives u_lattice = u_lid
+    if nu is None:
+        nu = (tau - 0.5) / 3.0 * (1.0 / res) / u_lid  # gives u_lattice = u_lid
     omega, u_lat = unit_converter(res, tau, 1.0, 1.0, nu)
     specs = [abi.make_spec(abi.XMIN, abi.BC_INTERP_VELOCITY),
              abi.make_spec(abi.XMAX, abi.BC_INTERP_VELOCITY),
@@ -125,7 +129,7 @@ def cavity(n_lat: int = 64, tau: float = 0.6, u_lid: float = 0.05, halo: int = 3
     stamp_tree_boundary(kinds, topo.tree_boundary, specs)
     return Case(name=f"cavity{n_lat}", topo=topo, omega=omega, handler=abi.HANDLER_CONCRETE,
                 const_rho=1, bouzidi=0, snapshot=0, specs=specs, kinds=kinds,
-                init_mask=0xFF, u_ref=u_lat, info={"u_lid": u_lat, "tau": tau})
+                init_mask=0xFF, u_ref=u_lat, info={"u_lid": u_lat, "tau": tau, "resolution": res})
 
 
 # ------------------------------------------------------------- cylinder (C2)

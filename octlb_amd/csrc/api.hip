@@ -335,7 +335,9 @@ void octlb_dev::Build(const octlb_topology* t, const octlb_params* p) {
   n = t->n;
   h = t->halo;
   if (nb < 1) throw ArgError("topology has no blocks");
-  if (n < 4 || (n % 2) != 0) throw ArgError("block edge n must be even and >= 4");
+  if (n < 4) throw ArgError("block edge n must be >= 4");
+  if ((n % 2) != 0 && t->num_coarse_fine > 0)  // LevelCoupler (level_coupler.cpp:72-74)
+    throw ArgError("coarse-fine coupling needs an even block edge n");
   if (h < 1) throw ArgError("halo width must be >= 1");
   if (p->precision != 64 && p->precision != 32) throw ArgError("precision must be 64 or 32");
   if (!(p->omega > 0.0)) throw ArgError("omega must be positive");

@@ -109,3 +109,39 @@ def test_amr_writer_cavity_style(tmp_path):
     assert (tmp_path / "cav_T00005.vtm").read_text().count("<Block index=") == sim.nb
     assert 'file="cav_T00005.vtm"' in (tmp_path / "cav.pvd").read_text()
     sim.close()
+
+
+# OpenLB cavity3d at its defaults, converged (the profile the reference's
+# test_cavity3d_serial.cpp:56-81 compares with, cavity3d_case.h:114-120)
+OPENLB_CAVITY3D_U_OVER_LID = (1.0, 0.629597, 0.506129, 0.429369, 0.371218, 0.110531, 0.0733846, 0.0481395,
+                              0.00778279, -0.0115753, -0.152547, -0.213294, -0.167816, -0.124812, -0.112694,
+                              -0.0995179, 0.0)
+
+
+def test_cavity_centerline_matches_openlb(tmp_path):
+    """Cavity3dSerial.OpenLbCenterline_RelativeL2_Below2Pct: the
+    OpenLbCavity3dDefaults cavity (resolution 30, tau 0.509, nu 0.001) after
+    5,269 steps, centreline within 2 % (relative L2) of OpenLB's."""
+    from octlb_amd.output import cavity_centerline, cavity_ux_field, write_centerline_csv
+
+    from oracle.binding import OracleSim
+
+    case = cases.cavity(n_lat=31, tau=0.509, nu=0.001)
+    sim = GpuSolver(case.topo, case.params(64))
+    setup(sim, case)
+    # an odd block edge: the first steps bit for bit against the oracle
+    ora = setup(OracleSim(case.topo, case.params(64)), case)
+    sim.advance(3)
+    ora.advance(3)
+    np.testing.assert_array_equal(sim.download_core(), ora.download_core())
+    ora.close()
+    print("cavity 31^3, 5269 steps", flush=True)
+    sim.advance_graph(5266)
+    prof = cavity_centerline(cavity_ux_field(sim, case), case.info["resolution"], case.u_ref)
+    sim.close()
+    ref = np.array(OPENLB_CAVITY3D_U_OVER_LID)
+    l2 = float(np.sqrt(np.sum((np.array(prof) - ref) ** 2) / np.sum(ref ** 2)))
+    print(f"centreline relative L2 vs OpenLB: {l2:.4f}", flush=True)
+    assert l2 < 0.02
+    write_centerline_csv(tmp_path / "centerline.csv", prof)
+    assert (tmp_path / "centerline.csv").read_text().count("\n") == 18

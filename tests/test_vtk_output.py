@@ -140,3 +140,21 @@ def test_cd_csv_format(tmp_path):
     assert log.path.read_text() == "iT,Cd\n100,6.36123\n200,1e-07\n300,1.23457e+08\n400,-0.5\n"
     # Cylinder3dCase::drag_coefficient: 2 (-F . dir) / (rho0 u^2 A)
     assert drag_coefficient((-0.5, 0.1, 0.0), 1.0, 0.05, 64.0) == 2.0 * 0.5 / (0.05 * 0.05 * 64.0)
+
+
+def test_centerline_sampler_and_csv(tmp_path):
+    from octlb_amd.output import GHIA_RE100_Y_OVER_H, cavity_centerline, write_centerline_csv
+
+    # trilinear sampling reproduces a linear field exactly (away from clamping)
+    n, res = 31, 30
+    i = np.arange(n, dtype=float)
+    ux = (0.5 + 0.25 * i[:, None, None] + 2.0 * i[None, :, None] - 0.75 * i[None, None, :]) * np.ones((n, n, n))
+    prof = cavity_centerline(ux, res, u_lid=2.0)
+    dx = 1.0 / res
+    for y, p in zip(GHIA_RE100_Y_OVER_H, prof):
+        want = 0.5 + 0.25 * (0.5 / dx) + 2.0 * min(y / dx, n - 1) - 0.75 * (0.5 / dx)
+        assert p == pytest.approx(want / 2.0, rel=1e-12, abs=1e-12)
+    write_centerline_csv(tmp_path / "c.csv", prof)
+    lines = (tmp_path / "c.csv").read_text().splitlines()
+    assert lines[0] == "y_over_H,u_over_lid,ghia" and len(lines) == 18
+    assert lines[1].startswith("1,") and lines[1].endswith(",1") and lines[-1].startswith("0,")
