@@ -273,6 +273,18 @@ int octlb_build_links(int32_t num_blocks, int32_t n, const double* bounds, const
                       int64_t num_triangles, const double* triangles, int64_t capacity,
                       int64_t* num_links, int32_t* links, double* q);
 
+/* Re-adaptation (the survey's dynamic-AMR contract; the reference has
+ * no migration code): after the forest changes and a new handle dst is
+ * built for it, move src's state onto dst.  Same block edge, halo,
+ * precision and relaxation rate.  Unchanged blocks (same level and
+ * bounds) are copied whole; a block refined from a src block takes
+ * feq + s_p fneq of its parent cells (s_p = (tau - 1/4)/tau, the
+ * coupler's prolongation rule, level_coupler.cpp:163-214); a block
+ * coarsened from eight src blocks takes the mean rho, the
+ * momentum-conserving mean u and s_r times the mean fneq of its fine cells
+ * (s_r = tau/(tau - 1/4)).  Host round trip: once per adapt. */
+int octlb_dev_migrate(octlb_dev* dst, octlb_dev* src);
+
 /* ------------------------------------------------------------------ */
 /* Output from device buffers in the reference's VTK formats           */
 /* ------------------------------------------------------------------ */

@@ -1494,6 +1494,138 @@ int octlb_dev_member_average_rho(octlb_dev* d, int32_t member, double* out) {
   });
 }
 
+// Moves the state of src onto dst, a handle built for the forest after an
+// adapt (same block edge, halo, precision and relaxation rate).  Per dst
+// block: a src block with the same level and bounds is copied whole (its
+// halo-inclusive view); a block refined from a src block takes, in each
+// fine cell, the parent cell's olb (rho, u) and non-equilibrium part with the
+// coupler's prolongation rule f = feq + s_p fneq, s_p = (tau - 1/4)/tau
+// (level_coupler.cpp:163-214); a block coarsened from eight src blocks takes,
+// in each coarse cell, the mean rho, the momentum-conserving mean u and the
+// mean fneq of its 2x2x2 fine cells times s_r = tau/(tau - 1/4).  The
+// reference leaves re-adaptation to a future rebuild step (it has no
+// migration code), so the two interpolation rules are this library's.
+int octlb_dev_migrate(octlb_dev* dst, octlb_dev* src) {
+  return Guard([&] {
+    CheckDev(dst);
+    CheckDev(src);
+    if (dst->n != src->n || dst->h != src->h || dst->prec != src->prec || dst->sc.omega != src->sc.omega)
+      throw ArgError("migrate: handles differ in block edge, halo, precision or relaxation rate");
+    src->Sync();
+    dst->Sync();
+    const int n = dst->n, NP = dst->NP, h = dst->h;
+    const size_t view = static_cast<size_t>(dst->NS) * kQ;
+    auto slot = [&](int x, int y, int z) { return static_cast<size_t>(((x + h) * NP + y + h) * NP + z + h); };
+    auto same = [](const double* a, const double* b) {
+      for (int k = 0; k < 6; ++k)
+        if (a[k] != b[k]) return false;
+      return true;
+    };
+    auto inside = [](const double* in, const double* out) {  // in within out (closed)
+      return in[0] >= out[0] && in[1] >= out[1] && in[2] >= out[2] && in[3] <= out[3] && in[4] <= out[4] &&
+             in[5] <= out[5];
+    };
+    auto get = [&](octlb_dev* d, int b, std::vector<double>& v) {
+      v.assign(view, 0.0);
+      if (d->prec == 64) ToHostBlock<double>(d, b, v.data()); else ToHostBlock<float>(d, b, v.data());
+    };
+    auto put = [&](int b, const std::vector<double>& v) {
+      if (dst->prec == 64) FromHostBlock<double>(dst, b, v.data()); else FromHostBlock<float>(dst, b, v.data());
+    };
+    // olb::lbm::computeRhoU + fneq of one cell
+    auto macro = [](const double* f, double& rho, double* u, double* fneq) {
+      rho = 0.0;
+      for (int i = 0; i < kQ; ++i) rho += f[i];
+      rho += 1.0;
+      u[0] = u[1] = u[2] = 0.0;
+      for (int i = 0; i < kQ; ++i)
+        for (int d = 0; d < 3; ++d) u[d] += f[i] * Ci(i, d);
+      for (int d = 0; d < 3; ++d) u[d] /= rho;
+      const double usqr = NormSqr(u);
+      for (int i = 0; i < kQ; ++i) fneq[i] = f[i] - Feq<double>(i, rho, u, usqr);
+    };
+    const double tau = 1.0 / dst->sc.omega;
+    const double sp = (tau - 0.25) / tau, sr = tau / (tau - 0.25);
+    const double* sb = src->bounds.data();
+    const double* db = dst->bounds.data();
+    std::vector<double> in, out;
+    for (int b = 0; b < dst->nb; ++b) {
+      const double* bb = db + 6 * static_cast<size_t>(b);
+      int match = -1, parent = -1;
+      std::vector<int> kids;
+      for (int s = 0; s < src->nb; ++s) {
+        const double* sbb = sb + 6 * static_cast<size_t>(s);
+        if (src->level[s] == dst->level[b] && same(sbb, bb)) match = s;
+        else if (src->level[s] == dst->level[b] - 1 && inside(bb, sbb)) parent = s;
+        else if (src->level[s] == dst->level[b] + 1 && inside(sbb, bb)) kids.push_back(s);
+      }
+      if (match >= 0) {
+        get(src, match, in);
+        put(b, in);
+        continue;
+      }
+      out.assign(view, 0.0);
+      if (parent >= 0) {
+        get(src, parent, in);
+        const double* pb = sb + 6 * static_cast<size_t>(parent);
+        const double pcw = (pb[3] - pb[0]) / n, fcw = (bb[3] - bb[0]) / n;
+        for (int x = 0; x < n; ++x)
+          for (int y = 0; y < n; ++y)
+            for (int z = 0; z < n; ++z) {
+              const int cx = static_cast<int>(std::floor((bb[0] + (x + 0.5) * fcw - pb[0]) / pcw));
+              const int cy = static_cast<int>(std::floor((bb[1] + (y + 0.5) * fcw - pb[1]) / pcw));
+              const int cz = static_cast<int>(std::floor((bb[2] + (z + 0.5) * fcw - pb[2]) / pcw));
+              if (cx < 0 || cx >= n || cy < 0 || cy >= n || cz < 0 || cz >= n)
+                throw StateError("migrate: fine cell outside its parent");
+              double rho, u[3], fneq[kQ];
+              macro(&in[slot(cx, cy, cz) * kQ], rho, u, fneq);
+              const double usqr = NormSqr(u);
+              double* f = &out[slot(x, y, z) * kQ];
+              for (int i = 0; i < kQ; ++i) f[i] = Feq<double>(i, rho, u, usqr) + sp * fneq[i];
+            }
+      } else if (kids.size() == 8) {
+        const double cw = (bb[3] - bb[0]) / n;
+        std::vector<double> acc(static_cast<size_t>(n) * n * n * (kQ + 4), 0.0);  // rho, rho u, fneq sums
+        for (int s : kids) {
+          get(src, s, in);
+          const double* kb = sb + 6 * static_cast<size_t>(s);
+          const double kcw = (kb[3] - kb[0]) / n;
+          for (int x = 0; x < n; ++x)
+            for (int y = 0; y < n; ++y)
+              for (int z = 0; z < n; ++z) {
+                const int cx = static_cast<int>(std::floor((kb[0] + (x + 0.5) * kcw - bb[0]) / cw));
+                const int cy = static_cast<int>(std::floor((kb[1] + (y + 0.5) * kcw - bb[1]) / cw));
+                const int cz = static_cast<int>(std::floor((kb[2] + (z + 0.5) * kcw - bb[2]) / cw));
+                if (cx < 0 || cx >= n || cy < 0 || cy >= n || cz < 0 || cz >= n)
+                  throw StateError("migrate: child cell outside its parent");
+                double rho, u[3], fneq[kQ];
+                macro(&in[slot(x, y, z) * kQ], rho, u, fneq);
+                double* a = &acc[((static_cast<size_t>(cx) * n + cy) * n + cz) * (kQ + 4)];
+                a[0] += rho;
+                for (int d = 0; d < 3; ++d) a[1 + d] += rho * u[d];
+                for (int i = 0; i < kQ; ++i) a[4 + i] += fneq[i];
+              }
+        }
+        for (int x = 0; x < n; ++x)
+          for (int y = 0; y < n; ++y)
+            for (int z = 0; z < n; ++z) {
+              const double* a = &acc[((static_cast<size_t>(x) * n + y) * n + z) * (kQ + 4)];
+              const double rho = a[0] / 8.0;
+              const double u[3] = {a[1] / a[0], a[2] / a[0], a[3] / a[0]};
+              const double usqr = NormSqr(u);
+              double* f = &out[slot(x, y, z) * kQ];
+              for (int i = 0; i < kQ; ++i) f[i] = Feq<double>(i, rho, u, usqr) + sr * (a[4 + i] / 8.0);
+            }
+      } else {
+        throw ArgError("migrate: block " + std::to_string(b) + " has no source block, parent or eight children");
+      }
+      put(b, out);
+    }
+    dst->coupler_ready = false;
+    dst->Sync();
+  });
+}
+
 int octlb_dev_destroy(octlb_dev* dev) {
   if (!dev) return OCTLB_OK;
   if (dev->stream) (void)hipStreamSynchronize(dev->stream);  // teardown: nothing to report to

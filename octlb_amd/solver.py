@@ -107,6 +107,12 @@ class GpuSolver:
         abi.check(self._lib.octlb_dev_download_core(self._h, abi.f64p(a)), "download_core")
         return a.reshape(self.nb, self.n, self.n, self.n, abi.Q)
 
+    def migrate_from(self, other):
+        """Take over another handle's state after a forest change
+        (octlb_dev_migrate): unchanged blocks copied, refined blocks
+        prolonged, coarsened blocks restricted."""
+        abi.check(self._lib.octlb_dev_migrate(self._h, other._h), "migrate")
+
     # ---- output (the reference's VTK samplers and writers)
     def macro_fields(self):
         """(rho [nb, n, n, n], u [nb, n, n, n, 3]) per core cell in VTK cell

@@ -1,0 +1,108 @@
+"""octlb_dev_migrate: the state of one forest moved onto the forest after a
+refinement and back.  Unchanged blocks are copied bit for bit; refined
+blocks carry their parent cells' olb (rho, u) and s_p times the
+non-equilibrium part; refine-then-coarsen returns the original populations
+to rounding (s_r s_p = 1, every child of a cell identical); mass in
+fine-equivalent units is conserved.  The reference has no migration code,
+so the interpolation rules are checked against their own statement.
+"""
+import numpy as np
+import pytest
+
+from octlb_amd import cases
+from octlb_amd.forest import Forest
+from octlb_amd.solver import GpuSolver
+
+pytestmark = pytest.mark.gpu
+
+C = cases.C
+TAU = 0.6
+
+
+def _topo(refine):
+    f = Forest(np.array([0.0, 0.0, 0.0, 2.0, 1.0, 1.0]), (2, 1, 1))
+    f.refine_uniform(1)
+    if refine:
+        f.refine_box([0.01, 0.01, 0.01, 0.49, 0.49, 0.49], 2)
+        f.balance()
+    return f.topology(8, 1)
+
+
+def _solver(topo):
+    from octlb_amd import abi
+    s = GpuSolver(topo, abi.make_params(1.0 / TAU, precision=64))
+    s.set_bc_kinds(np.zeros((topo.num_blocks, 8, 8, 8), np.uint8))
+    return s
+
+
+def _macro(f):
+    rho = f.sum(-1) + 1.0
+    u = np.einsum("...i,id->...d", f, C.astype(float)) / rho[..., None]
+    usqr = (u * u).sum(-1)
+    w = np.array([1 / 3] + [1 / 18] * 3 + [1 / 36] * 6 + [1 / 18] * 3 + [1 / 36] * 6)
+    cu = np.einsum("...d,id->...i", u, C.astype(float))
+    feq = rho[..., None] * w * (1 + 3 * cu + 4.5 * cu * cu - 1.5 * usqr[..., None]) - w
+    return rho, u, f - feq
+
+
+def test_refine_and_coarsen_round_trip():
+    ta, tb = _topo(False), _topo(True)
+    assert tb.num_blocks == ta.num_blocks + 7
+    src = _solver(ta)
+    n = 8
+    nc = ta.num_blocks * n ** 3
+    rng = np.random.default_rng(3)
+    src.init_field(1.0 + 0.01 * rng.random(nc), 0.02 * (rng.random(nc * 3) - 0.5))
+    src.advance(3)  # a non-equilibrium part
+    a = src.download_core()
+    dst = _solver(tb)
+    dst.migrate_from(src)
+    b = dst.download_core()
+    tau_p = (TAU - 0.25) / TAU
+    refined = 0
+    for j in range(tb.num_blocks):
+        bb = tb.bounds[j]
+        same = [i for i in range(ta.num_blocks) if ta.level[i] == tb.level[j] and np.all(ta.bounds[i] == bb)]
+        if same:
+            np.testing.assert_array_equal(b[j], a[same[0]])
+            continue
+        par = [i for i in range(ta.num_blocks) if ta.level[i] == tb.level[j] - 1 and np.all(ta.bounds[i][:3] <= bb[:3])
+               and np.all(ta.bounds[i][3:] >= bb[3:])][0]
+        refined += 1
+        pb = ta.bounds[par]
+        off = np.rint((bb[:3] - pb[:3]) / ((pb[3] - pb[0]) / n)).astype(int)
+        idx = (np.arange(n) // 2)
+        parent_cells = a[par][np.ix_(off[0] + idx, off[1] + idx, off[2] + idx)]
+        rp, up, fp = _macro(parent_cells)
+        rc, uc, fc = _macro(b[j])
+        np.testing.assert_allclose(rc, rp, rtol=1e-14)
+        np.testing.assert_allclose(uc, up, rtol=0, atol=1e-15)
+        np.testing.assert_allclose(fc, tau_p * fp, rtol=0, atol=1e-15)
+    assert refined == 8
+    back = _solver(ta)
+    back.migrate_from(dst)
+    c = back.download_core()
+    np.testing.assert_allclose(c, a, rtol=0, atol=1e-14)
+    # fine-equivalent mass: a level-2 cell is 1/8 of a level-1 cell
+    ma = (a.sum(-1) + 1.0).sum()
+    wb = np.where(tb.level == 2, 1.0 / 8.0, 1.0)
+    mb = ((b.sum(-1) + 1.0).sum(axis=(1, 2, 3)) * wb).sum()
+    assert mb == pytest.approx(ma, rel=1e-13)
+    for s in (src, dst, back):
+        s.close()
+
+
+def test_identity_and_mismatch():
+    from octlb_amd import abi
+    ta = _topo(False)
+    src = _solver(ta)
+    src.init_equilibrium(1.0, (0.03, 0.0, 0.01), 0xFF)
+    src.advance(2)
+    dst = _solver(ta)
+    dst.migrate_from(src)
+    np.testing.assert_array_equal(dst.download_core(), src.download_core())
+    other = GpuSolver(ta, abi.make_params(1.0 / 0.7, precision=64))
+    with pytest.raises(abi.OctlbError):
+        other.migrate_from(src)
+    for s in (src, dst, other):
+        s.close()
