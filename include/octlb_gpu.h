@@ -209,8 +209,35 @@ int octlb_dev_download_core(octlb_dev* dev, double* out);
 int octlb_dev_advance(octlb_dev* dev, int32_t n_coarse_steps);
 /* Same as octlb_dev_advance but replays a captured hipGraph of one coarse step. */
 int octlb_dev_advance_graph(octlb_dev* dev, int32_t n_coarse_steps);
+/* Captures and instantiates every hipGraph octlb_dev_advance_graph replays
+ * (one step and 16 steps, from either level-0 parity) without running them,
+ * so no capture or instantiation happens inside a timed or latency-bound
+ * loop.  No reference counterpart (the reference has no graphs). */
+int octlb_dev_prepare_graphs(octlb_dev* dev);
 int octlb_dev_synchronize(octlb_dev* dev);
 int octlb_dev_set_step(octlb_dev* dev, int32_t step);
+
+/* TimeLoop::set_snapshot_post_collide (time_loop.h:84): take the post-collide
+ * snapshot of the drag links' outgoing populations in every following step
+ * (octlb_params.snapshot sets it at create). */
+int octlb_dev_set_snapshot(octlb_dev* dev, int32_t on);
+
+/* Update scheme of the fused collide+stream kernel (no reference
+ * counterpart; the reference has one CPU loop).  0: every 8^3 block pulls
+ * through an LDS box of its cells and one-cell apron, re-colliding the apron
+ * cells; 1 (default): blocks whose cells all stream plainly and whose 18
+ * halo parts are all exchanged at their level push their post-collide
+ * values to the destination cells (no apron), the others pull.  Both give
+ * bit-identical states. */
+int octlb_dev_set_kernel_variant(octlb_dev* dev, int32_t variant);
+/* The variant in use and how many blocks it updates by push. */
+int octlb_dev_kernel_variant(octlb_dev* dev, int32_t* variant, int32_t* push_blocks);
+/* Autotuner: times the fused kernel of every level under each variant on
+ * the current state (left unchanged), weights level l by its 2^l launches
+ * per coarse step, and keeps the fastest.  step_ms[2] (optional) receives
+ * the weighted kernel time per coarse step of variants 0 and 1 (0 when a
+ * variant does not apply). */
+int octlb_dev_autotune(octlb_dev* dev, int32_t reps, double* step_ms);
 
 /* MomentumExchangeDrag::force_on_fluid_if summed over blocks with the
  * obstacle predicate "neighbour cell centre inside bbox"

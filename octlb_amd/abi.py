@@ -173,6 +173,11 @@ _SIGNATURES = {
     "octlb_dev_download_core": ([_VP, _F64P], C.c_int),
     "octlb_dev_advance": ([_VP, C.c_int32], C.c_int),
     "octlb_dev_advance_graph": ([_VP, C.c_int32], C.c_int),
+    "octlb_dev_prepare_graphs": ([_VP], C.c_int),
+    "octlb_dev_set_snapshot": ([_VP, C.c_int32], C.c_int),
+    "octlb_dev_set_kernel_variant": ([_VP, C.c_int32], C.c_int),
+    "octlb_dev_kernel_variant": ([_VP, C.POINTER(C.c_int32), C.POINTER(C.c_int32)], C.c_int),
+    "octlb_dev_autotune": ([_VP, C.c_int32, _F64P], C.c_int),
     "octlb_dev_synchronize": ([_VP], C.c_int),
     "octlb_dev_set_step": ([_VP, C.c_int32], C.c_int),
     "octlb_dev_drag": ([_VP, _F64P, _F64P], C.c_int),

@@ -52,6 +52,19 @@ class Case:
                                const_rho=self.const_rho, stats_scope=self.stats_scope,
                                bouzidi=self.bouzidi, snapshot=self.snapshot, specs=self.specs)
 
+    def apply(self, sim):
+        """Stamp this case onto a solver handle: cell kinds, Bouzidi links
+        and the initial populations (the case constructors' install order,
+        cylinder3d_case.h:392-456).  Returns ``sim``."""
+        sim.set_bc_kinds(self.kinds)
+        if self.link.shape[0] > 0:
+            sim.set_links(self.link, self.q)
+        if self.init_field is not None:
+            sim.init_field(*self.init_field)
+        else:
+            sim.init_equilibrium(self.init_rho0, self.init_u0, self.init_mask)
+        return sim
+
     @property
     def n(self):
         return self.topo.n

@@ -10,6 +10,7 @@
 #include <cstdio>
 #include <cstring>
 #include <map>
+#include <unordered_map>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -126,6 +127,15 @@ struct octlb_dev {
   // ---- tiles per level
   std::vector<Tile> tiles;
   std::vector<int> tile_begin, tile_count;
+  std::vector<int> level_blocks;  // blocks per level (a push tile holds two)
+  std::vector<int> block_order;   // visiting order of the blocks of a level
+  std::vector<int> block_rank_host;
+  // Update scheme of the blocked fused kernel: 0 every block pulls through
+  // its LDS box (re-colliding the apron); 1 plain interior blocks push
+  // (push tiles), the rest pull.  Chosen by octlb_dev_set_kernel_variant or
+  // octlb_dev_autotune.
+  int variant = 1;
+  int npush_blocks = 0;
   // ---- device
   hipStream_t stream = nullptr;
   void* pops[2] = {nullptr, nullptr};
@@ -137,6 +147,7 @@ struct octlb_dev {
   bool pfull_fused = false;
   BlockTab* d_bt = nullptr;
   Tile* d_tiles = nullptr;
+  int* d_block_rank = nullptr;  // block -> index among its level's blocks in visiting order
   int* d_level = nullptr;
   uint32_t* d_qmask = nullptr;
   int* d_qoff = nullptr;
@@ -399,6 +410,7 @@ void octlb_dev::Build(const octlb_topology* t, const octlb_params* p) {
     r.level = level[b];
     r.flags = 0;
     for (int d = 0; d < 6; ++d) r.nbr[d] = fn[b][d];
+    for (int k = 0; k < 18; ++k) r.push_to[k] = -1;
     const int pairs[3][2] = {{0, 1}, {0, 2}, {1, 2}};
     for (int pi = 0; pi < 3; ++pi)
       for (int s1 = 0; s1 < 2; ++s1)
@@ -426,6 +438,10 @@ void octlb_dev::Build(const octlb_topology* t, const octlb_params* p) {
   plan.clear();
   for (const octlb_cf_face& f : cf) {
     if (f.coarse_id < 0 || f.coarse_id >= nb) throw ArgError("coarse-fine face out of range");
+    if (f.normal < 0 || f.normal > 5) throw ArgError("coarse-fine face normal out of range");
+    if (f.coarse_level < 0 || f.coarse_level + 1 >= kMaxLevels || f.coarse_level != level[f.coarse_id] ||
+        f.fine_level != f.coarse_level + 1)
+      throw ArgError("coarse-fine face levels disagree with the block levels");
     const double* cb = f.coarse_bounds;
     const double cc[3] = {0.5 * (cb[0] + cb[3]), 0.5 * (cb[1] + cb[4]), 0.5 * (cb[2] + cb[5])};
     const int axis = f.normal / 2, is_max = f.normal & 1;
@@ -434,6 +450,7 @@ void octlb_dev::Build(const octlb_topology* t, const octlb_params* p) {
     const int half = n / 2;
     for (int sl = 0; sl < 4; ++sl) {
       if (f.fine_ids[sl] < 0 || f.fine_ids[sl] >= nb) throw ArgError("coarse-fine face out of range");
+      if (level[f.fine_ids[sl]] != f.fine_level) throw ArgError("coarse-fine face levels disagree with the block levels");
       const double* fb = f.fine_bounds[sl];
       const double fc[3] = {0.5 * (fb[0] + fb[3]), 0.5 * (fb[1] + fb[4]), 0.5 * (fb[2] + fb[5])};
       const int b0 = (fc[t0] < cc[t0] ? 0 : 1) * half, b1 = (fc[t1] < cc[t1] ? 0 : 1) * half;
@@ -498,6 +515,7 @@ void octlb_dev::Build(const octlb_topology* t, const octlb_params* p) {
   // tiles per level (octants_by_level_, time_loop.h:155-160)
   tile_begin.assign(kMaxLevels, 0);
   tile_count.assign(kMaxLevels, 0);
+  level_blocks.assign(kMaxLevels, 0);
   tiles.clear();
   const int tpa = padded ? (NP + kTile - 1) / kTile : 1;
   // Blocks of a level are visited in lexicographic order of their origin, z
@@ -506,7 +524,8 @@ void octlb_dev::Build(const octlb_topology* t, const octlb_params* p) {
   // apron re-read hits L2 (4 % faster than octant order on the cylinder).
   // With ConstRho the octant order is kept: the average is summed in tile
   // order, which must be the reference's.
-  std::vector<int> order(nb);
+  std::vector<int>& order = block_order;
+  order.resize(nb);
   for (int b = 0; b < nb; ++b) order[b] = b;
   if (!padded && !sc.const_rho) {
     std::stable_sort(order.begin(), order.end(), [&](int a, int c) {
@@ -527,6 +546,7 @@ void octlb_dev::Build(const octlb_topology* t, const octlb_params* p) {
           for (int z = 0; z < tpa; ++z) tiles.push_back(Tile{b, x * kTile, y * kTile, z * kTile});
     }
     tile_count[l] = static_cast<int>(tiles.size()) - tile_begin[l];
+    level_blocks[l] = tile_count[l] / (tpa * tpa * tpa);
   }
 
   kinds.assign(static_cast<size_t>(nb) * ncore, 0);
@@ -547,6 +567,13 @@ void octlb_dev::Build(const octlb_topology* t, const octlb_params* p) {
   Upload(d_bt, bt, stream);
   d_tiles = Alloc<Tile>(tiles.size());
   Upload(d_tiles, tiles, stream);
+  {
+    std::vector<int> rank(nb, 0), next(kMaxLevels, 0);
+    for (int bi = 0; bi < nb; ++bi) rank[block_order[bi]] = next[level[block_order[bi]]]++;
+    d_block_rank = Alloc<int>(nb);
+    Upload(d_block_rank, rank, stream);
+    block_rank_host = rank;
+  }
   d_level = Alloc<int>(nb);
   Upload(d_level, level, stream);
   d_stats = Alloc<double>(2 * tiles.size());
@@ -753,6 +780,72 @@ void octlb_dev::Prepare() {
       if (live) r.flags |= kFlagShellLive;
     }
   }
+  // ---- push-class blocks and the tile list.  A block is push-class when
+  // every cell streams by the plain pull, its 18 halo parts are all exchanged
+  // at its level (no shell, no coupling, no FD face), nothing reads its
+  // post-collide values between collide and stream (no drag snapshot) and no
+  // coupling point writes it.  Reference-order ConstRho statistics keep one
+  // block per tile, so they keep the pull scheme.
+  npush_blocks = 0;
+  if (!padded) {
+    const bool push_ok = variant == 1 && !(sc.const_rho && !fast_stats);
+    std::vector<char> is_push(nb, 0);
+    for (int b = 0; b < nb; ++b) {
+      BlockTab& r = bt[b];
+      for (int k = 0; k < 18; ++k) r.push_to[k] = -1;
+      if (!push_ok) continue;
+      bool ok = (r.flags & kFlagPlain) && !(r.flags & (kFlagShellLive | kFlagDrag | kFlagProlong | kFlagLinks));
+      for (int k = 0; k < 18 && ok; ++k) ok = r.nbr[k] >= 0;
+      is_push[b] = ok ? 1 : 0;
+    }
+    auto opposite = [](int part) {
+      if (part < 6) return part ^ 1;
+      const int e = part - 6;
+      return 6 + (e & ~3) + (3 - (e & 3));  // both sides flipped
+    };
+    for (int b = 0; b < nb; ++b) {
+      if (!is_push[b]) continue;
+      bt[b].flags |= kFlagPush;
+      ++npush_blocks;
+      for (int k = 0; k < 18; ++k) {
+        const int src = bt[b].nbr[k];
+        const int back = opposite(k);
+        if (bt[src].push_to[back] >= 0 && bt[src].push_to[back] != b)
+          throw StateError("push tables: two blocks pull the same halo part");
+        bt[src].push_to[back] = b;
+      }
+    }
+    for (int b = 0; b < nb; ++b) {
+      if (is_push[b]) continue;
+      for (int k = 0; k < 18; ++k)
+        if (bt[b].push_to[k] >= 0) bt[b].flags |= kFlagPushOut;
+    }
+    // tiles: per level in visiting order; consecutive push-class blocks of
+    // one ensemble member pair up in a push tile
+    tiles.clear();
+    for (int l = 0; l <= max_level; ++l) {
+      tile_begin[l] = static_cast<int>(tiles.size());
+      int pending = -1;
+      for (int bi = 0; bi < nb; ++bi) {
+        const int b = block_order[bi];
+        if (level[b] != l) continue;
+        if (!is_push[b]) {
+          tiles.push_back(Tile{b, 0, 0, 0});
+          continue;
+        }
+        if (pending >= 0 && bt[pending].member == bt[b].member) {
+          tiles.push_back(Tile{pending, b, 1, 0});
+          pending = -1;
+          continue;
+        }
+        if (pending >= 0) tiles.push_back(Tile{pending, -1, 1, 0});
+        pending = b;
+      }
+      if (pending >= 0) tiles.push_back(Tile{pending, -1, 1, 0});
+      tile_count[l] = static_cast<int>(tiles.size()) - tile_begin[l];
+    }
+    Upload(d_tiles, tiles, stream);
+  }
   // ---- post-stream FD cell lists (concrete handler, tree-boundary blocks)
   std::vector<int2> fd_cells[3];
   std::vector<int> fd_index(nb, -1);
@@ -874,13 +967,26 @@ void octlb_dev::Prepare() {
   {
     const int l = max_level;
     const int tpb = padded ? ((NP + kTile - 1) / kTile) * ((NP + kTile - 1) / kTile) * ((NP + kTile - 1) / kTile) : 1;
+    // partial-sum slots of the member's blocks (blocked mode: one per block,
+    // in visiting order; padded mode: one per tile)
     std::vector<int> tiles_of(nmembers + 1, 0);
     int prev = 0;
-    for (int i = 0; i < tile_count[l]; ++i) {
-      const int m = bt[tiles[tile_begin[l] + i].block].member;
-      if (m < prev) throw ArgError("ensemble members must be contiguous in block order");
-      prev = m;
-      tiles_of[m + 1] += 1;
+    if (padded) {
+      for (int i = 0; i < tile_count[l]; ++i) {
+        const int m = bt[tiles[tile_begin[l] + i].block].member;
+        if (m < prev) throw ArgError("ensemble members must be contiguous in block order");
+        prev = m;
+        tiles_of[m + 1] += 1;
+      }
+    } else {
+      for (int bi = 0; bi < nb; ++bi) {
+        const int b = block_order[bi];
+        if (level[b] != l) continue;
+        const int m = bt[b].member;
+        if (m < prev) throw ArgError("ensemble members must be contiguous in block order");
+        prev = m;
+        tiles_of[m + 1] += 1;
+      }
     }
     std::vector<int> cells(nmembers + 1, 0);
     for (int m = 0; m < nmembers; ++m) {
@@ -962,6 +1068,7 @@ TileArgs<T> octlb_dev::MakeTileArgs(int l) {
   a.doff = d_doff;
   a.snap = static_cast<T*>(d_snap);
   a.stats = d_stats;
+  a.block_rank = d_block_rank;
   a.stat_val = d_stat_val;
   a.stat_use = d_stat_use;
   {
@@ -1063,7 +1170,8 @@ void octlb_dev::Level(int l) {
       tile_launches += 1;
       const int tpa = (NP + kTile - 1) / kTile;
       const int tpb = padded ? tpa * tpa * tpa : 1;
-      tile_updates += static_cast<double>(cnt / tpb) * static_cast<double>(ncore);
+      (void)tpb;
+      tile_updates += static_cast<double>(level_blocks[l]) * static_cast<double>(ncore);
     }
     par[l] ^= 1;
     if (a.want_stats == 1) LaunchStatsFinalize<T>(d_stats, d_stat_tiles, nmembers, d_avg, stream);
@@ -1511,20 +1619,14 @@ int octlb_dev_migrate(octlb_dev* dst, octlb_dev* src) {
     CheckDev(src);
     if (dst->n != src->n || dst->h != src->h || dst->prec != src->prec || dst->sc.omega != src->sc.omega)
       throw ArgError("migrate: handles differ in block edge, halo, precision or relaxation rate");
+    if (dst->nmembers != 1 || src->nmembers != 1)
+      throw ArgError("migrate: ensemble handles are not migrated (members share block bounds)");
+    if (dst->nb == 0 || src->nb == 0) throw ArgError("migrate: empty forest");
     src->Sync();
     dst->Sync();
     const int n = dst->n, NP = dst->NP, h = dst->h;
     const size_t view = static_cast<size_t>(dst->NS) * kQ;
     auto slot = [&](int x, int y, int z) { return static_cast<size_t>(((x + h) * NP + y + h) * NP + z + h); };
-    auto same = [](const double* a, const double* b) {
-      for (int k = 0; k < 6; ++k)
-        if (a[k] != b[k]) return false;
-      return true;
-    };
-    auto inside = [](const double* in, const double* out) {  // in within out (closed)
-      return in[0] >= out[0] && in[1] >= out[1] && in[2] >= out[2] && in[3] <= out[3] && in[4] <= out[4] &&
-             in[5] <= out[5];
-    };
     auto get = [&](octlb_dev* d, int b, std::vector<double>& v) {
       v.assign(view, 0.0);
       if (d->prec == 64) ToHostBlock<double>(d, b, v.data()); else ToHostBlock<float>(d, b, v.data());
@@ -1544,39 +1646,78 @@ int octlb_dev_migrate(octlb_dev* dst, octlb_dev* src) {
       const double usqr = NormSqr(u);
       for (int i = 0; i < kQ; ++i) fneq[i] = f[i] - Feq<double>(i, rho, u, usqr);
     };
-    const double tau = 1.0 / dst->sc.omega;
-    const double sp = (tau - 0.25) / tau, sr = tau / (tau - 0.25);
+    // Blocks of both forests keyed by (level, integer origin in that level's
+    // block widths): a hash lookup per destination block, so migration is
+    // linear in the block count.
     const double* sb = src->bounds.data();
     const double* db = dst->bounds.data();
+    double gmin[3] = {db[0], db[1], db[2]};
+    for (int k = 0; k < 3; ++k) {
+      for (int b = 0; b < dst->nb; ++b) gmin[k] = std::min(gmin[k], db[6 * static_cast<size_t>(b) + k]);
+      for (int b = 0; b < src->nb; ++b) gmin[k] = std::min(gmin[k], sb[6 * static_cast<size_t>(b) + k]);
+    }
+    const double w0 = (db[3] - db[0]) * std::ldexp(1.0, dst->level[0]);  // level-0 block width
+    auto key_of = [&](const double* bb, int lv) {
+      const double w = std::ldexp(w0, -lv);
+      if (std::fabs((bb[3] - bb[0]) - w) > 1e-9 * w0) throw ArgError("migrate: block width does not follow its level");
+      int64_t ix[3];
+      for (int k = 0; k < 3; ++k) {
+        const double c = (bb[k] - gmin[k]) / w;
+        ix[k] = std::llround(c);
+        if (std::fabs(c - static_cast<double>(ix[k])) > 1e-6) throw ArgError("migrate: block origin off its level's grid");
+      }
+      return std::array<int64_t, 4>{lv, ix[0], ix[1], ix[2]};
+    };
+    struct KeyHash {
+      size_t operator()(const std::array<int64_t, 4>& k) const {
+        uint64_t hsh = 1469598103934665603ull;
+        for (int64_t v : k) hsh = (hsh ^ static_cast<uint64_t>(v)) * 1099511628211ull;
+        return static_cast<size_t>(hsh);
+      }
+    };
+    std::unordered_map<std::array<int64_t, 4>, int, KeyHash> src_at;
+    src_at.reserve(static_cast<size_t>(src->nb) * 2);
+    for (int s = 0; s < src->nb; ++s) src_at[key_of(sb + 6 * static_cast<size_t>(s), src->level[s])] = s;
+    auto find = [&](const std::array<int64_t, 4>& k) {
+      auto it = src_at.find(k);
+      return it == src_at.end() ? -1 : it->second;
+    };
+    const double tau = 1.0 / dst->sc.omega;
+    const double sp = (tau - 0.25) / tau, sr = tau / (tau - 0.25);
     std::vector<double> in, out;
     for (int b = 0; b < dst->nb; ++b) {
       const double* bb = db + 6 * static_cast<size_t>(b);
-      int match = -1, parent = -1;
-      std::vector<int> kids;
-      for (int s = 0; s < src->nb; ++s) {
-        const double* sbb = sb + 6 * static_cast<size_t>(s);
-        if (src->level[s] == dst->level[b] && same(sbb, bb)) match = s;
-        else if (src->level[s] == dst->level[b] - 1 && inside(bb, sbb)) parent = s;
-        else if (src->level[s] == dst->level[b] + 1 && inside(sbb, bb)) kids.push_back(s);
-      }
+      const int lv = dst->level[b];
+      const auto key = key_of(bb, lv);
+      const int match = find(key);
       if (match >= 0) {
         get(src, match, in);
         put(b, in);
         continue;
       }
+      const int parent = lv > 0 ? find({lv - 1, key[1] >> 1, key[2] >> 1, key[3] >> 1}) : -1;
+      std::vector<int> kids;
+      if (parent < 0)
+        for (int c = 0; c < 8; ++c) {
+          const int s = find({lv + 1, 2 * key[1] + (c & 1), 2 * key[2] + ((c >> 1) & 1), 2 * key[3] + (c >> 2)});
+          if (s >= 0) kids.push_back(s);
+        }
       out.assign(view, 0.0);
       if (parent >= 0) {
+        // every slot of the fine block, halo included, from the parent's
+        // halo-inclusive view (a fine halo slot lies in the parent's core or
+        // one-slot halo)
         get(src, parent, in);
         const double* pb = sb + 6 * static_cast<size_t>(parent);
         const double pcw = (pb[3] - pb[0]) / n, fcw = (bb[3] - bb[0]) / n;
-        for (int x = 0; x < n; ++x)
-          for (int y = 0; y < n; ++y)
-            for (int z = 0; z < n; ++z) {
+        for (int x = -h; x < n + h; ++x)
+          for (int y = -h; y < n + h; ++y)
+            for (int z = -h; z < n + h; ++z) {
               const int cx = static_cast<int>(std::floor((bb[0] + (x + 0.5) * fcw - pb[0]) / pcw));
               const int cy = static_cast<int>(std::floor((bb[1] + (y + 0.5) * fcw - pb[1]) / pcw));
               const int cz = static_cast<int>(std::floor((bb[2] + (z + 0.5) * fcw - pb[2]) / pcw));
-              if (cx < 0 || cx >= n || cy < 0 || cy >= n || cz < 0 || cz >= n)
-                throw StateError("migrate: fine cell outside its parent");
+              if (cx < -h || cx >= n + h || cy < -h || cy >= n + h || cz < -h || cz >= n + h)
+                throw StateError("migrate: fine cell outside its parent's view");
               double rho, u[3], fneq[kQ];
               macro(&in[slot(cx, cy, cz) * kQ], rho, u, fneq);
               const double usqr = NormSqr(u);
@@ -1616,12 +1757,37 @@ int octlb_dev_migrate(octlb_dev* dst, octlb_dev* src) {
               double* f = &out[slot(x, y, z) * kQ];
               for (int i = 0; i < kQ; ++i) f[i] = Feq<double>(i, rho, u, usqr) + sr * (a[4 + i] / 8.0);
             }
+        // halo slots repeat the nearest core cell (the block's own boundary
+        // layer): a defined value for the shell slots no exchange refills
+        for (int x = -h; x < n + h; ++x)
+          for (int y = -h; y < n + h; ++y)
+            for (int z = -h; z < n + h; ++z) {
+              if (x >= 0 && x < n && y >= 0 && y < n && z >= 0 && z < n) continue;
+              const int cx = std::min(std::max(x, 0), n - 1), cy = std::min(std::max(y, 0), n - 1),
+                        cz = std::min(std::max(z, 0), n - 1);
+              std::copy_n(&out[slot(cx, cy, cz) * kQ], kQ, &out[slot(x, y, z) * kQ]);
+            }
       } else {
         throw ArgError("migrate: block " + std::to_string(b) + " has no source block, parent or eight children");
       }
       put(b, out);
     }
-    dst->coupler_ready = false;
+    // the time state moves with the populations: the step counter (inlet
+    // ramps) and the lagged ConstRho average
+    dst->step = src->step;
+    HIP_CHECK(hipMemcpyAsync(dst->d_step + dst->step_slot, &dst->step, sizeof(int), hipMemcpyHostToDevice,
+                             dst->stream));
+    HIP_CHECK(hipMemcpyAsync(dst->d_avg, src->d_avg, sizeof(double), hipMemcpyDeviceToDevice, dst->stream));
+    // the coupler's previous-step macros (level_coupler.cpp:438-478) carry
+    // over when the coupling plan is unchanged; otherwise the new plan's
+    // points start from the current coarse state, as at construction
+    if (dst->npts > 0 && dst->plan == src->plan && src->coupler_ready) {
+      HIP_CHECK(hipMemcpyAsync(dst->d_prev, src->d_prev, static_cast<size_t>(dst->npts) * kMacro * dst->elem,
+                               hipMemcpyDeviceToDevice, dst->stream));
+      dst->coupler_ready = true;
+    } else {
+      dst->coupler_ready = false;
+    }
     dst->Sync();
   });
 }
@@ -1767,6 +1933,48 @@ int octlb_dev_advance(octlb_dev* d, int32_t steps) {
   });
 }
 
+namespace {
+
+// Captures and instantiates the graph of `count` coarse steps that starts
+// from level-0 parity slot & 1 (slots 0, 1: one step; 2, 3: kGraphSteps).
+// Capture only records launches: the device state and the host's parity and
+// step-slot bookkeeping are as before.
+void CaptureGraph(octlb_dev* d, int slot, int count) {
+  int saved[kMaxLevels];
+  std::memcpy(saved, d->par, sizeof(saved));
+  const int saved_slot = d->step_slot;
+  d->par[0] = slot & 1;
+  d->step_slot = d->par[0];
+  hipGraph_t g = nullptr;
+  d->capturing = true;
+  HIP_CHECK(hipStreamBeginCapture(d->stream, hipStreamCaptureModeThreadLocal));
+  for (int k = 0; k < count; ++k) {
+    if (d->prec == 64) {
+      d->CoarseStep<double>();
+    } else {
+      d->CoarseStep<float>();
+    }
+  }
+  HIP_CHECK(hipStreamEndCapture(d->stream, &g));
+  d->capturing = false;
+  std::memcpy(d->par, saved, sizeof(saved));
+  d->step_slot = saved_slot;
+  HIP_CHECK(hipGraphInstantiate(&d->gexec[slot], g, nullptr, nullptr, 0));
+  HIP_CHECK(hipGraphDestroy(g));
+}
+
+}  // namespace
+
+int octlb_dev_prepare_graphs(octlb_dev* d) {
+  return Guard([&] {
+    CheckDev(d);
+    EnsureReady(d);
+    for (int slot = 0; slot < 4; ++slot)
+      if (!d->gexec[slot]) CaptureGraph(d, slot, slot < 2 ? 1 : octlb_dev::kGraphSteps);
+    d->Sync();
+  });
+}
+
 int octlb_dev_advance_graph(octlb_dev* d, int32_t steps) {
   return Guard([&] {
     CheckDev(d);
@@ -1775,38 +1983,17 @@ int octlb_dev_advance_graph(octlb_dev* d, int32_t steps) {
     // Replays captured hipGraphs: kGraphSteps coarse steps per launch while
     // enough remain (one graph launch per coarse step would leave a launch gap
     // between every step), single steps for the rest.
-    auto capture = [&](int slot, int count) {
-      int saved[kMaxLevels];
-      std::memcpy(saved, d->par, sizeof(saved));
-      const int saved_slot = d->step_slot;
-      hipGraph_t g = nullptr;
-      d->capturing = true;
-      HIP_CHECK(hipStreamBeginCapture(d->stream, hipStreamCaptureModeThreadLocal));
-      for (int k = 0; k < count; ++k) {
-        if (d->prec == 64) {
-          d->CoarseStep<double>();
-        } else {
-          d->CoarseStep<float>();
-        }
-      }
-      HIP_CHECK(hipStreamEndCapture(d->stream, &g));
-      d->capturing = false;
-      std::memcpy(d->par, saved, sizeof(saved));
-      d->step_slot = saved_slot;
-      HIP_CHECK(hipGraphInstantiate(&d->gexec[slot], g, nullptr, nullptr, 0));
-      HIP_CHECK(hipGraphDestroy(g));
-    };
     while (steps > 0) {
       const int p0 = d->par[0];
       if (steps >= octlb_dev::kGraphSteps) {
         // an even number of coarse steps leaves every level's parity as it was
-        if (!d->gexec[2 + p0]) capture(2 + p0, octlb_dev::kGraphSteps);
+        if (!d->gexec[2 + p0]) CaptureGraph(d, 2 + p0, octlb_dev::kGraphSteps);
         HIP_CHECK(hipGraphLaunch(d->gexec[2 + p0], d->stream));
         d->step += octlb_dev::kGraphSteps;
         steps -= octlb_dev::kGraphSteps;
         continue;
       }
-      if (!d->gexec[p0]) capture(p0, 1);
+      if (!d->gexec[p0]) CaptureGraph(d, p0, 1);
       HIP_CHECK(hipGraphLaunch(d->gexec[p0], d->stream));
       d->par[0] ^= 1;  // finer levels step an even number of times per coarse step
       d->step_slot = d->par[0];
@@ -1830,6 +2017,17 @@ int octlb_dev_set_step(octlb_dev* d, int32_t step) {
     d->step = step;
     HIP_CHECK(hipMemcpyAsync(d->d_step + d->step_slot, &d->step, sizeof(int), hipMemcpyHostToDevice, d->stream));
     d->Sync();
+  });
+}
+
+int octlb_dev_set_snapshot(octlb_dev* d, int32_t on) {
+  return Guard([&] {
+    CheckDev(d);
+    if ((on != 0) != d->snapshot) {
+      d->Sync();
+      d->snapshot = on != 0;
+      d->prepared = false;  // drag flags and snapshot slots are laid out by Prepare
+    }
   });
 }
 
@@ -1952,7 +2150,63 @@ int octlb_dev_time_kernel(octlb_dev* d, int32_t level, int32_t reps, double* avg
     if (cell_updates) {
       const int tpa = (d->NP + kTile - 1) / kTile;
       const int tpb = d->padded ? tpa * tpa * tpa : 1;
-      *cell_updates = static_cast<double>(d->tile_count[level] / tpb) * static_cast<double>(d->ncore);
+      (void)tpb;
+      *cell_updates = static_cast<double>(d->level_blocks[level]) * static_cast<double>(d->ncore);
+    }
+  });
+}
+
+int octlb_dev_set_kernel_variant(octlb_dev* d, int32_t variant) {
+  return Guard([&] {
+    CheckDev(d);
+    if (variant != 0 && variant != 1) throw ArgError("kernel variant must be 0 (pull) or 1 (push where plain)");
+    if (variant != d->variant) {
+      d->Sync();
+      d->variant = variant;
+      d->prepared = false;  // the block classes and the tile list are laid out by Prepare
+    }
+  });
+}
+
+int octlb_dev_kernel_variant(octlb_dev* d, int32_t* variant, int32_t* push_blocks) {
+  return Guard([&] {
+    CheckDev(d);
+    EnsureReady(d);
+    if (variant) *variant = d->variant;
+    if (push_blocks) *push_blocks = d->npush_blocks;
+  });
+}
+
+int octlb_dev_autotune(octlb_dev* d, int32_t reps, double* step_ms) {
+  return Guard([&] {
+    CheckDev(d);
+    if (reps <= 0) throw ArgError("reps must be positive");
+    // Per candidate: the fused kernel of every level replayed `reps` times
+    // on the current state (inputs only read, outputs to the idle buffers),
+    // weighted by its 2^l launches per coarse step.  The state is unchanged.
+    double best = 0.0;
+    int best_v = d->variant;
+    for (int v = 0; v < 2; ++v) {
+      d->variant = v;
+      d->prepared = false;
+      EnsureReady(d);
+      if (v == 1 && d->npush_blocks == 0) continue;
+      double ms = 0.0;
+      for (int l = 0; l <= d->max_level; ++l) {
+        if (d->tile_count[l] == 0) continue;
+        const double t = d->prec == 64 ? d->TimeKernel<double>(l, reps) : d->TimeKernel<float>(l, reps);
+        ms += t * static_cast<double>(1 << l);
+      }
+      if (step_ms) step_ms[v] = ms;
+      if (v == 0 || ms < best) {
+        best = ms;
+        best_v = v;
+      }
+    }
+    if (d->variant != best_v) {
+      d->variant = best_v;
+      d->prepared = false;
+      EnsureReady(d);
     }
   });
 }

@@ -51,21 +51,72 @@ enum : int {
 __host__ __device__ inline bool ReflectsOnPull(int k) { return k == kVelocity || k == kPressure || k == kBouzidi; }
 __host__ __device__ inline bool IsFd(int k) { return k == kVelocity || k == kPressure; }
 
+// FP32 arithmetic on shifted populations.  Storing f - t pays only if the
+// arithmetic stays shifted too: a density carried as rho = 1 + sum(f) in FP32
+// resolves 2^-24 in absolute terms, and rho t (1 + X) - t then cancels the
+// weight t and loses the low bits of every population.  The FP32 path
+// therefore carries the density deviation d = rho - 1 (summed from the
+// shifted populations, never adding the 1) wherever the FP64 path carries
+// rho, and forms feq - t = t d + (1 + d) t X, which has no cancellation.
+// FP64 keeps the reference's expressions bit for bit.  In the functions
+// below a parameter `r` is rho for FP64 and rho - 1 for FP32.
+template <typename T>
+constexpr bool kShiftedArith = sizeof(T) == 4;
+
+// rho from r.
+template <typename T>
+__host__ __device__ inline T RhoOf(T r) {
+  if constexpr (kShiftedArith<T>) {
+    return T(1) + r;
+  } else {
+    return r;
+  }
+}
+// r of the rest density (rho = 1).
+template <typename T>
+__host__ __device__ inline T RestR() {
+  return kShiftedArith<T> ? T(0) : T(1);
+}
+// r of a density given in FP64.
+template <typename T>
+__host__ __device__ inline T ROf(double rho) {
+  if constexpr (kShiftedArith<T>) {
+    return static_cast<T>(rho - 1.0);
+  } else {
+    return static_cast<T>(rho);
+  }
+}
+// rho in FP64 from r.
+template <typename T>
+__host__ __device__ inline double RhoDouble(T r) {
+  if constexpr (kShiftedArith<T>) {
+    return 1.0 + static_cast<double>(r);
+  } else {
+    return static_cast<double>(r);
+  }
+}
+
 // Terms c_i,d * x with c_i,d = 0 are skipped: for finite x they add a
 // signed zero to an accumulator that starts at +0 and never becomes -0, so
 // every result is bit-identical to the full sum, and the unrolled loops lose
 // a third of their multiplies and adds.  (i is a compile-time constant in all
 // callers after unrolling, so the tests fold away.)
 
-// equilibrium::secondOrder on shifted populations (dynamics/lbm.h:51-62).
+// equilibrium::secondOrder on shifted populations (dynamics/lbm.h:51-62);
+// r as above.
 template <typename T>
-__host__ __device__ inline T Feq(int i, T rho, const T* u, T usqr) {
+__host__ __device__ inline T Feq(int i, T r, const T* u, T usqr) {
   T cu = T(0);
   if (Cx(i) != 0) cu += T(Cx(i)) * u[0];
   if (Cy(i) != 0) cu += T(Cy(i)) * u[1];
   if (Cz(i) != 0) cu += T(Cz(i)) * u[2];
   const T t = W<T>(i);
-  return rho * t * (T(1) + T(3) * cu + T(3) * T(3) * T(0.5) * cu * cu - T(3) * T(0.5) * usqr) - t;
+  if constexpr (kShiftedArith<T>) {
+    return t * r + (T(1) + r) * t * (T(3) * cu + T(3) * T(3) * T(0.5) * cu * cu - T(3) * T(0.5) * usqr);
+  } else {
+    const T rho = r;
+    return rho * t * (T(1) + T(3) * cu + T(3) * T(3) * T(0.5) * cu * cu - T(3) * T(0.5) * usqr) - t;
+  }
 }
 
 template <typename T>
@@ -78,8 +129,26 @@ __host__ __device__ inline T NormSqr(const T* u) {
 }
 
 // CellProxy::computeRhoU: rho starts at 1 (block_lattice.h:113-125).
+// Returns r (rho for FP64, rho - 1 for FP32) in `rho`.
 template <typename T>
 __host__ __device__ inline void ProxyRhoU(const T* f, T& rho, T* u) {
+  if constexpr (kShiftedArith<T>) {
+    T d = T(0);
+    u[0] = u[1] = u[2] = T(0);
+#pragma unroll
+    for (int i = 0; i < kQ; ++i) {
+      d += f[i];
+      if (Cx(i) != 0) u[0] += f[i] * T(Cx(i));
+      if (Cy(i) != 0) u[1] += f[i] * T(Cy(i));
+      if (Cz(i) != 0) u[2] += f[i] * T(Cz(i));
+    }
+    const T full = T(1) + d;
+    u[0] /= full;
+    u[1] /= full;
+    u[2] /= full;
+    rho = d;
+    return;
+  }
   T r = T(1);
   u[0] = u[1] = u[2] = T(0);
 #pragma unroll
@@ -96,11 +165,27 @@ __host__ __device__ inline void ProxyRhoU(const T* f, T& rho, T* u) {
 }
 
 // olb::lbm::computeRhoU: rho = sum(f) + 1 (dynamics/lbm.h:290-331).
+// Returns r (rho for FP64, rho - 1 for FP32) in `rho`.
 template <typename T>
 __host__ __device__ inline void OlbRhoU(const T* f, T& rho, T* u) {
   T r = T(0);
 #pragma unroll
   for (int i = 0; i < kQ; ++i) r += f[i];
+  if constexpr (kShiftedArith<T>) {
+    u[0] = u[1] = u[2] = T(0);
+#pragma unroll
+    for (int i = 0; i < kQ; ++i) {
+      if (Cx(i) != 0) u[0] += f[i] * T(Cx(i));
+      if (Cy(i) != 0) u[1] += f[i] * T(Cy(i));
+      if (Cz(i) != 0) u[2] += f[i] * T(Cz(i));
+    }
+    const T full = T(1) + r;
+    u[0] /= full;
+    u[1] /= full;
+    u[2] /= full;
+    rho = r;
+    return;
+  }
   r += T(1);
   u[0] = u[1] = u[2] = T(0);
 #pragma unroll
@@ -139,10 +224,23 @@ __host__ __device__ inline void CollideBgkAt(T* f, T omega) {
   }
 }
 
-// CollideConstRhoBgkWithMacroscopic (block_lattice.h:133-151).
+// CollideConstRhoBgkWithMacroscopic (block_lattice.h:133-151).  avg is the
+// average density for FP64 and its deviation avg - 1 for FP32 (AvgArg).
+// FP32: ratio (feq + t) - t = (rho + 1 - avg) t (1 + X) - t = t e + (1 + e) t X
+// with e = (rho - 1) - (avg - 1).
 template <typename T>
 __host__ __device__ inline void ConstRhoBgk(T* f, T omega, T avg, T rho, const T* u) {
   const T usqr = NormSqr(u);
+  if constexpr (kShiftedArith<T>) {
+    const T e = (T(1) + rho) > T(1e-12) ? rho - avg : rho;
+#pragma unroll
+    for (int i = 0; i < kQ; ++i) {
+      const T fe = Feq(i, rho, u, usqr);
+      const T scaled = Feq(i, e, u, usqr);
+      f[i] = scaled + (T(1) - omega) * (f[i] - fe);
+    }
+    return;
+  }
   const T delta = T(1) - avg;
   const T ratio = rho > T(1e-12) ? T(1) + delta / rho : T(1);
 #pragma unroll
@@ -192,6 +290,8 @@ __host__ __device__ inline double BouzidiPull(double f_bb_opp, double f_interior
 
 // VelocityBoundaryRhoFromPop (interpolated_velocity.h:170-211) for a fixed
 // face direction / orientation (compile-time, like the reference's template).
+// Returns r: rho for FP64; for FP32 rho - 1 = (2 normal + on_wall - O u) /
+// (1 + O u), the same quotient without the cancelling 1.
 template <int D, int O, typename T>
 __host__ __device__ inline T VelRhoImpl(const T* f, const T* u) {
   T on_wall = T(0), normal = T(0);
@@ -201,7 +301,21 @@ __host__ __device__ inline T VelRhoImpl(const T* f, const T* u) {
 #pragma unroll
   for (int i = 0; i < kQ; ++i)
     if (Ci(i, D) == O) normal += f[i];
-  return (T(2) * normal + on_wall + T(1)) / (T(1) + T(O) * u[D]);
+  if constexpr (kShiftedArith<T>) {
+    return (T(2) * normal + on_wall - T(O) * u[D]) / (T(1) + T(O) * u[D]);
+  } else {
+    return (T(2) * normal + on_wall + T(1)) / (T(1) + T(O) * u[D]);
+  }
+}
+
+// The ConstRho average as ConstRhoBgk takes it: avg (FP64), avg - 1 (FP32).
+template <typename T>
+__host__ __device__ inline T AvgArg(double avg) {
+  if constexpr (kShiftedArith<T>) {
+    return static_cast<T>(avg - 1.0);
+  } else {
+    return static_cast<T>(avg);
+  }
 }
 
 // Runtime dispatch; a zero orientation takes the +1 instantiation like the

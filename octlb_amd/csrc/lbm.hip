@@ -67,6 +67,33 @@ __device__ inline void BlockReduceStats(double s, int c, double* red_s, int* red
   }
 }
 
+// Statistics of a push tile, one partial per block: each half reduces its
+// 512 values with the tree a pull tile applies to its cells (whose other
+// half holds zeros), so a block's partial does not depend on the tiling.
+template <typename T>
+__device__ inline void HalfReduceStats(double s, int c, double* red_s, int* red_c, double* out0, double* out1) {
+  const int tid = threadIdx.x;
+  __syncthreads();  // red_s / red_c alias the streaming planes
+  red_s[tid] = s;
+  red_c[tid] = c;
+  __syncthreads();
+  for (int w = kTileCells / 2; w > 0; w >>= 1) {
+    if ((tid & (kTileCells - 1)) < w) {
+      red_s[tid] += red_s[tid + w];
+      red_c[tid] += red_c[tid + w];
+    }
+    __syncthreads();
+  }
+  if (tid == 0) {
+    out0[0] = red_s[0];
+    out0[1] = static_cast<double>(red_c[0]);
+  }
+  if (tid == kTileCells && out1 != nullptr) {
+    out1[0] = red_s[kTileCells];
+    out1[1] = static_cast<double>(red_c[kTileCells]);
+  }
+}
+
 // Pull-stream rule of one core cell (block_lattice.cpp:579-674) for
 // population i (a constant after the caller's unrolling), reading the LDS
 // box of post-collide values: pl = plane of population i, po = plane of its
@@ -93,6 +120,85 @@ __device__ __forceinline__ T StreamPop(int i, const T* pl, const T* po, const ui
     return own_opp;
   }
   return pulled;
+}
+
+// ------------------------------------------------------------- push streaming
+// Push form of the plain pull (block_lattice.cpp:579-674 read from the source
+// side): the post-collide value f of population i at core cell (x, y, z) of
+// block b streams into cell (x, y, z) + c_i -- in b, or in the push-class
+// block bt.push_to[part] whose halo part opposite to `part` holds b's cells
+// (that block would have pulled it from there).  Destinations in pull-class
+// blocks are skipped: those blocks pull (and re-collide) the value themselves.
+template <typename T>
+__device__ __forceinline__ void PushPop(int i, T f, const BlockTab& bt, int b, int x, int y, int z, T* dst) {
+  constexpr int n = kTile, ncore = kTileCells;
+  const int dx = x + Cx(i), dy = y + Cy(i), dz = z + Cz(i);
+  const int ox = dx < 0 ? -1 : (dx >= n ? 1 : 0);
+  const int oy = dy < 0 ? -1 : (dy >= n ? 1 : 0);
+  const int oz = dz < 0 ? -1 : (dz >= n ? 1 : 0);
+  int db = b;
+  if (ox != 0 || oy != 0 || oz != 0) {
+    int part;
+    if ((ox != 0) + (oy != 0) + (oz != 0) == 1) {
+      part = ox != 0 ? (ox > 0 ? 1 : 0) : (oy != 0 ? 2 + (oy > 0 ? 1 : 0) : 4 + (oz > 0 ? 1 : 0));
+    } else {
+      const int a1 = ox != 0 ? 0 : 1, a2 = oz != 0 ? 2 : 1;
+      const int s1 = (a1 == 0 ? ox : oy) > 0 ? 1 : 0, s2 = (a2 == 2 ? oz : oy) > 0 ? 1 : 0;
+      part = EdgeSlot(a1, s1, a2, s2);
+    }
+    db = bt.push_to[part];
+    if (db < 0) return;
+  }
+  const int wx = dx - ox * n, wy = dy - oy * n, wz = dz - oz * n;
+  dst[static_cast<size_t>(db) * kQ * ncore + static_cast<size_t>(i) * ncore + (wx * n + wy) * n + wz] = f;
+}
+
+// Pushes the populations of a boundary cell that cross the block's surface
+// into push-class neighbours (pull-class blocks with kFlagPushOut).
+template <typename T>
+__device__ __forceinline__ void PushOut(const T* f, const BlockTab& bt, int b, int x, int y, int z, T* dst) {
+  constexpr int n = kTile;
+  if (x > 0 && x < n - 1 && y > 0 && y < n - 1 && z > 0 && z < n - 1) return;
+#pragma unroll
+  for (int i = 1; i < kQ; ++i) {
+    const int dx = x + Cx(i), dy = y + Cy(i), dz = z + Cz(i);
+    if (dx >= 0 && dx < n && dy >= 0 && dy < n && dz >= 0 && dz < n) continue;
+    PushPop<T>(i, f[i], bt, b, x, y, z, dst);
+  }
+}
+
+// Push tile body: two push-class blocks, one per half of the 1,024-thread
+// workgroup, one thread per cell: load the cell's 19 populations
+// (coalesced), collide, push.  No apron, no LDS exchange, no barrier (except
+// for the statistics reduction).
+template <typename T>
+__device__ __forceinline__ void PushTile(const TileArgs<T>& a, const Tile& tl, double* red_s, int* red_c) {
+  constexpr int n = kTile, ncore = kTileCells;
+  const int tid = threadIdx.x;
+  const int half = tid >= ncore ? 1 : 0;
+  const int b = half ? tl.ox : tl.block;
+  const int c = tid - half * ncore;
+  double ssum = 0.0;
+  int scnt = 0;
+  if (b >= 0) {
+    const BlockTab& bt = *(const BlockTab*)&Ro(a.bt)[b];
+    const SimConst& sc = *(const SimConst*)&a.scm[bt.member];
+    const int x = c >> 6, y = (c >> 3) & 7, z = c & 7;
+    T f[kQ];
+    const size_t base = static_cast<size_t>(b) * kQ * ncore + c;
+#pragma unroll
+    for (int i = 0; i < kQ; ++i) f[i] = Ro(a.src)[base + static_cast<size_t>(i) * ncore];
+    const int k = Ro(a.kind)[static_cast<size_t>(b) * ncore + c];
+    const double t = static_cast<double>(*Ro(a.step));
+    const T avg = AvgArg<T>(sc.const_rho ? Ro(a.avg_rho)[bt.member] : 1.0);
+    const Origin og{bt.origin[0], bt.origin[1], bt.origin[2], bt.cw};
+    CollideCell<T>(f, k, sc, og, x, y, z, t, avg, a.want_stats != 0, ssum, scnt);
+#pragma unroll
+    for (int i = 0; i < kQ; ++i) PushPop<T>(i, f[i], bt, b, x, y, z, a.dst);
+  }
+  if (a.want_stats == 1)
+    HalfReduceStats<T>(ssum, scnt, red_s, red_c, a.stats + 2 * a.block_rank[tl.block],
+                       tl.ox >= 0 ? a.stats + 2 * a.block_rank[tl.ox] : nullptr);
 }
 
 // ------------------------------------------------------- blocked fused kernel
@@ -158,7 +264,12 @@ __global__ __launch_bounds__(1024, BlockOccupancy<T>::kWavesPerSimd) void KBlock
     return;
   }
   const int ti = XcdTile(static_cast<int>(blockIdx.x) - a.fd_wgs - a.pc_wgs, a.ntiles);
-  const int b = Ro(a.tiles)[ti].block;
+  const Tile tl = Ro(a.tiles)[ti];
+  if (tl.oy == 1) {
+    PushTile<T>(a, tl, red_s, red_c);
+    return;
+  }
+  const int b = tl.block;
   constexpr int n = kTile, ncore = kTileCells;
   const int tid = threadIdx.x;
   const bool own = tid < ncore;
@@ -235,11 +346,12 @@ __global__ __launch_bounds__(1024, BlockOccupancy<T>::kWavesPerSimd) void KBlock
   int scnt = 0;
   if (src_b >= 0) {
     const double t = static_cast<double>(*Ro(a.step));
-    const T avg = sc.const_rho ? static_cast<T>(Ro(a.avg_rho)[bt.member]) : T(1);
+    const T avg = AvgArg<T>(sc.const_rho ? Ro(a.avg_rho)[bt.member] : 1.0);
     const BlockTab& st = *(const BlockTab*)&Ro(a.bt)[src_b];
     const Origin og{st.origin[0], st.origin[1], st.origin[2], st.cw};
     CollideCell<T>(f, k, sc, og, cx, cy, cz, t, avg, own && a.want_stats != 0, ssum, scnt);
   }
+  if (own && (bflags & kFlagPushOut)) PushOut<T>(f, bt, b, cx, cy, cz, a.dst);
   if (l >= 0) {
 #pragma unroll
     for (int i = 0; i < kQ; ++i) plane[i][l] = f[i];
@@ -295,9 +407,9 @@ __global__ __launch_bounds__(1024, BlockOccupancy<T>::kWavesPerSimd) void KBlock
     }
   }
   if (a.want_stats == 1) {
-    BlockReduceStats<T>(ssum, scnt, red_s, red_c, a.stats + 2 * ti);
+    BlockReduceStats<T>(ssum, scnt, red_s, red_c, a.stats + 2 * a.block_rank[b]);
   } else if (a.want_stats == 2 && own) {
-    const size_t ord = static_cast<size_t>(ti) * ncore + tid;
+    const size_t ord = static_cast<size_t>(a.block_rank[b]) * ncore + tid;
     a.stat_val[ord] = ssum;
     a.stat_use[ord] = static_cast<uint8_t>(scnt);
   }
@@ -325,7 +437,7 @@ __global__ __launch_bounds__(512) void KPadded(TileArgs<T> a) {
   const int tid = threadIdx.x;
   const size_t bbase = static_cast<size_t>(b) * kQ * NS;
   const double t = static_cast<double>(*a.step);
-  const T avg = sc.const_rho ? static_cast<T>(*a.avg_rho) : T(1);
+  const T avg = AvgArg<T>(sc.const_rho ? *a.avg_rho : 1.0);
   const BlockTab& bt = a.bt[b];
   const Origin borg{bt.origin[0], bt.origin[1], bt.origin[2], bt.cw};
   double ssum = 0.0;
@@ -644,11 +756,11 @@ struct FdView {
   }
   __device__ T PrescribedRho(int x, int y, int z) const {
     int dir, ori;
-    if (FlatInfo(sc->n, x, y, z, dir, ori)) return static_cast<T>(sc->spec[dir * 2 + (ori < 0 ? 0 : 1)].rho_target);
-    return T(1);
+    if (FlatInfo(sc->n, x, y, z, dir, ori)) return ROf<T>(sc->spec[dir * 2 + (ori < 0 ? 0 : 1)].rho_target);
+    return RestR<T>();
   }
   __device__ T ComputeRho(int x, int y, int z) const {
-    if (Padding(x, y, z)) return T(1);
+    if (Padding(x, y, z)) return RestR<T>();
     if (!InDomain(x, y, z)) {
       const int n = sc->n;
       x = Mirror(x, n);
@@ -716,9 +828,10 @@ __device__ inline void WriteFromPi(T* f, T rho, const T* u, const T* pi) {
   for (int i = 0; i < kQ; ++i) f[i] = Feq(i, rho, u, usqr) + PiToFneq(i, pi);
 }
 
+// rho is r (rho for FP64, rho - 1 for FP32)
 template <typename T>
 __device__ inline void PiFromGrad(T rho, T omega, const T* dx, const T* dy, const T* dz, T* pi) {
-  const T s = -rho / T(3) / omega;
+  const T s = -RhoOf(rho) / T(3) / omega;
   pi[0] = T(2) * dx[0] * s;
   pi[3] = T(2) * dy[1] * s;
   pi[5] = T(2) * dz[2] * s;
@@ -787,7 +900,7 @@ __device__ inline void FdCell(const V& v, int ix, int iy, int iz, T* fnew) {
     v.NormalGrad(ix, iy, iz, d2, n2, g[d2]);
     T u[3];
     v.ComputeU(ix, iy, iz, u);
-    const T s = -rho / T(3) / omega;
+    const T s = -RhoOf(rho) / T(3) / omega;
     pi[0] = T(2) * g[0][0] * s;
     pi[3] = T(2) * g[1][1] * s;
     pi[5] = T(2) * g[2][2] * s;
@@ -902,7 +1015,7 @@ struct FdLds {
   }
   __device__ T ComputeRho(int x, int y, int z) const {
     constexpr int n = kTile;
-    if (Padding(x, y, z)) return T(1);
+    if (Padding(x, y, z)) return RestR<T>();
     const int c = (Mirror(x, n) * n + Mirror(y, n)) * n + Mirror(z, n);
     return (slot[c] >= 0 && c < self) ? rho_post[c] : rho_pre[c];
   }
@@ -935,7 +1048,7 @@ template <typename T>
 __device__ inline T FdRhoOf(int k, const T* f, const T* pu, const SimConst& sc, int ix, int iy, int iz) {
   int dir, ori;
   const bool flat = FlatInfo(sc.n, ix, iy, iz, dir, ori);
-  if (k == kPressure) return flat ? static_cast<T>(sc.spec[dir * 2 + (ori < 0 ? 0 : 1)].rho_target) : T(1);
+  if (k == kPressure) return flat ? ROf<T>(sc.spec[dir * 2 + (ori < 0 ? 0 : 1)].rho_target) : RestR<T>();
   if (k == kVelocity && flat) return VelRhoFromPop(dir, ori, f, pu);
   T rho, u[3];
   ProxyRhoU(f, rho, u);
@@ -955,7 +1068,12 @@ __device__ inline T FdPopulation(const int* c, T t, T rho, const T* u, T usqr, c
   cu += T(c[0]) * u[0];
   cu += T(c[1]) * u[1];
   cu += T(c[2]) * u[2];
-  const T feq = rho * t * (T(1) + T(3) * cu + T(3) * T(3) * T(0.5) * cu * cu - T(3) * T(0.5) * usqr) - t;
+  T feq;
+  if constexpr (kShiftedArith<T>) {  // rho is rho - 1 (shifted FP32 arithmetic)
+    feq = t * rho + (T(1) + rho) * t * (T(3) * cu + T(3) * T(3) * T(0.5) * cu * cu - T(3) * T(0.5) * usqr);
+  } else {
+    feq = rho * t * (T(1) + T(3) * cu + T(3) * T(3) * T(0.5) * cu * cu - T(3) * T(0.5) * usqr) - t;
+  }
   T fneq = T(0);
   int ipi = 0;
 #pragma unroll
@@ -1448,56 +1566,90 @@ __global__ __launch_bounds__(512) void KFdCouple(FdArgs<T> fa, const int* fd_blo
 // MomentumExchangeDrag::force_on_fluid_if (drag_postprocessor.h:66-115):
 // one term per wall link, summed per block in the reference's loop order
 // (iz, iy, ix, iPop) and over blocks in block order, all in FP64.
+// MomentumExchangeDrag::force_on_fluid_if of one block per workgroup
+// (drag_postprocessor.h:66-115): the threads evaluate the block's link terms
+// (f_out + f_bar, the bbox predicate of cylinder3d_case.h:483-490) into LDS,
+// then one thread sums them in the reference's loop order -- bit-equal to the
+// sequential loop, with no global load on the dependent summation chain.
+constexpr int kDragThreads = 256;
+constexpr int kDragChunk = 2048;
+
 template <typename T>
-__global__ void KDragTerms(DragArgs<T> a, int nlinks) {
-  const int j = static_cast<int>(blockIdx.x * blockDim.x + threadIdx.x);
-  if (j >= nlinks) return;
-  const int4 L = a.links[j];  // block, cell, pop, snapshot index
-  const int b = L.x, c = L.y, i = L.z;
+__global__ __launch_bounds__(kDragThreads) void KDragBlock(DragArgs<T> a, const int* block_begin, double* out) {
+  __shared__ double term[kDragChunk];
+  __shared__ int8_t pop[kDragChunk];  // -1: link outside the predicate
+  const int j = blockIdx.x, tid = threadIdx.x;
+  const int begin = block_begin[j], end = block_begin[j + 1];
   const SimConst& sc = a.sc;
   const int n = sc.n;
-  const int ix = c / (n * n), iy = (c / n) % n, iz = c % n;
-  const double* org = a.bt[b].origin;
-  const double cw = a.bt[b].cw;
-  const double px = org[0] + (ix + Cx(i) + 0.5) * cw;
-  const double py = org[1] + (iy + Cy(i) + 0.5) * cw;
-  const double pz = org[2] + (iz + Cz(i) + 0.5) * cw;
-  const bool in = px >= a.bbox[0] && px <= a.bbox[3] && py >= a.bbox[1] && py <= a.bbox[4] &&
-                  pz >= a.bbox[2] && pz <= a.bbox[5];
-  a.use[j] = in ? 1 : 0;
-  const T* live = (a.par.v[a.bt[b].level] ? a.pops1 : a.pops0) + static_cast<size_t>(b) * kQ * sc.plane;
-  const int s = CoreSlot(sc, c);
-  const double fbar = static_cast<double>(live[static_cast<size_t>(Opp(i)) * sc.plane + s]);
-  const double fout = a.snap ? static_cast<double>(a.snap[L.w]) : static_cast<double>(live[static_cast<size_t>(i) * sc.plane + s]);
-  a.term[j] = fout + fbar;
-}
-
-__global__ void KDragBlocks(const int4* links, const int* block_begin, const int* blocks, int nblk,
-                            const double* term, const uint8_t* use, double* out) {
-  const int j = static_cast<int>(blockIdx.x * blockDim.x + threadIdx.x);
-  if (j >= nblk) return;
-  double fo[3] = {0.0, 0.0, 0.0};
-  for (int e = block_begin[j]; e < block_begin[j + 1]; ++e) {
-    if (!use[e]) continue;
-    const int i = links[e].z;
-    fo[0] += term[e] * static_cast<double>(Cx(i));
-    fo[1] += term[e] * static_cast<double>(Cy(i));
-    fo[2] += term[e] * static_cast<double>(Cz(i));
+  double fo0 = 0.0, fo1 = 0.0, fo2 = 0.0;
+  for (int c0 = begin; c0 < end; c0 += kDragChunk) {
+    const int m = min(kDragChunk, end - c0);
+    for (int e = tid; e < m; e += kDragThreads) {
+      const int4 L = a.links[c0 + e];  // block, cell, pop, snapshot index
+      const int b = L.x, c = L.y, i = L.z;
+      const int ix = c / (n * n), iy = (c / n) % n, iz = c % n;
+      const double* org = a.bt[b].origin;
+      const double cw = a.bt[b].cw;
+      const double px = org[0] + (ix + kCxT[i] + 0.5) * cw;
+      const double py = org[1] + (iy + kCyT[i] + 0.5) * cw;
+      const double pz = org[2] + (iz + kCzT[i] + 0.5) * cw;
+      const bool in = px >= a.bbox[0] && px <= a.bbox[3] && py >= a.bbox[1] && py <= a.bbox[4] &&
+                      pz >= a.bbox[2] && pz <= a.bbox[5];
+      const T* live = (a.par.v[a.bt[b].level] ? a.pops1 : a.pops0) + static_cast<size_t>(b) * kQ * sc.plane;
+      const int sl = CoreSlot(sc, c);
+      const int opp = i == 0 ? 0 : (i <= 9 ? i + 9 : i - 9);
+      const double fbar = static_cast<double>(live[static_cast<size_t>(opp) * sc.plane + sl]);
+      const double fout =
+          a.snap ? static_cast<double>(a.snap[L.w]) : static_cast<double>(live[static_cast<size_t>(i) * sc.plane + sl]);
+      term[e] = fout + fbar;
+      pop[e] = in ? static_cast<int8_t>(i) : static_cast<int8_t>(-1);
+    }
+    __syncthreads();
+    if (tid == 0) {
+      for (int e = 0; e < m; ++e) {
+        const int i = pop[e];
+        if (i < 0) continue;
+        const double t = term[e];
+        fo0 += t * static_cast<double>(kCxT[i]);
+        fo1 += t * static_cast<double>(kCyT[i]);
+        fo2 += t * static_cast<double>(kCzT[i]);
+      }
+    }
+    __syncthreads();
   }
-  out[3 * j] = -fo[0];
-  out[3 * j + 1] = -fo[1];
-  out[3 * j + 2] = -fo[2];
-  (void)blocks;
+  if (tid == 0) {
+    out[3 * j] = -fo0;
+    out[3 * j + 1] = -fo1;
+    out[3 * j + 2] = -fo2;
+  }
 }
 
-__global__ void KSum3(const double* v, int count, double* out) {
-  if (threadIdx.x != 0 || blockIdx.x != 0) return;
-  double s[3] = {0.0, 0.0, 0.0};
-  for (int j = 0; j < count; ++j)
-    for (int d = 0; d < 3; ++d) s[d] += v[3 * j + d];
-  out[0] = s[0];
-  out[1] = s[1];
-  out[2] = s[2];
+// The cylinder case's sum of the per-block forces in block order
+// (cylinder3d_case.h:492-497), staged through LDS so the sequential chain
+// reads no global memory.  Blocks without links add -0.0, which leaves any
+// sum unchanged, so only blocks with links are listed.
+__global__ __launch_bounds__(kDragThreads) void KSum3(const double* v, int count, double* out) {
+  __shared__ double st[3 * kDragChunk];
+  const int tid = threadIdx.x;
+  double s0 = 0.0, s1 = 0.0, s2 = 0.0;
+  for (int c0 = 0; c0 < count; c0 += kDragChunk) {
+    const int m = min(kDragChunk, count - c0);
+    for (int e = tid; e < 3 * m; e += kDragThreads) st[e] = v[3 * c0 + e];
+    __syncthreads();
+    if (tid == 0)
+      for (int e = 0; e < m; ++e) {
+        s0 += st[3 * e];
+        s1 += st[3 * e + 1];
+        s2 += st[3 * e + 2];
+      }
+    __syncthreads();
+  }
+  if (tid == 0) {
+    out[0] = s0;
+    out[1] = s1;
+    out[2] = s2;
+  }
 }
 
 // Cylinder3dCase::total_mass (cylinder3d_case.h:601-622): rho over kBulk
@@ -1517,7 +1669,7 @@ __global__ __launch_bounds__(256) void KMass(const T* pops0, const T* pops1, Par
     for (int i = 0; i < kQ; ++i) f[i] = base[static_cast<size_t>(i) * sc.plane + sl];
     T rho, u[3];
     ProxyRhoU(f, rho, u);
-    s += static_cast<double>(rho);
+    s += RhoDouble(rho);
   }
   red[threadIdx.x] = s;
   __syncthreads();
@@ -1528,11 +1680,20 @@ __global__ __launch_bounds__(256) void KMass(const T* pops0, const T* pops1, Par
   if (threadIdx.x == 0) partial[b] = red[0];
 }
 
-__global__ void KSum1(const double* v, int count, double* out) {
-  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+// Sum of the per-block masses: each thread sums a fixed strided subset,
+// then a fixed tree -- deterministic, one workgroup.
+__global__ __launch_bounds__(256) void KSum1(const double* v, int count, double* out) {
+  __shared__ double red[256];
+  const int tid = threadIdx.x;
   double s = 0.0;
-  for (int j = 0; j < count; ++j) s += v[j];
-  *out = s;
+  for (int j = tid; j < count; j += 256) s += v[j];
+  red[tid] = s;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if (tid < w) red[tid] += red[tid + w];
+    __syncthreads();
+  }
+  if (tid == 0) *out = red[0];
 }
 
 // ---------------------------------------------------------- instantiations
@@ -1630,18 +1791,16 @@ void LaunchRestrict(const CoupleArgs<T>& a, const int* list, int count, hipStrea
 template <typename T>
 void LaunchDrag(const DragArgs<T>& a, int nlinks, const int* block_begin, int nblk, double* block_out,
                 double* out, hipStream_t s) {
-  if (nlinks > 0) hipLaunchKernelGGL(KDragTerms<T>, Grid(nlinks, 256), dim3(256), 0, s, a, nlinks);
-  if (nblk > 0)
-    hipLaunchKernelGGL(KDragBlocks, Grid(nblk, 64), dim3(64), 0, s, a.links, block_begin, nullptr, nblk,
-                       a.term, a.use, block_out);
-  hipLaunchKernelGGL(KSum3, dim3(1), dim3(1), 0, s, block_out, nblk, out);
+  (void)nlinks;
+  if (nblk > 0) hipLaunchKernelGGL(KDragBlock<T>, dim3(nblk), dim3(kDragThreads), 0, s, a, block_begin, block_out);
+  hipLaunchKernelGGL(KSum3, dim3(1), dim3(kDragThreads), 0, s, block_out, nblk, out);
 }
 
 template <typename T>
 void LaunchMass(const T* p0, const T* p1, const ParArray& par, const BlockTab* bt, const uint8_t* kind,
                 const SimConst& sc, int nb, double* partial, double* out, hipStream_t s) {
   hipLaunchKernelGGL(KMass<T>, dim3(nb), dim3(256), 0, s, p0, p1, par, bt, kind, sc, partial);
-  hipLaunchKernelGGL(KSum1, dim3(1), dim3(1), 0, s, partial, nb, out);
+  hipLaunchKernelGGL(KSum1, dim3(1), dim3(256), 0, s, partial, nb, out);
 }
 
 // Density and velocity per core cell for output (CellProxy::computeRhoU,
@@ -1660,7 +1819,7 @@ __global__ __launch_bounds__(256) void KMacro(const T* pops0, const T* pops1, Pa
     T r, v[3];
     ProxyRhoU(f, r, v);
     const size_t o = static_cast<size_t>(b) * sc.ncore + t;
-    rho[o] = static_cast<double>(r);
+    rho[o] = RhoDouble(r);
     for (int d = 0; d < 3; ++d) u[3 * o + d] = static_cast<double>(v[d]);
   }
 }

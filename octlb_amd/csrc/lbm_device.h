@@ -114,21 +114,33 @@ enum : int {
   kFlagDrag = 4,       // block has drag-snapshot links
   kFlagPlain = 8,      // every cell streams by the plain pull (no per-cell rule applies)
   kFlagProlong = 16,   // block holds fine-side targets of a prolongation
+  kFlagPush = 32,      // push-class: updated by the push body (KBlock), see push_to
+  kFlagPushOut = 64,   // pull-class block with push-class blocks pulling from it
 };
 
+// Halo parts of a block: faces 0..5 (x-, x+, y-, y+, z-, z+), then the 12
+// edges 6 + pair * 4 + s1 * 2 + s2 for the axis pairs (x,y), (x,z), (y,z)
+// with s1, s2 the side (0 low, 1 high) on each axis.
 struct BlockTab {
   int level;
   int shell;    // shell index or -1
   int flags;
   int member;   // ensemble member whose constants (SimConst) the block uses
-  int nbr[18];  // faces 0..5, edges 6..17; -1 = halo part not exchanged
+  int nbr[18];  // block whose cells this block's halo part holds; -1 = not exchanged
+  // push_to[s]: the push-class block whose halo part opposite to s is this
+  // block (it pulls this block's cells that stream across side s), or -1
+  int push_to[18];
   double origin[3];
   double cw;
 };
 
+// blocked mode: block = the tile's block; ox = second block of a push tile
+// (-1: none), oy = 1 for a push tile (two push-class blocks, one per half
+// of the workgroup), 0 for a pull tile.  padded mode: ox, oy, oz = origin
+// of the tile in the padded box.
 struct Tile {
   int block;
-  int ox, oy, oz;  // blocked: unused (tile = block); padded: padded-box origin
+  int ox, oy, oz;
 };
 
 struct ParArray {
@@ -185,7 +197,8 @@ struct TileArgs {
   const uint32_t* dmask;
   const int* doff;
   T* snap;
-  double* stats;          // [ntiles*2] partial sums (want_stats == 1)
+  double* stats;          // [blocks*2] partial sums per block of the level (want_stats == 1)
+  const int* block_rank;  // block -> its index among its level's blocks in visiting order
   double* stat_val;       // per-cell statistic in reference order (want_stats == 2)
   uint8_t* stat_use;
   int tiles_per_block;
@@ -362,7 +375,11 @@ __device__ __forceinline__ void CollideCell(T* f, int k, const SimConst& sc, con
       T rho, u[3];
       ProxyRhoU(f, rho, u);
       if (want) {
-        ssum += static_cast<double>(rho + (T(1) - avg));
+        if constexpr (kShiftedArith<T>) {
+          ssum += 1.0 + (static_cast<double>(rho) - static_cast<double>(avg));  // rho + (1 - avg)
+        } else {
+          ssum += static_cast<double>(rho + (T(1) - avg));
+        }
         scnt += 1;
       }
       ConstRhoBgk(f, omega, avg, rho, u);
@@ -402,7 +419,7 @@ __device__ __forceinline__ void CollideCell(T* f, int k, const SimConst& sc, con
     }
     OlbBgk(f, rho, u, omega);
     if (want) {
-      ssum += static_cast<double>(rho);
+      ssum += RhoDouble(rho);
       scnt += 1;
     }
   }

@@ -135,6 +135,33 @@ class GpuSolver:
     def advance_graph(self, steps=1):
         abi.check(self._lib.octlb_dev_advance_graph(self._h, int(steps)), "advance_graph")
 
+    def set_snapshot(self, on=True):
+        """TimeLoop::set_snapshot_post_collide: snapshot the drag links'
+        post-collide populations in every following step."""
+        abi.check(self._lib.octlb_dev_set_snapshot(self._h, 1 if on else 0), "set_snapshot")
+
+    def set_kernel_variant(self, variant):
+        """0: every block pulls through its LDS box; 1: plain interior blocks
+        push, the rest pull (default)."""
+        abi.check(self._lib.octlb_dev_set_kernel_variant(self._h, int(variant)), "set_kernel_variant")
+
+    def kernel_variant(self):
+        """(variant in use, blocks updated by push)."""
+        v, n = C.c_int32(), C.c_int32()
+        abi.check(self._lib.octlb_dev_kernel_variant(self._h, C.byref(v), C.byref(n)), "kernel_variant")
+        return int(v.value), int(n.value)
+
+    def autotune(self, reps=20):
+        """Pick the faster kernel variant on the current state; returns
+        (chosen variant, per-variant fused-kernel ms per coarse step)."""
+        ms = np.zeros(2)
+        abi.check(self._lib.octlb_dev_autotune(self._h, int(reps), abi.f64p(ms)), "autotune")
+        return self.kernel_variant()[0], ms
+
+    def prepare_graphs(self):
+        """Capture every graph advance_graph replays, without running it."""
+        abi.check(self._lib.octlb_dev_prepare_graphs(self._h), "prepare_graphs")
+
     def synchronize(self):
         abi.check(self._lib.octlb_dev_synchronize(self._h), "synchronize")
 

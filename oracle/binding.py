@@ -232,14 +232,7 @@ def build_links(bounds, n, material, triangles):
 
 def setup(sim, case, precision_note=None):
     """Apply a ``octlb_amd.cases.Case`` to a GpuSolver or OracleSim."""
-    sim.set_bc_kinds(case.kinds)
-    if case.link.shape[0] > 0:
-        sim.set_links(case.link, case.q)
-    if case.init_field is not None:
-        sim.init_field(*case.init_field)
-    else:
-        sim.init_equilibrium(case.init_rho0, case.init_u0, case.init_mask)
-    return sim
+    return case.apply(sim)
 
 
 # ------------------------------------------------------------ reference harness

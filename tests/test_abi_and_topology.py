@@ -172,3 +172,27 @@ def test_coarse_fine_faces_sorted_by_coarse_level():
     topo = f.topology(8, 1)
     levels = [topo.coarse_fine[k].coarse_level for k in range(topo.num_cf)]
     assert levels == sorted(levels) and len(set(levels)) >= 2
+
+
+@pytest.mark.parametrize("field,value,needle", [
+    ("normal", 7, b"normal"),
+    ("normal", -1, b"normal"),
+    ("coarse_level", 0, b"levels"),
+    ("fine_level", 3, b"levels"),
+    ("coarse_level", 99, b"levels"),
+])
+def test_create_rejects_inconsistent_coarse_fine_faces(field, value, needle):
+    # a face whose normal or levels disagree with the block table is refused
+    # before any device work (no out-of-range plan rows, no wrong parities)
+    import ctypes as C
+    from octlb_amd import cases
+
+    case = cases.cylinder_channel(bricks_x=2, base_level=1, fine=True, xc=0.5, radius=0.1,
+                                  refine_lo=0.25, refine_hi=1.0)
+    topo = case.topo
+    assert topo.num_cf > 0
+    setattr(topo.coarse_fine[0], field, value)
+    h = C.c_void_p()
+    rc = abi.lib().octlb_dev_create(C.byref(topo.struct()), C.byref(case.params(64)), C.byref(h))
+    assert rc != 0 and not h.value
+    assert needle in abi.lib().octlb_last_error()

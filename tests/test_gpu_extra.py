@@ -115,3 +115,28 @@ def test_time_kernel_leaves_state_unchanged():
     g2.advance(5)
     np.testing.assert_array_equal(g1.download_core(), g2.download_core())
     np.testing.assert_array_equal(g1.drag(case.drag_bbox), g2.drag(case.drag_bbox))
+
+
+def test_snapshot_switch_matches_create_time_flag():
+    # octlb_dev_set_snapshot (TimeLoop::set_snapshot_post_collide) on a
+    # handle created without the snapshot gives the drag of a handle created
+    # with it; switching it off and on again in mid-run as well
+    import dataclasses
+    case = _two_level()
+    assert case.snapshot == 1
+    a = make_gpu(case, 64)
+    b = make_gpu(dataclasses.replace(case, snapshot=0), 64)
+    b.set_snapshot(True)
+    a.advance(3)
+    b.advance(3)
+    np.testing.assert_array_equal(b.drag(case.drag_bbox), a.drag(case.drag_bbox))
+    b.set_snapshot(False)
+    a.advance_graph(2)
+    b.advance_graph(2)
+    b.set_snapshot(True)
+    a.advance_graph(17)
+    b.advance_graph(17)
+    np.testing.assert_array_equal(b.download_core(), a.download_core())
+    np.testing.assert_array_equal(b.drag(case.drag_bbox), a.drag(case.drag_bbox))
+    a.close()
+    b.close()

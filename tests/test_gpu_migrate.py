@@ -106,3 +106,61 @@ def test_identity_and_mismatch():
         other.migrate_from(src)
     for s in (src, dst, other):
         s.close()
+
+
+def test_migrated_refined_state_steps_like_the_oracle():
+    # the first steps after a migration onto the refined forest: the GPU and
+    # the oracle, both started from the migrated halo-inclusive views,
+    # agree to 1e-12 (new blocks' halo slots hold defined values)
+    from octlb_amd import abi
+    from oracle.binding import OracleSim
+    ta, tb = _topo(False), _topo(True)
+    src = _solver(ta)
+    nc = ta.num_blocks * 8 ** 3
+    rng = np.random.default_rng(5)
+    src.init_field(1.0 + 0.01 * rng.random(nc), 0.02 * (rng.random(nc * 3) - 0.5))
+    src.advance(3)
+    dst = _solver(tb)
+    dst.migrate_from(src)
+    ora = OracleSim(tb, abi.make_params(1.0 / TAU, precision=64))
+    ora.set_bc_kinds(np.zeros((tb.num_blocks, 8, 8, 8), np.uint8))
+    for b in range(tb.num_blocks):
+        ora.upload_block(b, dst.download_block(b))
+    dst.advance(4)
+    ora.advance(4)
+    g, o = dst.download_core(), ora.download_core()
+    assert np.abs(g - o).max() <= 1e-12 * np.abs(o).max()
+    for s in (src, dst, ora):
+        s.close()
+
+
+def test_migrate_carries_step_and_const_rho_average():
+    # a ramped inlet and ConstRhoBGK: the step counter and the lagged average
+    # move with the populations, so the migrated handle continues bit for bit
+    import dataclasses
+    case = cases.cylinder_channel(bricks_x=2, base_level=1, fine=True, xc=0.5, radius=0.1,
+                                  refine_lo=0.25, refine_hi=1.0, ramp_end_t=40.0)
+    case = dataclasses.replace(case, const_rho=1)
+    src = case.apply(GpuSolver(case.topo, case.params(64)))
+    src.advance(5)
+    dst = case.apply(GpuSolver(case.topo, case.params(64)))
+    dst.migrate_from(src)
+    assert dst.average_rho() == src.average_rho()
+    src.advance(6)
+    dst.advance(6)
+    np.testing.assert_array_equal(dst.download_core(), src.download_core())
+    np.testing.assert_array_equal(dst.drag(case.drag_bbox), src.drag(case.drag_bbox))
+    assert dst.average_rho() == src.average_rho()
+    for s in (src, dst):
+        s.close()
+
+
+def test_migrate_rejects_ensembles():
+    from octlb_amd import abi
+    ens = cases.cavity_ensemble(count=2, sizes=(32,), seed=1)
+    a = GpuSolver.ensemble(ens, 64)
+    b = GpuSolver.ensemble(ens, 64)
+    with pytest.raises(abi.OctlbError, match="ensemble"):
+        b.migrate_from(a)
+    for s in (a, b):
+        s.close()
