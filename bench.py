@@ -68,6 +68,7 @@ def run_gpu(case, precision, steps, warmup, timing_steps):
     sim = GpuSolver(case.topo, case.params(precision))
     case.apply(sim)
     actual, feq = sim.update_counts()
+    variant, tuned_ms = sim.autotune(reps=10)  # update scheme of the fused kernel, state unchanged
     sim.prepare_graphs()  # every graph the timed region replays exists before the timer
     sim.advance_graph(max(warmup, 1))
     sim.synchronize()
@@ -93,6 +94,9 @@ def run_gpu(case, precision, steps, warmup, timing_steps):
         "mass": mass,
         "actual_per_step": actual,
         "feq_per_step": feq,
+        "autotune": {"variant": {0: "pull", 1: "push where plain"}[variant], "push_blocks": sim.kernel_variant()[1],
+                     "fused_ms_per_coarse_step": {"pull": round(float(tuned_ms[0]), 4),
+                                                  "push where plain": round(float(tuned_ms[1]), 4)}},
     }
     sim.close()
     return res
@@ -118,6 +122,7 @@ def run_ensemble(precs, steps, warmup, timing_steps, count=64):
         sim = GpuSolver.ensemble(ens, p)
         sim.set_bc_kinds(ens.kinds)
         sim.init_equilibrium(1.0, (0.0, 0.0, 0.0), 0xFF)
+        variant, tuned_ms = sim.autotune(reps=5)
         sim.prepare_graphs()
         sim.advance_graph(max(warmup, 1))
         sim.synchronize()
@@ -129,6 +134,8 @@ def run_ensemble(precs, steps, warmup, timing_steps, count=64):
         ach = BYTES_PER_UPDATE[p] * upd / (ms * 1e-3) / 1e9
         out[f"fp{p}"] = {"mlups": round(cells * steps / dt / 1e6, 3), "ms_per_step": round(1e3 * dt / steps, 6),
                          "steps": steps,
+                         "autotune": {"variant": {0: "pull", 1: "push where plain"}[variant],
+                                      "fused_ms": [round(float(x), 4) for x in tuned_ms]},
                          "roofline": {"bound": "hbm", "achieved": round(ach, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                                       "frac": round(ach / HBM_PEAK_GBS, 4), "kernel": "fused collide+stream (KBlock)",
                                       "cells_per_launch": upd, "avg_launch_us": round(1e3 * ms, 3)}}
@@ -402,6 +409,7 @@ def main():
                          "level 2, 2:1 balanced), divergence-free Fourier initial velocity |k| <= 4, "
                          "max |u| 0.01, bounce-back walls, tau 0.6"),
             "blocks_per_level": [int((lv == l).sum()) for l in range(int(lv.max()) + 1)],
+            "autotune": {f"fp{p}": sres[p]["autotune"] for p in sres},
             "fine_equivalent_updates_per_step": sh["feq_per_step"],
             "actual_updates_per_step": sh["actual_per_step"],
             "fp32_mlups": round(sres[32]["mlups_feq"], 3) if 32 in sres else None,
@@ -463,6 +471,7 @@ def main():
             "actual_updates_per_step": r["actual_per_step"],
             "fine_equivalent_updates_per_step": r["feq_per_step"],
             "timed_region": "hipGraph replay of whole coarse steps",
+            "autotune": r["autotune"],
         },
         "mlups_actual": round(r["mlups_actual"], 3),
         "fp32_mlups": round(res[32]["mlups_feq"], 3) if 32 in res else None,
