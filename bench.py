@@ -145,6 +145,53 @@ def run_ensemble(precs, steps, warmup, timing_steps, count=64):
     return out
 
 
+def run_adjoint_channel(steps, checkpoint):
+    """Discrete adjoint of the benchmarked 2-level cylinder channel itself
+    (octlb_adj_run, multi-level engine: Lagrava coupler, FD velocity inlet
+    and pressure outlet, Bouzidi links).  J = mean x-drag over the second
+    half of `steps` coarse steps from rest; gradients w.r.t. the inlet
+    velocity and every link q; checkpoint every `checkpoint` coarse steps.
+    The ratio divides the adjoint run's device time by the same number of
+    production FP64 coarse steps (hipGraph replay).  The FP32-forward mode
+    (FP32 states, FP64 adjoint) is reported beside it with its gradient
+    error against the FP64 mode."""
+    from octlb_amd.adjoint import DragAdjoint
+    from octlb_amd.solver import GpuSolver
+
+    case = build_case("cylinder")
+    win = (steps // 2, steps)
+    adj = DragAdjoint(case, steps, win, checkpoint=checkpoint)
+    J, _, dq = adj.evaluate()
+    fw, bw = adj.timing
+    gi = adj.inlet_grad.copy()
+    adj32 = DragAdjoint(case, steps, win, checkpoint=checkpoint, precision=32)
+    J32, _, dq32 = adj32.evaluate()
+    fw32, bw32 = adj32.timing
+    gi32 = adj32.inlet_grad.copy()
+    sim = case.apply(GpuSolver(case.topo, case.params(64)))
+    sim.autotune(reps=5)
+    sim.prepare_graphs()
+    sim.advance_graph(16)
+    sim.synchronize()
+    t0 = time.perf_counter()
+    sim.advance_graph(steps)
+    sim.synchronize()
+    fwd_ms = 1e3 * (time.perf_counter() - t0) / steps
+    sim.close()
+    return {"workload": "the benchmarked 2-level cylinder channel (1,152 blocks), from rest, inlet u = 0.04",
+            "coarse_steps": steps, "window": list(win), "checkpoint": checkpoint, "links": int(len(case.q)),
+            "J": J, "dJ_du_inlet": [float(v) for v in gi], "max_abs_dJ_dq": float(np.abs(dq).max()),
+            "adjoint_ms": round(fw + bw, 3), "forward_sweep_ms": round(fw, 3), "reverse_sweep_ms": round(bw, 3),
+            "forward_fp64_ms_per_coarse_step": round(fwd_ms, 5),
+            "adjoint_over_forward_fp64": round((fw + bw) / (steps * fwd_ms), 3),
+            "fp32_forward": {"adjoint_ms": round(fw32 + bw32, 3),
+                             "over_forward_fp64": round((fw32 + bw32) / (steps * fwd_ms), 3),
+                             "rel_err_vs_fp64_adjoint": {
+                                 "J": abs(J32 - J) / abs(J),
+                                 "dJ_du_inlet": abs(gi32[0] - gi[0]) / abs(gi[0]),
+                                 "dJ_dq_max": float(np.abs(dq32 - dq).max() / np.abs(dq).max())}}}
+
+
 def run_adjoint(steps, checkpoint):
     """Discrete adjoint of the single-level step (octlb_adj_run) on the
     cylinder at the fine resolution of the 2-level channel, as one level:
@@ -351,6 +398,8 @@ def main():
     ap.add_argument("--adjoint-steps", type=int, default=4000,
                     help="adjoint run length; J averages the drag over its second half")
     ap.add_argument("--adjoint-checkpoint", type=int, default=100)
+    ap.add_argument("--channel-adjoint-steps", type=int, default=4000,
+                    help="coarse steps of the 2-level channel's adjoint run (J over the second half)")
     ap.add_argument("--precision", default="both", choices=["both", "32", "64"])
     ap.add_argument("--calibrate-copy-mb", type=int, default=0,
                     help="run one device-to-device copy of this many MiB first (profiler "
@@ -432,10 +481,14 @@ def main():
         log("bouzidi link build")
         links = run_link_build(not args.no_cpu)
         log(f"  {links['links']} links from {links['triangles']} triangles in {links['gpu_ms']} ms")
-    adjo = None
+    adjo = adjc = None
     if not args.no_adjoint and args.workload == "cylinder":
         log("adjoint")
         adjo = run_adjoint(args.adjoint_steps, args.adjoint_checkpoint)
+        log("adjoint of the 2-level channel")
+        adjc = run_adjoint_channel(args.channel_adjoint_steps, args.adjoint_checkpoint)
+        log(f"  {adjc['coarse_steps']} coarse steps: adjoint {adjc['adjoint_ms']} ms, ratio "
+            f"{adjc['adjoint_over_forward_fp64']} (FP32 forward {adjc['fp32_forward']['over_forward_fp64']})")
         log(f"  FP32 forward / FP64 adjoint: {adjo['fp32_forward']['adjoint_ms']} ms -> "
             f"{adjo['fp32_forward']['over_forward_fp64']}x FP64 forward, errors {adjo['fp32_forward']['rel_err_vs_fp64_adjoint']}")
         log(f"  {adjo['steps']} steps: adjoint {adjo['adjoint_ms']} ms (forward sweep {adjo['forward_sweep_ms']}), "
@@ -478,7 +531,7 @@ def main():
         "fp64_mlups": round(res[64]["mlups_feq"], 3) if 64 in res else None,
         "fp32_over_fp64": (round(res[32]["mlups_feq"] / res[64]["mlups_feq"], 3)
                            if 32 in res and 64 in res else None),
-        "adjoint_over_forward": adjo["adjoint_over_forward_fp64"] if adjo else None,
+        "adjoint_over_forward": adjc["adjoint_over_forward_fp64"] if adjc else None,
         "roofline": {
             "bound": "hbm",
             "achieved": round(achieved, 2),
@@ -497,7 +550,8 @@ def main():
         "synthetic_amr": synth,
         "ensemble": ens,
         "link_build": links,
-        "adjoint": adjo,
+        "adjoint": adjc,
+        "adjoint_single_level": adjo,
         "drag_fp32": res[32]["drag"] if 32 in res else None,
         "drag_fp64": res[64]["drag"] if 64 in res else None,
     }

@@ -343,11 +343,13 @@ int octlb_vtk_append_pvd(const char* pvd_path, int32_t iT, const char* vtm_relat
 /* ------------------------------------------------------------------ */
 /* Discrete adjoint (no counterpart in the reference, which has none): */
 /* gradients of a time-averaged drag objective through the step that   */
-/* octlb_dev_advance runs, for one level of 8^3 blocks with the        */
-/* concrete handler, plain BGK and Bouzidi streaming, without FD       */
-/* boundary kinds.  The survey's adjoint entry octlb_adj_run(dev, ...)  */
-/* is split into create / run so the transposed stream operator is     */
-/* built once per adapt.                                               */
+/* octlb_dev_advance runs, with the concrete handler, plain BGK and     */
+/* Bouzidi streaming on 8^3 blocks.  One level without FD boundary      */
+/* kinds runs the single-level engine (fused kernels); several levels   */
+/* (Lagrava coupling, level_coupler.cpp) and/or FD inlet / outlet cells */
+/* (interpolated_velocity.h) run the multi-level engine.  The survey's  */
+/* adjoint entry octlb_adj_run(dev, ...) is split into create / run so  */
+/* the transposed stream operator is built once per adapt.              */
 /* ------------------------------------------------------------------ */
 typedef struct octlb_adj octlb_adj;
 
@@ -356,9 +358,10 @@ typedef struct octlb_adj octlb_adj;
 int octlb_adj_create(octlb_dev* dev, octlb_adj** out);
 int octlb_adj_destroy(octlb_adj* adj);
 /* From feq(rho0, u0) on the core cells whose BcKind is in kind_mask (all
- * other slots zero, step 0), run `steps` updates of the blocks' level (a
- * forest whose blocks all sit on level L updates 2^L times per coarse step
- * of octlb_dev_advance), keeping a checkpoint every
+ * other slots zero, step 0), run `steps` steps -- single-level engine:
+ * updates of the blocks' level (a forest whose blocks all sit on level L
+ * updates 2^L times per coarse step of octlb_dev_advance); multi-level
+ * engine: coarse steps, time t = 0 for the first --, keeping a checkpoint every
  * `checkpoint` steps, and return J = mean over steps win_begin+1 .. win_end
  * of dir . drag(bbox) (octlb_dev_drag's sum), dJ/du0[3] and, for each of the
  * num_links (block, cell, pop) triples in link, dJ/dq of that link's wall
@@ -375,6 +378,12 @@ int octlb_adj_run(octlb_adj* adj, int32_t steps, int32_t win_begin, int32_t win_
 /* Core populations after the last run's forward sweep, in download_core
  * order [block][cell][19] (for checks against octlb_dev_advance). */
 int octlb_adj_final_core(octlb_adj* adj, double* out);
+/* dJ/d(inlet velocity) of the last run: the derivative with respect to
+ * the inlet_u vector of the velocity boundary's inlet field (uniform,
+ * Poiseuille or channel profile, inlet_velocity_field.h:47-270) through the
+ * velocity-Dirichlet collide and the FD pass; zeros for the single-level
+ * engine (no FD cells). */
+int octlb_adj_inlet_grad(const octlb_adj* adj, double out[3]);
 /* Device time of the last run's forward sweep and reverse sweep (ms). */
 int octlb_adj_timing(const octlb_adj* adj, double* forward_ms, double* reverse_ms);
 

@@ -204,6 +204,7 @@ _SIGNATURES = {
     "octlb_adj_run": ([_VP, C.c_int32, C.c_int32, C.c_int32, C.c_int32, _F64P, _F64P, C.c_double, _F64P,
                        C.c_uint32, C.c_int64, _I32P, _F64P, _F64P, _F64P], C.c_int),
     "octlb_adj_final_core": ([_VP, _F64P], C.c_int),
+    "octlb_adj_inlet_grad": ([_VP, _F64P], C.c_int),
     "octlb_adj_timing": ([_VP, _F64P, _F64P], C.c_int),
 }
 

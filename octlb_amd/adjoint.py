@@ -9,8 +9,11 @@ fraction q (bouzidi_pull.h:8-26).  ``DragObjective`` wraps it as a
 torch.autograd.Function so that u0 and q can sit in a torch graph.
 
 The reference has no adjoint; the gradients are defined by the forward this
-library runs (one level, concrete handler, BGK, bounce-back and Bouzidi
-cells).  Everything runs in the HIP library; there is no CPU path.
+library runs (concrete handler, BGK, bounce-back and Bouzidi cells; with
+several levels the Lagrava coupler, with velocity / pressure boundaries the
+FD pass and the velocity-Dirichlet collide, and then also the gradient with
+respect to the inlet velocity).  Everything runs in the HIP library; there is
+no CPU path.
 """
 from __future__ import annotations
 
@@ -34,9 +37,19 @@ class DragAdjoint:
         self.rho0 = float(rho0)
         self.precision = int(precision)  # forward states: 64, or 32 with the adjoint still in FP64
         self.timing = (0.0, 0.0)
+        self.inlet_grad = np.zeros(3)
 
-    def _handles(self, q):
-        sol = GpuSolver(self.case.topo, self.case.params(self.precision))
+    def _params(self, u_in):
+        p = self.case.params(self.precision)
+        if u_in is not None:
+            for k in range(p.num_specs):
+                if p.specs[k].inlet != abi.INLET_NONE:
+                    for d in range(3):
+                        p.specs[k].inlet_u[d] = float(u_in[d])
+        return p
+
+    def _handles(self, q, u_in=None):
+        sol = GpuSolver(self.case.topo, self._params(u_in))
         sol.set_bc_kinds(self.case.kinds)
         if len(self.case.link):
             sol.set_links(self.case.link, q)
@@ -44,14 +57,15 @@ class DragAdjoint:
         abi.check(abi.lib().octlb_adj_create(sol._h, C.byref(h)), "adjoint create")
         return sol, h
 
-    def evaluate(self, u0=None, q=None, final_core: bool = False):
+    def evaluate(self, u0=None, q=None, final_core: bool = False, u_in=None):
         """(J, dJ/du0 [3], dJ/dq [links]) and, if asked, the core populations
-        after the forward sweep."""
+        after the forward sweep.  u_in replaces the inlet velocity vector of
+        the case's inlet field; dJ/du_in of the run is in ``inlet_grad``."""
         lib = abi.lib()
         u0 = np.ascontiguousarray(self.case.init_u0 if u0 is None else u0, dtype=np.float64)
         q = np.ascontiguousarray(self.case.q if q is None else q, dtype=np.float64)
         link = np.ascontiguousarray(self.case.link, dtype=np.int32).reshape(-1, 3)
-        sol, h = self._handles(q)
+        sol, h = self._handles(q, u_in)
         try:
             J = np.zeros(1)
             du = np.zeros(3)
@@ -63,6 +77,9 @@ class DragAdjoint:
                                         abi.f64p(du), abi.f64p(dq)), "adjoint run")
             fw, bw = np.zeros(1), np.zeros(1)
             abi.check(lib.octlb_adj_timing(h, abi.f64p(fw), abi.f64p(bw)), "adjoint timing")
+            gi = np.zeros(3)
+            abi.check(lib.octlb_adj_inlet_grad(h, abi.f64p(gi)), "adjoint inlet gradient")
+            self.inlet_grad = gi
             self.timing = (float(fw[0]), float(bw[0]))
             core = None
             if final_core:

@@ -39,6 +39,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <functional>
 #include <stdexcept>
 #include <string>
 #include <unordered_map>
@@ -47,6 +48,8 @@
 #include "d3q19.h"
 #include "octlb_gpu.h"
 #include "octlb_internal.h"
+#include "adjoint_ml.h"
+#include "adjoint_topo.h"
 
 namespace octlb {
 namespace {
@@ -401,6 +404,260 @@ __global__ __launch_bounds__(kThreads) void KAdjInitGrad(const double* lx, const
 
 inline unsigned Blocks(long long n) { return static_cast<unsigned>((n + kThreads - 1) / kThreads); }
 
+// ------------------------------------------------------------------------
+// Multi-level step (several refinement levels, FD inlet/outlet cells, the
+// Lagrava coupler): the step is the recursion of time_loop.h:201-246 as a
+// list of operations on the live state X (all levels) and the coupler's
+// previous macros P.  A level visit collides the level's cells with the
+// production collide (CollideCell: BGK bulk, bounce-back, the velocity-
+// Dirichlet collide with the prescribed velocity at time t, no-op kinds)
+// into Y and streams them back, X = M_l Y; the FD pass, prolongation and
+// restriction run the production device code on the same layout
+// (adjoint_ml.inc).  Core cell c of block b is entry boff[b] + i cnt[b] + c.
+
+__device__ __forceinline__ Origin CellOrigin(const double* ocw, int b) {
+  const double* o = ocw + 4 * static_cast<size_t>(b);
+  return Origin{o[0], o[1], o[2], o[3]};
+}
+
+// y = C(x) over cells [c0, c1) (halo slots are carried: y = x)
+template <typename S>
+__global__ __launch_bounds__(kThreads) void KMLCollide(const S* x, S* y, const Cell* cells, const uint8_t* ckind,
+                                                       const int* cblock, const int* boff, const double* ocw,
+                                                       SimConst sc, double t, int c0, int c1) {
+  const int cl = c0 + static_cast<int>(blockIdx.x) * kThreads + static_cast<int>(threadIdx.x);
+  if (cl >= c1) return;
+  const Cell c = cells[cl];
+  S f[kQ];
+  LoadCell(x, c, f);
+  const uint8_t k = ckind[cl];
+  if (k != kHalo) {
+    const int b = cblock[cl];
+    const int rank = c.base - boff[b];
+    double ssum = 0.0;
+    int scnt = 0;
+    CollideCell<S>(f, k, sc, CellOrigin(ocw, b), rank >> 6, (rank >> 3) & 7, rank & 7, t, AvgArg<S>(1.0), false,
+                   ssum, scnt);
+  }
+#pragma unroll
+  for (int i = 0; i < kQ; ++i) y[c.base + i * c.stride] = f[i];
+}
+
+// x = M y over the entries of cells [c0, c1)
+template <typename S>
+__global__ __launch_bounds__(kThreads) void KMLGather(const S* y, S* x, const int* src, const AdjLink* links,
+                                                      const Cell* cells, int c0, int c1) {
+  const int cl = c0 + static_cast<int>(blockIdx.x) * kThreads + static_cast<int>(threadIdx.x);
+  if (cl >= c1) return;
+  const Cell c = cells[cl];
+#pragma unroll
+  for (int i = 0; i < kQ; ++i) {
+    const int e = c.base + i * c.stride;
+    x[e] = RowValue<S>(y, src, links, e);
+  }
+}
+
+// Velocity-Dirichlet collide VJP (CollideDirichletBoundaryCellAt,
+// interpolated_velocity.h:858-884): f' = (1 - w) f + w feq(rho, u) with u
+// prescribed and rho = VelocityBoundaryRhoFromPop (flat face cells) or
+// CellProxy rho (edges, corners).  l is updated in place; du receives
+// d/du of the prescribed velocity.
+__device__ inline void VelocityCollideVjp(const double* f, double* l, double omega, const double* u, int n, int ix,
+                                          int iy, int iz, double* du) {
+  const int fc = (ix == 0) + (ix == n - 1) + (iy == 0) + (iy == n - 1) + (iz == 0) + (iz == n - 1);
+  int dir = 0, ori = 0;
+  double rho, drho_df[kQ], drho_du = 0.0;
+  if (fc >= 2) {
+    double r, ut[3];
+    ProxyRhoU(f, r, ut);
+    rho = r;
+    for (int j = 0; j < kQ; ++j) drho_df[j] = 1.0;
+  } else {
+    FlatInfo(n, ix, iy, iz, dir, ori);
+    rho = VelRhoFromPop(dir, ori, f, u);
+    const double den = 1.0 + ori * u[dir];
+    for (int j = 0; j < kQ; ++j) {
+      const int cj = Ci(j, dir);
+      drho_df[j] = (cj == 0 ? 1.0 : (cj == ori ? 2.0 : 0.0)) / den;
+    }
+    drho_du = -ori * rho / den;
+  }
+  const double usqr = NormSqr(u);
+  double A = 0.0, G[3] = {0.0, 0.0, 0.0};
+#pragma unroll
+  for (int i = 0; i < kQ; ++i) {
+    const double cu = Cx(i) * u[0] + Cy(i) * u[1] + Cz(i) * u[2];
+    const double w = W<double>(i);
+    A += l[i] * w * (1.0 + 3.0 * cu + 4.5 * cu * cu - 1.5 * usqr);
+    const double g = l[i] * rho * w;
+    G[0] += g * (3.0 * Cx(i) + 9.0 * cu * Cx(i) - 3.0 * u[0]);
+    G[1] += g * (3.0 * Cy(i) + 9.0 * cu * Cy(i) - 3.0 * u[1]);
+    G[2] += g * (3.0 * Cz(i) + 9.0 * cu * Cz(i) - 3.0 * u[2]);
+  }
+#pragma unroll
+  for (int j = 0; j < kQ; ++j) l[j] = (1.0 - omega) * l[j] + omega * A * drho_df[j];
+  for (int d = 0; d < 3; ++d) du[d] = omega * G[d];
+  if (fc < 2) du[dir] += omega * A * drho_du;
+}
+
+// One level visit reversed over cells [c0, c1): lxn = C^T (M^T lx + snap
+// seeds) at the visit's pre-collide state xp; d/d(inlet velocity) of the
+// velocity cells as one partial per workgroup (fixed tree) in dui.
+template <typename S>
+__global__ __launch_bounds__(kThreads) void KMLVisitVjp(const double* lx, double* lxn, const S* xp, const int* rd,
+                                                        const int* ovo, const int2* ove, const AdjLink* links,
+                                                        const Cell* cells, const uint8_t* ckind, const int* cblock,
+                                                        const int* boff, const double* ocw, SimConst sc, double t,
+                                                        const int* cseed, const double* seed_w, double snap_scale,
+                                                        int c0, int c1, double* dui) {
+  __shared__ double red[3][kThreads];
+  const int cl = c0 + static_cast<int>(blockIdx.x) * kThreads + static_cast<int>(threadIdx.x);
+  double g[3] = {0.0, 0.0, 0.0};
+  if (cl < c1) {
+    const Cell c = cells[cl];
+    double l[kQ];
+#pragma unroll
+    for (int i = 0; i < kQ; ++i) {
+      const int r = rd[c.base + i * c.stride];
+      double v = 0.0;
+      if (r >= 0) {
+        v = lx[r];
+      } else if (r < -1) {
+        const int k = -2 - r;
+        for (int j = ovo[k]; j < ovo[k + 1]; ++j) {
+          const int2 o = ove[j];
+          v += (o.y < 0 ? 1.0 : ReaderCoef(links[o.y >> 1], o.y & 1)) * lx[o.x];
+        }
+      }
+      l[i] = v;
+    }
+    const int sk = cseed[cl];
+    if (sk >= 0 && snap_scale != 0.0) {
+#pragma unroll
+      for (int i = 0; i < kQ; ++i) l[i] += snap_scale * seed_w[static_cast<size_t>(sk) * 2 * kQ + i];
+    }
+    const uint8_t k = ckind[cl];
+    if (k == kBulk) {
+      double f[kQ];
+      LoadCell(xp, c, f);
+      BgkVjp(f, l, sc.omega);
+    } else if (k == kBounceBack) {
+      double tmp[kQ];
+#pragma unroll
+      for (int i = 0; i < kQ; ++i) tmp[i] = l[i];
+#pragma unroll
+      for (int i = 0; i < kQ; ++i) l[i] = tmp[Opp(i)];
+    } else if (k == kVelocity) {
+      double f[kQ];
+      LoadCell(xp, c, f);
+      const int b = cblock[cl];
+      const int rank = c.base - boff[b];
+      const int ix = rank >> 6, iy = (rank >> 3) & 7, iz = rank & 7;
+      const Origin og = CellOrigin(ocw, b);
+      const double org[3] = {og.x, og.y, og.z};
+      double ud[3];
+      PrescribedU(sc, org, og.cw, ix, iy, iz, t, ud);
+      // the forward rounds the prescribed velocity to the state's precision
+      const double u[3] = {static_cast<double>(static_cast<S>(ud[0])), static_cast<double>(static_cast<S>(ud[1])),
+                           static_cast<double>(static_cast<S>(ud[2]))};
+      double du[3];
+      VelocityCollideVjp(f, l, sc.omega, u, kN, ix, iy, iz, du);
+      const double fct = InletFactor(sc, org, og.cw, ix, iy, iz, t);
+      for (int d = 0; d < 3; ++d) g[d] = du[d] * fct;
+    }
+#pragma unroll
+    for (int i = 0; i < kQ; ++i) lxn[c.base + i * c.stride] = l[i];
+  }
+  for (int d = 0; d < 3; ++d) red[d][threadIdx.x] = g[d];
+  __syncthreads();
+  for (int w = kThreads / 2; w > 0; w >>= 1) {
+    if (static_cast<int>(threadIdx.x) < w)
+      for (int d = 0; d < 3; ++d) red[d][threadIdx.x] += red[d][threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0)
+    for (int d = 0; d < 3; ++d) dui[3 * blockIdx.x + d] = red[d][0];
+}
+
+// post-collide value of one population of a live cell (production collide)
+template <typename S>
+__device__ double MLPostCollide(const S* x, const Cell* cells, const uint8_t* ckind, const int* cblock,
+                                const int* boff, const double* ocw, const SimConst& sc, double t, int cell, int pop) {
+  const Cell c = cells[cell];
+  S f[kQ];
+  LoadCell(x, c, f);
+  const uint8_t k = ckind[cell];
+  if (k != kHalo) {
+    const int b = cblock[cell];
+    const int rank = c.base - boff[b];
+    double ssum = 0.0;
+    int scnt = 0;
+    CollideCell<S>(f, k, sc, CellOrigin(ocw, b), rank >> 6, (rank >> 3) & 7, rank & 7, t, AvgArg<S>(1.0), false,
+                   ssum, scnt);
+  }
+  double v = 0.0;
+#pragma unroll
+  for (int i = 0; i < kQ; ++i) v = (i == pop) ? static_cast<double>(f[i]) : v;
+  return v;
+}
+
+// dq += lx[row] d BouzidiPull / dq for links [k0, k1), sources recollided
+// from the visit's pre-collide state
+template <typename S>
+__global__ __launch_bounds__(kThreads) void KMLLinkGrad(const double* lx, const S* xp, const Cell* cells,
+                                                        const uint8_t* ckind, const int* cblock, const int* boff,
+                                                        const double* ocw, SimConst sc, double t,
+                                                        const AdjLink* links, int k0, int k1, double* dq) {
+  const int k = k0 + static_cast<int>(blockIdx.x) * kThreads + static_cast<int>(threadIdx.x);
+  if (k >= k1) return;
+  const AdjLink L = links[k];
+  const double fbb = MLPostCollide(xp, cells, ckind, cblock, boff, ocw, sc, t, L.bb_cell, L.bb_pop);
+  const double fo = MLPostCollide(xp, cells, ckind, cblock, boff, ocw, sc, t, L.ot_cell, L.ot_pop);
+  const double d = L.upper ? -(fbb - fo) / (2.0 * L.q * L.q) : 2.0 * fbb - 2.0 * fo;
+  dq[k] += lx[L.row] * d;
+}
+
+// sum of w (snap + live) over the drag terms after a coarse step: the
+// post-collide snapshot of the drag cells' last visit (y) and the final
+// post-stream state (x); one partial per workgroup
+template <typename S>
+__global__ __launch_bounds__(kThreads) void KMLDrag(const S* y, const S* x, const DragTerm* terms, int nterms,
+                                                    double* out) {
+  __shared__ double red[kThreads];
+  const int t = blockIdx.x * kThreads + threadIdx.x;
+  double acc = 0.0;
+  if (t < nterms) {
+    const DragTerm d = terms[t];
+    acc = d.w * (static_cast<double>(y[d.snap]) + static_cast<double>(x[d.live]));
+  }
+  red[threadIdx.x] = acc;
+  __syncthreads();
+  for (int w = kThreads / 2; w > 0; w >>= 1) {
+    if (static_cast<int>(threadIdx.x) < w) red[threadIdx.x] += red[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) out[blockIdx.x] = red[0];
+}
+
+// acc[0..2] += sum of part[3 j + d] over j in order (one thread)
+__global__ void KMLAccum(const double* part, int n, double* acc) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  double s[3] = {0.0, 0.0, 0.0};
+  for (int j = 0; j < n; ++j)
+    for (int d = 0; d < 3; ++d) s[d] += part[3 * j + d];
+  for (int d = 0; d < 3; ++d) acc[d] += s[d];
+}
+
+// copies the entries of the listed blocks (one workgroup per block)
+template <typename S>
+__global__ __launch_bounds__(256) void KMLCopyBlocks(S* dst, const S* src, const int* blocks, const int* boff,
+                                                     const int* cnt) {
+  const int b = blocks[blockIdx.x];
+  const int n = kQ * cnt[b];
+  for (int e = threadIdx.x; e < n; e += 256) dst[boff[b] + e] = src[boff[b] + e];
+}
+
+
 }  // namespace
 }  // namespace octlb
 
@@ -418,6 +675,9 @@ struct octlb_adj {
   std::vector<int2> nbase;     // [nb][27] (entry base, plane length) of the block at (dx, dy, dz), or -1
   std::vector<double> origin_cw;
   std::unordered_map<int64_t, int> link_index;  // (block*512 + cell)*19 + pop -> Bouzidi row
+  std::vector<int> border;                      // blocks in level order
+  std::vector<int> lv_cell, lv_entry, lv_link;  // per level: [begin, end) of cells, entries, links
+  std::vector<int> block_off, block_cnt;        // per block: first entry, plane length
   hipStream_t stream = nullptr;
   Cell* d_cells = nullptr;
   int* d_cblock = nullptr;
@@ -438,6 +698,34 @@ struct octlb_adj {
   size_t selem = 8;
   std::vector<void*> owned;
   double fwd_ms = 0.0, bwd_ms = 0.0;
+
+  // ---- multi-level step (levels, FD cells, coupling); see KMLCollide
+  enum MLOp { kOpVisit = 0, kOpFd = 1, kOpHalf = 2, kOpFull = 3, kOpRestrict = 4 };
+  bool ml = false;
+  int nlev = 1, nfd = 0, npts = 0;
+  AdjMLTopo mtopo;
+  std::vector<int2> ops;  // (op, level): level visited, coarse level coupled, fine level restricted
+  int* d_boff = nullptr;
+  int* d_cnt = nullptr;
+  double* d_ocw = nullptr;
+  uint8_t* d_kind_blk = nullptr;  // [nb][512] BcKind bytes (production layout)
+  BlockTab* d_bt = nullptr;
+  SimConst* d_scm = nullptr;
+  int* d_fd_blocks = nullptr;
+  short* d_fd_slot = nullptr;
+  short* d_fd_list = nullptr;
+  int4* d_fd_count = nullptr;
+  int* d_plan = nullptr;
+  std::vector<int*> d_plist, d_ppts, d_pbeg, d_rlist, d_rebeg, d_recell;
+  std::vector<int2*> d_rent;
+  std::vector<int> np_l, npg_l, nr_l, nrc_l;
+  int* d_ipts = nullptr;
+  int* d_ibeg = nullptr;
+  int nig = 0;
+  double* d_dui = nullptr;       // d J / d(inlet velocity), accumulated
+  double* d_dui_part = nullptr;  // per-workgroup partials of one kernel
+  double* d_rbuf = nullptr;      // restriction: adjoint of the restricted coarse cells
+  double dui[3] = {0.0, 0.0, 0.0};
 
   template <typename P>
   P* Alloc(size_t count) {
@@ -477,7 +765,18 @@ struct octlb_adj {
     auto in_core = [](int v) { return v >= 0 && v < kN; };
     std::vector<int> srcf(static_cast<size_t>(nfull));
     std::vector<AdjLink> links;
-    for (int b = 0; b < nb; ++b)
+    // blocks in level order (a stable sort: one level keeps the block order),
+    // so that every level's cells, entries and links are contiguous ranges
+    border.resize(nb);
+    for (int b = 0; b < nb; ++b) border[b] = b;
+    if (!t.level.empty())
+      std::stable_sort(border.begin(), border.end(), [&](int a, int c) { return t.level[a] < t.level[c]; });
+    const int nlev = t.level.empty() ? 1 : 1 + *std::max_element(t.level.begin(), t.level.end());
+    auto lvl = [&](int b) { return t.level.empty() ? 0 : t.level[b]; };
+    lv_link.assign(nlev + 1, 0);
+    for (int bi = 0; bi < nb; ++bi) {
+      const int b = border[bi];
+      lv_link[lvl(b) + 1] = -1;  // filled below
       for (int hx = 0; hx < kNP; ++hx)
         for (int hy = 0; hy < kNP; ++hy)
           for (int hz = 0; hz < kNP; ++hz) {
@@ -541,6 +840,9 @@ struct octlb_adj {
               srcf[e] = resolve(b, (sx * kNP + sy) * kNP + sz, i);
             }
           }
+      lv_link[lvl(b) + 1] = static_cast<int>(links.size());
+    }
+    for (int l = 1; l <= nlev; ++l) lv_link[l] = std::max(lv_link[l], lv_link[l - 1]);
     nlinks = static_cast<int>(links.size());
     // ---- live slots: the core and whatever the core's rows reach
     std::vector<uint8_t> live(static_cast<size_t>(nb) * kNS, 0);
@@ -580,7 +882,12 @@ struct octlb_adj {
     cell_block.clear();
     cell_slot.clear();
     core_cell.assign(static_cast<size_t>(nb) * kNC, -1);
-    for (int b = 0; b < nb; ++b) {
+    lv_cell.assign(nlev + 1, 0);
+    lv_entry.assign(nlev + 1, 0);
+    block_off.assign(nb, 0);
+    block_cnt.assign(nb, 0);
+    for (int bi = 0; bi < nb; ++bi) {
+      const int b = border[bi];
       // core cells first in (x, y, z) order, then the live halo slots
       std::vector<int> order;
       for (int c = 0; c < kNC; ++c)
@@ -607,6 +914,14 @@ struct octlb_adj {
         cell_block.push_back(b);
         cell_slot.push_back(slot);
       }
+      lv_cell[lvl(b) + 1] = static_cast<int>(cells.size());
+      lv_entry[lvl(b) + 1] = total;
+      block_off[b] = boff[b];
+      block_cnt[b] = r;
+    }
+    for (int l = 1; l <= nlev; ++l) {
+      lv_cell[l] = std::max(lv_cell[l], lv_cell[l - 1]);
+      lv_entry[l] = std::max(lv_entry[l], lv_entry[l - 1]);
     }
     ntot = total;
     ncells = static_cast<int>(cells.size());
@@ -786,8 +1101,411 @@ struct octlb_adj {
     }
   }
 
+  // ---- multi-level engine: tables on top of Build's (call after Build)
+  template <typename P>
+  P* Upload(const std::vector<P>& v) {
+    P* d = Alloc<P>(v.size());
+    if (!v.empty()) ADJ_CHECK(hipMemcpy(d, v.data(), v.size() * sizeof(P), hipMemcpyHostToDevice));
+    return d;
+  }
+
+  void BuildML(const AdjTopo& t, const AdjMLTopo& m) {
+    ml = true;
+    mtopo = m;
+    nlev = m.max_level + 1;
+    d_boff = Upload(block_off);
+    d_cnt = Upload(block_cnt);
+    d_ocw = Upload(origin_cw);
+    d_kind_blk = Upload(t.kinds);
+    d_bt = Upload(m.bt);
+    d_scm = Upload(std::vector<SimConst>{m.sc});
+    nfd = static_cast<int>(m.fd_blocks.size());
+    d_fd_blocks = Upload(m.fd_blocks);
+    d_fd_slot = Upload(m.fd_slot);
+    d_fd_list = Upload(m.fd_list);
+    d_fd_count = Upload(m.fd_count);
+    npts = static_cast<int>(m.plan.size() / 12);
+    d_plan = Upload(m.plan);
+    // prolongation: per coarse level, the active points and their grouping by
+    // coarse cell (first appearance order; points of a group in list order)
+    auto cell_key = [](int b, int c) { return static_cast<int64_t>(b) * kNC + c; };
+    d_plist.assign(nlev, nullptr);
+    d_ppts.assign(nlev, nullptr);
+    d_pbeg.assign(nlev, nullptr);
+    np_l.assign(nlev, 0);
+    npg_l.assign(nlev, 0);
+    auto group = [&](const std::vector<int>& list, std::vector<int>* pts, std::vector<int>* beg) {
+      std::unordered_map<int64_t, int> gi;
+      std::vector<std::vector<int>> gs;
+      for (int p : list) {
+        const int* r = &m.plan[12 * static_cast<size_t>(p)];
+        const int64_t key = cell_key(r[0], (r[1] * kN + r[2]) * kN + r[3]);
+        auto it = gi.find(key);
+        if (it == gi.end()) {
+          it = gi.emplace(key, static_cast<int>(gs.size())).first;
+          gs.emplace_back();
+        }
+        gs[it->second].push_back(p);
+      }
+      pts->clear();
+      beg->assign(1, 0);
+      for (const auto& g : gs) {
+        pts->insert(pts->end(), g.begin(), g.end());
+        beg->push_back(static_cast<int>(pts->size()));
+      }
+      return static_cast<int>(gs.size());
+    };
+    for (int l = 0; l + 1 < nlev; ++l) {
+      const std::vector<int>& list = m.prolong_active[l];
+      np_l[l] = static_cast<int>(list.size());
+      d_plist[l] = Upload(list);
+      std::vector<int> pts, beg;
+      npg_l[l] = group(list, &pts, &beg);
+      d_ppts[l] = Upload(pts);
+      d_pbeg[l] = Upload(beg);
+    }
+    {
+      std::vector<int> all(npts), pts, beg;
+      for (int p = 0; p < npts; ++p) all[p] = p;
+      nig = group(all, &pts, &beg);
+      d_ipts = Upload(pts);
+      d_ibeg = Upload(beg);
+    }
+    // restriction: per fine level, the active points and, per fine cell they
+    // read, the (point, neighbour slot) entries in point order
+    d_rlist.assign(nlev, nullptr);
+    d_rent.assign(nlev, nullptr);
+    d_rebeg.assign(nlev, nullptr);
+    d_recell.assign(nlev, nullptr);
+    nr_l.assign(nlev, 0);
+    nrc_l.assign(nlev, 0);
+    size_t rmax = 1;
+    for (int l = 1; l < nlev; ++l) {
+      const std::vector<int>& list = m.restrict_active[l];
+      nr_l[l] = static_cast<int>(list.size());
+      rmax = std::max(rmax, list.size());
+      d_rlist[l] = Upload(list);
+      std::unordered_map<int64_t, int> gi;
+      std::vector<std::vector<int2>> gs;
+      std::vector<int> cells_bc;
+      for (int j = 0; j < static_cast<int>(list.size()); ++j) {
+        const int* r = &m.plan[12 * static_cast<size_t>(list[j])];
+        for (int jp = 0; jp < kQ; ++jp) {
+          const int x = r[5] + Cx(jp), y = r[6] + Cy(jp), z = r[7] + Cz(jp);
+          if (x < 0 || x >= kN || y < 0 || y >= kN || z < 0 || z >= kN) continue;
+          const int c = (x * kN + y) * kN + z;
+          const int64_t key = cell_key(r[4], c);
+          auto it = gi.find(key);
+          if (it == gi.end()) {
+            it = gi.emplace(key, static_cast<int>(gs.size())).first;
+            gs.emplace_back();
+            cells_bc.push_back(r[4]);
+            cells_bc.push_back(c);
+          }
+          gs[it->second].push_back(make_int2(j, jp));
+        }
+      }
+      std::vector<int2> ent;
+      std::vector<int> beg(1, 0);
+      for (const auto& g : gs) {
+        ent.insert(ent.end(), g.begin(), g.end());
+        beg.push_back(static_cast<int>(ent.size()));
+      }
+      nrc_l[l] = static_cast<int>(gs.size());
+      d_rent[l] = Upload(ent);
+      d_rebeg[l] = Upload(beg);
+      d_recell[l] = Upload(cells_bc);
+    }
+    d_rbuf = Alloc<double>(rmax * kQ);
+    d_dui = Alloc<double>(3);
+    d_dui_part = Alloc<double>(3 * static_cast<size_t>(std::max<long long>(Blocks(ncells), std::max(nfd, 1))));
+    // the coarse step as operations (TimeLoop::advance, time_loop.h:201-240)
+    ops.clear();
+    std::function<void(int)> adv = [&](int l) {
+      ops.push_back(make_int2(kOpVisit, l));
+      if (nfd > 0) ops.push_back(make_int2(kOpFd, l));
+      if (l + 1 >= nlev) return;
+      ops.push_back(make_int2(kOpHalf, l));
+      adv(l + 1);
+      ops.push_back(make_int2(kOpFull, l));
+      adv(l + 1);
+      ops.push_back(make_int2(kOpRestrict, l + 1));
+    };
+    adv(0);
+  }
+
+  template <typename S>
+  FdArgs<S> MLFdArgs() const {
+    FdArgs<S> a{};
+    a.kind = d_kind_blk;
+    a.bt = d_bt;
+    a.scm = (decltype(a.scm))d_scm;
+    a.fd_list = d_fd_list;
+    a.fd_count = d_fd_count;
+    a.sc = mtopo.sc;
+    return a;
+  }
+
+  // one operation forward on (x, y, prev) at time t; when tape is given, the
+  // state the operation's adjoint needs is copied there first
+  template <typename S>
+  void MLApply(int2 op, S* x, S* y, S* prev, double t, S* tape, S* ptape) {
+    const size_t pm = static_cast<size_t>(npts) * kMacro;
+    const int l = op.y;
+    switch (op.x) {
+      case kOpVisit: {
+        const int c0 = lv_cell[l], c1 = lv_cell[l + 1];
+        if (c1 <= c0) break;
+        if (tape)
+          ADJ_CHECK(hipMemcpyAsync(tape + lv_entry[l], x + lv_entry[l],
+                                   static_cast<size_t>(lv_entry[l + 1] - lv_entry[l]) * sizeof(S),
+                                   hipMemcpyDeviceToDevice, stream));
+        hipLaunchKernelGGL(KMLCollide<S>, dim3(Blocks(c1 - c0)), dim3(kThreads), 0, stream, x, y, d_cells, d_ckind,
+                           d_cblock, d_boff, d_ocw, mtopo.sc, t, c0, c1);
+        hipLaunchKernelGGL(KMLGather<S>, dim3(Blocks(c1 - c0)), dim3(kThreads), 0, stream, y, x, d_src, d_links,
+                           d_cells, c0, c1);
+        break;
+      }
+      case kOpFd:
+        if (tape) hipLaunchKernelGGL(KMLCopyBlocks<S>, dim3(nfd), dim3(256), 0, stream, tape, x, d_fd_blocks, d_boff, d_cnt);
+        LaunchMLFd<S>(MLFdArgs<S>(), d_fd_blocks, d_fd_slot, nfd, x, d_boff, d_cnt, t, stream);
+        break;
+      case kOpHalf:
+      case kOpFull:
+        if (tape) {
+          ADJ_CHECK(hipMemcpyAsync(tape + lv_entry[l], x + lv_entry[l],
+                                   static_cast<size_t>(lv_entry[l + 1] - lv_entry[l]) * sizeof(S),
+                                   hipMemcpyDeviceToDevice, stream));
+          if (pm) ADJ_CHECK(hipMemcpyAsync(ptape, prev, pm * sizeof(S), hipMemcpyDeviceToDevice, stream));
+        }
+        LaunchMLProlong<S>(d_plan, d_plist[l], np_l[l], op.x == kOpHalf ? 1 : 0, x, d_boff, d_cnt, prev,
+                           mtopo.prolong_scale, stream);
+        break;
+      case kOpRestrict:
+        if (tape)
+          ADJ_CHECK(hipMemcpyAsync(tape + lv_entry[l], x + lv_entry[l],
+                                   static_cast<size_t>(lv_entry[l + 1] - lv_entry[l]) * sizeof(S),
+                                   hipMemcpyDeviceToDevice, stream));
+        LaunchMLRestrict<S>(d_plan, d_rlist[l], nr_l[l], x, d_boff, d_cnt, mtopo.restrict_scale, stream);
+        break;
+    }
+    ADJ_CHECK(hipGetLastError());
+  }
+
+  // the adjoint of operation op (its tape as MLApply left it)
+  template <typename S>
+  void MLReverse(int2 op, double t, const S* tape, const S* ptape, double* lp, double snap_scale,
+                 const int* cseed, const double* seed_w) {
+    const int l = op.y;
+    switch (op.x) {
+      case kOpVisit: {
+        const int c0 = lv_cell[l], c1 = lv_cell[l + 1];
+        if (c1 <= c0) break;
+        const int k0 = lv_link[l], k1 = lv_link[l + 1];
+        if (k1 > k0)
+          hipLaunchKernelGGL(KMLLinkGrad<S>, dim3(Blocks(k1 - k0)), dim3(kThreads), 0, stream, d_lx, tape, d_cells,
+                             d_ckind, d_cblock, d_boff, d_ocw, mtopo.sc, t, d_links, k0, k1, d_dq);
+        const unsigned nwg = Blocks(c1 - c0);
+        hipLaunchKernelGGL(KMLVisitVjp<S>, dim3(nwg), dim3(kThreads), 0, stream, d_lx, d_ln, tape, d_rd, d_ovo,
+                           d_ove, d_links, d_cells, d_ckind, d_cblock, d_boff, d_ocw, mtopo.sc, t, cseed, seed_w,
+                           snap_scale, c0, c1, d_dui_part);
+        ADJ_CHECK(hipMemcpyAsync(d_lx + lv_entry[l], d_ln + lv_entry[l],
+                                 static_cast<size_t>(lv_entry[l + 1] - lv_entry[l]) * sizeof(double),
+                                 hipMemcpyDeviceToDevice, stream));
+        hipLaunchKernelGGL(KMLAccum, dim3(1), dim3(1), 0, stream, d_dui_part, static_cast<int>(nwg), d_dui);
+        break;
+      }
+      case kOpFd:
+        LaunchMLFdVjp<S>(MLFdArgs<S>(), d_fd_blocks, d_fd_slot, nfd, tape, d_lx, d_boff, d_cnt, t, d_dui_part,
+                         stream);
+        hipLaunchKernelGGL(KMLAccum, dim3(1), dim3(1), 0, stream, d_dui_part, nfd, d_dui);
+        break;
+      case kOpHalf:
+      case kOpFull:
+        LaunchMLProlongVjp<S>(d_plan, d_ppts[l], d_pbeg[l], npg_l[l], op.x == kOpHalf ? 1 : 0, tape, d_lx, d_boff,
+                              d_cnt, ptape, lp, mtopo.prolong_scale, stream);
+        break;
+      case kOpRestrict:
+        LaunchMLRestrictTake(d_plan, d_rlist[l], nr_l[l], d_lx, d_boff, d_cnt, d_rbuf, stream);
+        LaunchMLRestrictVjp<S>(d_plan, d_rlist[l], d_rent[l], d_rebeg[l], d_recell[l], nrc_l[l], tape, d_lx, d_boff,
+                               d_cnt, d_rbuf, mtopo.restrict_scale, stream);
+        break;
+    }
+    ADJ_CHECK(hipGetLastError());
+  }
+
+  // Multi-level run: steps are coarse steps (t = 0 for the first); J is the
+  // mean of dir . F after each coarse step of the window; dJ/du0 of the
+  // initial feq(rho0, u0) on the masked cells, dJ/d(inlet velocity) in dui,
+  // dJ/dq in d_dq.  Memory: checkpoints every K steps, the K states of one
+  // segment, and per step one tape per operation.
+  template <typename S>
+  void RunMLT(int steps, int w0, int w1, int K, const double* dir, const double* bbox, double rho0, const double* u0,
+              uint32_t mask, double* J, double* du0) {
+    std::vector<DragTerm> terms;
+    std::vector<int> seed_cell;
+    std::vector<double> seed_w;
+    Terms(dir, bbox, &terms, &seed_cell, &seed_w);
+    const int nterms = static_cast<int>(terms.size()), nseed = static_cast<int>(seed_cell.size());
+    const int nck = (steps + K - 1) / K;
+    const size_t st = static_cast<size_t>(ntot), pm = static_cast<size_t>(npts) * kMacro;
+    const size_t pmn = std::max<size_t>(pm, 1);
+    std::vector<void*> run_owned;
+    auto alloc = [&](size_t bytes) {
+      void* p = nullptr;
+      ADJ_CHECK(hipMalloc(&p, bytes == 0 ? 1 : bytes));
+      run_owned.push_back(p);
+      return p;
+    };
+    try {
+      DragTerm* d_terms = static_cast<DragTerm*>(alloc(terms.size() * sizeof(DragTerm)));
+      std::vector<int> cseed(static_cast<size_t>(ncells), -1);
+      for (int k = 0; k < nseed; ++k) cseed[seed_cell[k]] = k;
+      int* d_cseed = static_cast<int*>(alloc(cseed.size() * sizeof(int)));
+      double* d_seed_w = static_cast<double*>(alloc(std::max<size_t>(seed_w.size(), 1) * sizeof(double)));
+      const int ndg = std::max(1, static_cast<int>(Blocks(nterms)));
+      double* d_jv = static_cast<double*>(alloc(static_cast<size_t>(steps + 1) * ndg * sizeof(double)));
+      S* x = static_cast<S*>(alloc(st * sizeof(S)));
+      S* y = static_cast<S*>(alloc(st * sizeof(S)));
+      S* pv = static_cast<S*>(alloc(pmn * sizeof(S)));
+      S* ck = static_cast<S*>(alloc(static_cast<size_t>(nck) * st * sizeof(S)));
+      S* ckp = static_cast<S*>(alloc(static_cast<size_t>(nck) * pmn * sizeof(S)));
+      S* seg = static_cast<S*>(alloc(static_cast<size_t>(K) * st * sizeof(S)));
+      S* segp = static_cast<S*>(alloc(static_cast<size_t>(K) * pmn * sizeof(S)));
+      const size_t nops = ops.size();
+      S* tape = static_cast<S*>(alloc(nops * st * sizeof(S)));
+      S* ptape = static_cast<S*>(alloc(nops * pmn * sizeof(S)));
+      double* lp = static_cast<double*>(alloc(pmn * sizeof(double)));
+      const unsigned ncb = Blocks(ncells);
+      double* d_part = static_cast<double*>(alloc(static_cast<size_t>(ncb) * 3 * sizeof(double)));
+      ADJ_CHECK(hipMemcpyAsync(d_cseed, cseed.data(), cseed.size() * sizeof(int), hipMemcpyHostToDevice, stream));
+      if (nterms) {
+        ADJ_CHECK(hipMemcpyAsync(d_terms, terms.data(), terms.size() * sizeof(DragTerm), hipMemcpyHostToDevice, stream));
+        ADJ_CHECK(hipMemcpyAsync(d_seed_w, seed_w.data(), seed_w.size() * sizeof(double), hipMemcpyHostToDevice,
+                                 stream));
+      }
+      ADJ_CHECK(hipMemsetAsync(d_jv, 0, static_cast<size_t>(steps + 1) * ndg * sizeof(double), stream));
+      ADJ_CHECK(hipMemsetAsync(d_dq, 0, static_cast<size_t>(nlinks > 0 ? nlinks : 1) * sizeof(double), stream));
+      ADJ_CHECK(hipMemsetAsync(d_dui, 0, 3 * sizeof(double), stream));
+      ADJ_CHECK(hipMemsetAsync(x, 0, st * sizeof(S), stream));
+      ADJ_CHECK(hipMemsetAsync(y, 0, st * sizeof(S), stream));
+      hipEvent_t ev[3];
+      for (auto& e : ev) ADJ_CHECK(hipEventCreate(&e));
+      ADJ_CHECK(hipEventRecord(ev[0], stream));
+      auto step_fwd = [&](S* xs, S* ps, double t) {
+        for (const int2 op : ops) MLApply<S>(op, xs, y, ps, t, nullptr, nullptr);
+      };
+      // ---- forward sweep: initial state, coupler init, checkpoints, J
+      hipLaunchKernelGGL(KAdjInit<S>, dim3(ncb), dim3(kThreads), 0, stream, x, d_cells, d_ckind, ncells, rho0, u0[0],
+                         u0[1], u0[2], mask);
+      LaunchMLCouplerInit<S>(d_plan, npts, x, d_boff, d_cnt, pv, stream);
+      for (int s = 1; s <= steps; ++s) {
+        if ((s - 1) % K == 0) {
+          const int c = (s - 1) / K;
+          ADJ_CHECK(hipMemcpyAsync(ck + static_cast<size_t>(c) * st, x, st * sizeof(S), hipMemcpyDeviceToDevice, stream));
+          if (pm)
+            ADJ_CHECK(hipMemcpyAsync(ckp + static_cast<size_t>(c) * pmn, pv, pm * sizeof(S), hipMemcpyDeviceToDevice,
+                                     stream));
+        }
+        step_fwd(x, pv, static_cast<double>(s - 1));
+        if (s > w0 && s <= w1 && nterms)
+          hipLaunchKernelGGL(KMLDrag<S>, dim3(ndg), dim3(kThreads), 0, stream, y, x, d_terms, nterms,
+                             d_jv + static_cast<size_t>(s) * ndg);
+      }
+      ADJ_CHECK(hipMemcpyAsync(d_final, x, st * sizeof(S), hipMemcpyDeviceToDevice, stream));
+      ADJ_CHECK(hipEventRecord(ev[1], stream));
+      // ---- reverse, one checkpoint segment at a time, one step at a time
+      const double scale = 1.0 / static_cast<double>(w1 - w0);
+      ADJ_CHECK(hipMemsetAsync(d_lx, 0, st * sizeof(double), stream));
+      ADJ_CHECK(hipMemsetAsync(d_ln, 0, st * sizeof(double), stream));
+      ADJ_CHECK(hipMemsetAsync(lp, 0, pmn * sizeof(double), stream));
+      for (int c = nck - 1; c >= 0; --c) {
+        const int s0 = c * K, s1 = std::min(steps, s0 + K);
+        // segment states x_{s0} .. x_{s1 - 1} (and their prev macros)
+        ADJ_CHECK(hipMemcpyAsync(seg, ck + static_cast<size_t>(c) * st, st * sizeof(S), hipMemcpyDeviceToDevice, stream));
+        if (pm)
+          ADJ_CHECK(hipMemcpyAsync(segp, ckp + static_cast<size_t>(c) * pmn, pm * sizeof(S), hipMemcpyDeviceToDevice,
+                                   stream));
+        for (int j = 1; j < s1 - s0; ++j) {
+          ADJ_CHECK(hipMemcpyAsync(seg + static_cast<size_t>(j) * st, seg + static_cast<size_t>(j - 1) * st,
+                                   st * sizeof(S), hipMemcpyDeviceToDevice, stream));
+          if (pm)
+            ADJ_CHECK(hipMemcpyAsync(segp + static_cast<size_t>(j) * pmn, segp + static_cast<size_t>(j - 1) * pmn,
+                                     pm * sizeof(S), hipMemcpyDeviceToDevice, stream));
+          step_fwd(seg + static_cast<size_t>(j) * st, segp + static_cast<size_t>(j) * pmn,
+                   static_cast<double>(s0 + j - 1));
+        }
+        for (int s = s1; s > s0; --s) {
+          const double t = static_cast<double>(s - 1);
+          // re-run step s from x_{s-1}, taping what each operation's adjoint reads
+          ADJ_CHECK(hipMemcpyAsync(x, seg + static_cast<size_t>(s - 1 - s0) * st, st * sizeof(S),
+                                   hipMemcpyDeviceToDevice, stream));
+          if (pm)
+            ADJ_CHECK(hipMemcpyAsync(pv, segp + static_cast<size_t>(s - 1 - s0) * pmn, pm * sizeof(S),
+                                     hipMemcpyDeviceToDevice, stream));
+          for (size_t k = 0; k < nops; ++k)
+            MLApply<S>(ops[k], x, y, pv, t, tape + k * st, ptape + k * pmn);
+          const bool win = s > w0 && s <= w1 && nterms > 0;
+          if (win) hipLaunchKernelGGL(KAdjSeedLive, dim3(Blocks(nterms)), dim3(kThreads), 0, stream, d_lx, d_terms,
+                                      nterms, scale);
+          std::vector<char> seeded(nlev, 0);
+          for (size_t k = nops; k-- > 0;) {
+            double ss = 0.0;
+            if (ops[k].x == kOpVisit && win && !seeded[ops[k].y]) {
+              ss = scale;  // the level's last visit took the drag snapshot
+              seeded[ops[k].y] = 1;
+            }
+            MLReverse<S>(ops[k], t, tape + k * st, ptape + k * pmn, lp, ss, d_cseed, d_seed_w);
+          }
+        }
+      }
+      // the initial prev macros came from the initial state
+      LaunchMLCouplerInitVjp<S>(d_plan, d_ipts, d_ibeg, nig, ck, d_lx, d_boff, d_cnt, lp, stream);
+      hipLaunchKernelGGL(KAdjInitGrad, dim3(ncb), dim3(kThreads), 0, stream, d_lx, d_cells, d_ckind, ncells, rho0,
+                         u0[0], u0[1], u0[2], mask, d_part);
+      ADJ_CHECK(hipGetLastError());
+      ADJ_CHECK(hipEventRecord(ev[2], stream));
+      std::vector<double> jv(static_cast<size_t>(steps + 1) * ndg), part(static_cast<size_t>(ncb) * 3);
+      ADJ_CHECK(hipMemcpyAsync(jv.data(), d_jv, jv.size() * sizeof(double), hipMemcpyDeviceToHost, stream));
+      ADJ_CHECK(hipMemcpyAsync(part.data(), d_part, part.size() * sizeof(double), hipMemcpyDeviceToHost, stream));
+      ADJ_CHECK(hipMemcpyAsync(dui, d_dui, 3 * sizeof(double), hipMemcpyDeviceToHost, stream));
+      ADJ_CHECK(hipStreamSynchronize(stream));
+      float a = 0.f, b = 0.f;
+      ADJ_CHECK(hipEventElapsedTime(&a, ev[0], ev[1]));
+      ADJ_CHECK(hipEventElapsedTime(&b, ev[1], ev[2]));
+      fwd_ms = a;
+      bwd_ms = b;
+      for (auto& e : ev) ADJ_CHECK(hipEventDestroy(e));
+      double js = 0.0;
+      for (int s = w0 + 1; s <= w1; ++s) {
+        double step_sum = 0.0;
+        for (int g = 0; g < ndg; ++g) step_sum += jv[static_cast<size_t>(s) * ndg + g];
+        js += step_sum;
+      }
+      *J = js * scale;
+      du0[0] = du0[1] = du0[2] = 0.0;
+      for (unsigned g = 0; g < ncb; ++g)
+        for (int d = 0; d < 3; ++d) du0[d] += part[3 * g + d];
+    } catch (...) {
+      (void)hipStreamSynchronize(stream);
+      for (void* p : run_owned) (void)hipFree(p);
+      throw;
+    }
+    ADJ_CHECK(hipStreamSynchronize(stream));
+    for (void* p : run_owned) ADJ_CHECK(hipFree(p));
+  }
+
   void Run(int steps, int w0, int w1, int K, const double* dir, const double* bbox, double rho0, const double* u0,
            uint32_t mask, double* J, double* du0) {
+    if (ml) {
+      if (prec == 32) {
+        RunMLT<float>(steps, w0, w1, K, dir, bbox, rho0, u0, mask, J, du0);
+      } else {
+        RunMLT<double>(steps, w0, w1, K, dir, bbox, rho0, u0, mask, J, du0);
+      }
+      return;
+    }
+    dui[0] = dui[1] = dui[2] = 0.0;
     if (prec == 32) {
       RunT<float>(steps, w0, w1, K, dir, bbox, rho0, u0, mask, J, du0);
     } else {
@@ -952,8 +1670,15 @@ int octlb_adj_create(octlb_dev* dev, octlb_adj** out) {
   if (!a) return SetError(OCTLB_ENOMEM, "host allocation failed");
   const int rc = AdjGuard([&] {
     AdjTopo t;
-    ExportAdjTopo(dev, &t);
-    a->Build(t);
+    if (AdjNeedsMultiLevel(dev)) {
+      AdjMLTopo m;
+      ExportAdjMLTopo(dev, &t, &m);
+      a->Build(t);
+      a->BuildML(t, m);
+    } else {
+      ExportAdjTopo(dev, &t);
+      a->Build(t);
+    }
   });
   if (rc != OCTLB_OK) {
     a->FreeAll();
@@ -1015,6 +1740,13 @@ int octlb_adj_final_core(octlb_adj* adj, double* out) {
         for (int i = 0; i < kQ; ++i)
           out[(static_cast<size_t>(b) * kNC + c) * kQ + i] = s[static_cast<size_t>(cl.base) + static_cast<size_t>(i) * cl.stride];
       }
+  });
+}
+
+int octlb_adj_inlet_grad(const octlb_adj* adj, double out[3]) {
+  return AdjGuard([&] {
+    if (!adj || !out) throw std::invalid_argument("null argument");
+    for (int d = 0; d < 3; ++d) out[d] = adj->dui[d];
   });
 }
 

@@ -18,6 +18,7 @@
 #include "lbm_device.h"
 #include "octlb_gpu.h"
 #include "octlb_internal.h"
+#include "adjoint_topo.h"
 
 namespace octlb {
 
@@ -198,6 +199,10 @@ struct octlb_dev {
   int* d_fd_blocks = nullptr;  // blocked mode: boundary blocks with FD cells
   short* d_fd_list = nullptr;  // blocked mode: FD cells per FD block, phase order
   int4* d_fd_count = nullptr;  // blocked mode: phase end offsets per FD block
+  // host copies of the blocked FD tables (the multi-level adjoint takes them)
+  std::vector<int> fd_blocks_host;
+  std::vector<short> fd_slot_host, fd_list_host;
+  std::vector<int4> fd_count_host;
   short* d_fd_slot = nullptr;  // [fd block][512] FD slot of each cell or -1
   int nfd_blocks = 0;
   std::vector<void*> owned;
@@ -623,6 +628,10 @@ void octlb_dev::Prepare() {
   Sync();
   FreePrepared();
   nfd_blocks = 0;
+  fd_blocks_host.clear();
+  fd_slot_host.clear();
+  fd_list_host.clear();
+  fd_count_host.clear();
   fd_merge_ok = false;
   fd_pending = false;
   fd_pend_cnt = 0;
@@ -908,6 +917,10 @@ void octlb_dev::Prepare() {
         lists.insert(lists.end(), ls.begin(), ls.end());
       }
       nfd_blocks = static_cast<int>(fdb.size());
+      fd_blocks_host = fdb;
+      fd_slot_host = slots;
+      fd_list_host = lists;
+      fd_count_host = counts;
       d_fd_blocks = PAlloc<int>(fdb.size());
       Upload(d_fd_blocks, fdb, stream);
       d_fd_slot = PAlloc<short>(slots.size());
@@ -1550,6 +1563,70 @@ void ExportAdjTopo(const octlb_dev* d, AdjTopo* t) {
               nbk * NS + ((cx + 1) * NP + cy + 1) * NP + cz + 1;
         }
   }
+}
+
+bool AdjNeedsMultiLevel(const octlb_dev* d) {
+  if (!d) return false;
+  for (int lv : d->level)
+    if (lv != d->max_level) return true;
+  for (uint8_t k : d->kinds)
+    if (IsFd(k)) return true;
+  return false;
+}
+
+void ExportAdjMLTopo(octlb_dev* d, AdjTopo* t, AdjMLTopo* m) {
+  if (!d || !t || !m) throw std::invalid_argument("null argument");
+  if (!d->prepared) d->Prepare();
+  if (d->padded || d->n != kTile || d->h != 1)
+    throw std::invalid_argument("adjoint: needs blocks of 8^3 cells with a one-cell halo");
+  if (d->nmembers != 1) throw std::invalid_argument("adjoint: ensembles are not differentiated");
+  if (d->sc.handler != OCTLB_HANDLER_CONCRETE || d->sc.const_rho || !d->sc.bouzidi)
+    throw std::invalid_argument("adjoint: needs the concrete handler, plain BGK and Bouzidi streaming");
+  constexpr int NP = kTile + 2, NS = NP * NP * NP;
+  t->nb = d->nb;
+  t->prec = d->prec;
+  t->omega = d->sc.omega;
+  t->kinds = d->kinds;
+  t->link_key.clear();
+  t->link_q.clear();
+  for (const auto& kv : d->links) {
+    t->link_key.push_back(kv.first);
+    t->link_q.push_back(kv.second);
+  }
+  t->origin_cw.resize(static_cast<size_t>(d->nb) * 4);
+  t->halo_src.assign(static_cast<size_t>(d->nb) * NS, -1);
+  for (int b = 0; b < d->nb; ++b) {
+    const BlockTab& r = d->bt[b];
+    for (int a = 0; a < 3; ++a) t->origin_cw[4 * b + a] = r.origin[a];
+    t->origin_cw[4 * b + 3] = r.cw;
+    for (int px = 0; px < NP; ++px)
+      for (int py = 0; py < NP; ++py)
+        for (int pz = 0; pz < NP; ++pz) {
+          const int x = px - 1, y = py - 1, z = pz - 1;
+          if (x >= 0 && x < kTile && y >= 0 && y < kTile && z >= 0 && z < kTile) continue;
+          int cc = 0;
+          const int nbk = ApronNeighbour(r, x, y, z, &cc);
+          if (nbk < 0) continue;
+          const int cx = cc / (kTile * kTile), cy = (cc / kTile) % kTile, cz = cc % kTile;
+          t->halo_src[static_cast<size_t>(b) * NS + (px * NP + py) * NP + pz] =
+              nbk * NS + ((cx + 1) * NP + cy + 1) * NP + cz + 1;
+        }
+  }
+  t->level = d->level;
+  m->max_level = d->max_level;
+  m->level = d->level;
+  m->plan = d->plan;
+  m->prolong_active.assign(d->prolong_active.begin(), d->prolong_active.end());
+  m->restrict_active.assign(d->restrict_active.begin(), d->restrict_active.end());
+  const double tau = 1.0 / d->sc.omega;  // level_coupler.cpp:226-228
+  m->prolong_scale = (tau - 0.25) / tau;
+  m->restrict_scale = tau / (tau - 0.25);
+  m->fd_blocks = d->fd_blocks_host;
+  m->fd_slot = d->fd_slot_host;
+  m->fd_list = d->fd_list_host;
+  m->fd_count = d->fd_count_host;
+  m->bt = d->bt;
+  m->sc = d->sc;
 }
 
 }  // namespace octlb

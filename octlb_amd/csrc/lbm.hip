@@ -1123,9 +1123,11 @@ struct FdSmem {
 
 // One FD block pass by a workgroup of 512 or more threads (threads beyond
 // the block's 512 cells take part in the barriers and the task loop only).
+// The pass over block fd_blocks[fb] whose populations are
+// cur[i * plane + cell] (plane = 512 in the production layout), at time t.
 template <typename T>
-__device__ inline void FdBlockBody(const FdArgs<T>& a, const int* fd_blocks, const short* fd_slot, int fb,
-                                   FdSmem<T>& sm) {
+__device__ inline void FdBlockCore(const FdArgs<T>& a, const int* fd_blocks, const short* fd_slot, int fb, T* cur,
+                                   int plane, double t, FdSmem<T>& sm) {
   T* rho_pre = sm.rho_pre;
   T* rho_post = sm.rho_post;
   T* ux = sm.ux;
@@ -1153,17 +1155,15 @@ __device__ inline void FdBlockBody(const FdArgs<T>& a, const int* fd_blocks, con
   const int b = fd_blocks[fb];
   const int tid = threadIdx.x;
   const SimConst& sc = *(const SimConst*)&a.scm[a.bt[b].member];
-  T* cur = (a.par.v[a.level[b]] ? a.pops1 : a.pops0) + static_cast<size_t>(b) * kQ * kTileCells;
   if (act) {
     const int ix = tid >> 6, iy = (tid >> 3) & 7, iz = tid & 7;
     const int k = a.kind[static_cast<size_t>(b) * kTileCells + tid];
     kind[tid] = static_cast<uint8_t>(k);
     slot[tid] = fd_slot[static_cast<size_t>(fb) * kTileCells + tid];
-    const double t = static_cast<double>(*a.step);
     const Origin og{a.bt[b].origin[0], a.bt[b].origin[1], a.bt[b].origin[2], a.bt[b].cw};
     T f[kQ];
 #pragma unroll
-    for (int i = 0; i < kQ; ++i) f[i] = cur[static_cast<size_t>(i) * kTileCells + tid];
+    for (int i = 0; i < kQ; ++i) f[i] = cur[static_cast<size_t>(i) * plane + tid];
     T pu[3] = {T(0), T(0), T(0)};
     if (k == kVelocity) {
       const double org[3] = {og.x, og.y, og.z};
@@ -1377,7 +1377,7 @@ __device__ inline void FdBlockBody(const FdArgs<T>& a, const int* fd_blocks, con
       const int te = begin + w / kQ, i = w % kQ;
       const T f = FdPopulation<T>(ctab[i], wtab[i], crho[te], cu[te], cusq[te], cpi[te]);
       fbuf[te][i] = f;
-      cur[static_cast<size_t>(i) * kTileCells + lst[te]] = f;
+      cur[static_cast<size_t>(i) * plane + lst[te]] = f;
     }
     __syncthreads();
     if (myph == ph) {
@@ -1387,6 +1387,14 @@ __device__ inline void FdBlockBody(const FdArgs<T>& a, const int* fd_blocks, con
     }
     __syncthreads();
   }
+}
+
+template <typename T>
+__device__ inline void FdBlockBody(const FdArgs<T>& a, const int* fd_blocks, const short* fd_slot, int fb,
+                                   FdSmem<T>& sm) {
+  const int b = fd_blocks[fb];
+  T* cur = (a.par.v[a.level[b]] ? a.pops1 : a.pops0) + static_cast<size_t>(b) * kQ * kTileCells;
+  FdBlockCore<T>(a, fd_blocks, fd_slot, fb, cur, kTileCells, static_cast<double>(*a.step), sm);
 }
 
 template <typename T>
@@ -1853,5 +1861,7 @@ void LaunchMacro(const T* p0, const T* p1, const ParArray& par, const BlockTab* 
 
 OCTLB_INSTANTIATE(double)
 OCTLB_INSTANTIATE(float)
+
+#include "adjoint_ml.inc"
 
 }  // namespace octlb

@@ -304,6 +304,34 @@ __device__ inline void InletU(const DevSpec& sp, int ix, int iy, int iz, double 
   for (int d = 0; d < 3; ++d) u[d] = r * sp.inlet_u[d];
 }
 
+// Scale factor of the prescribed velocity with respect to the inlet
+// velocity vector of the cell's inlet face (InletU: u = factor * inlet_u);
+// 0 where the velocity does not come from an inlet field.
+__device__ inline double InletFactor(const SimConst& sc, const double* org, double cw, int ix, int iy, int iz,
+                                     double t) {
+  const int n = sc.n;
+  const bool on[6] = {ix == 0, ix == n - 1, iy == 0, iy == n - 1, iz == 0, iz == n - 1};
+  for (int f = 0; f < 6; ++f) {
+    if (!on[f]) continue;
+    const DevSpec& sp = sc.spec[f];
+    if (sp.inlet == OCTLB_INLET_NONE) continue;
+    const double r = RampD(t, sp.ramp_end_t);
+    if (sp.inlet == OCTLB_INLET_UNIFORM) return r;
+    const bool xf = sp.inlet_face == OCTLB_XMIN || sp.inlet_face == OCTLB_XMAX;
+    const bool zf = sp.inlet_face == OCTLB_ZMIN || sp.inlet_face == OCTLB_ZMAX;
+    if (sp.inlet == OCTLB_INLET_CHANNEL) {
+      const double px = org[0] + (ix + 0.5) * cw, py = org[1] + (iy + 0.5) * cw, pz = org[2] + (iz + 0.5) * cw;
+      const double c0 = Clamp01(((xf ? py : px) - sp.lo0) / (sp.hi0 - sp.lo0));
+      const double c1 = Clamp01(((zf ? py : pz) - sp.lo1) / (sp.hi1 - sp.lo1));
+      return r * sp.u_peak * 4.0 * c0 * (1.0 - c0) * 4.0 * c1 * (1.0 - c1);
+    }
+    const double c0 = (static_cast<double>(xf ? iy : ix) + 0.5) / static_cast<double>(sp.n0);
+    const double c1 = (static_cast<double>(zf ? iy : iz) + 0.5) / static_cast<double>(sp.n1);
+    return r * sp.u_peak * 4.0 * c0 * (1.0 - c0) * 4.0 * c1 * (1.0 - c1);
+  }
+  return 0.0;
+}
+
 // detail::PrescribedBoundaryU (interpolated_velocity.h:69-117), block-local faces.
 __device__ inline void PrescribedU(const SimConst& sc, const double* org, double cw, int ix, int iy,
                                    int iz, double t, double* u) {

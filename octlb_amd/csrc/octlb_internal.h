@@ -25,6 +25,7 @@ struct AdjTopo {
   std::vector<double> link_q;     // its wall fraction
   std::vector<double> origin_cw;  // [nb][4] origin x, y, z and cell width
   std::vector<int> halo_src;      // [nb][1000] exchanged halo slot -> source block*1000 + slot, or -1
+  std::vector<int> level;         // [nb] refinement level (empty: one level)
 };
 
 // VTK writers (vtk_out.cpp); throw on bad arguments or file errors.

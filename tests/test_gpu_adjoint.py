@@ -103,16 +103,25 @@ def test_autograd_function(box, device_grad):
     np.testing.assert_array_equal(q.grad.numpy(), 2.0 * device_grad[2])
 
 
-def test_create_rejects_what_it_does_not_differentiate():
+def test_create_scope():
+    # velocity / pressure boundaries and several levels run the multi-level
+    # engine; ConstRhoBGK and ensembles are not differentiated
     open_channel = cases.cylinder_channel(bricks_x=4, base_level=0, fine=False, bricks_yz=2, xc=0.6,
                                           radius=0.125, u_in=0.05, tau=0.6)
     sol = GpuSolver(open_channel.topo, open_channel.params(64))
     sol.set_bc_kinds(open_channel.kinds)
     h = C.c_void_p()
-    rc = abi.lib().octlb_adj_create(sol._h, C.byref(h))
-    msg = abi.lib().octlb_last_error().decode()
+    assert abi.lib().octlb_adj_create(sol._h, C.byref(h)) == 0 and h.value
+    abi.lib().octlb_adj_destroy(h)
     sol.close()
-    assert rc == -1 and "FD" in msg  # OCTLB_EINVAL
+    ens = cases.cavity_ensemble(count=2, sizes=(32,), seed=2)
+    e = GpuSolver.ensemble(ens, 64)
+    e.set_bc_kinds(ens.kinds)
+    h = C.c_void_p()
+    rc = abi.lib().octlb_adj_create(e._h, C.byref(h))
+    msg = abi.lib().octlb_last_error().decode()
+    e.close()
+    assert rc == -1 and "ensembles" in msg  # OCTLB_EINVAL
 
 
 def test_fp32_forward_fp64_adjoint(box, device_grad):

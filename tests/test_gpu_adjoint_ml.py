@@ -1,0 +1,124 @@
+"""Multi-level discrete adjoint (two refinement levels with the Lagrava
+coupler, FD velocity inlet and pressure outlet, Bouzidi cylinder) on a
+reduced 2-level channel: 72 blocks, 24 coarse steps, J = mean x-drag over
+coarse steps 13..24, checkpoint every 5 steps, inlet ramp over 8 steps.
+
+* Its forward sweep is the production coarse step bit for bit (FP64 core
+  populations after octlb_dev_advance) and J equals the production drag.
+* dJ/du_in (through the velocity-Dirichlet collide and the FD pass, whose
+  stencils read the prescribed velocity) and dJ/dq (through the Bouzidi
+  rows of the fine level's stream, whose adjoint crosses the coupler and the
+  FD pass) match central differences of the FP64 oracle's forward
+  (eps = 1e-6) to 1e-4: u_in's x component, the three largest |dJ/dq|, and
+  a random direction over all links.
+* The gradients do not depend on the checkpoint interval.
+"""
+import numpy as np
+import pytest
+
+from octlb_amd import abi, cases
+from octlb_amd.adjoint import DragAdjoint
+from octlb_amd.solver import GpuSolver
+from oracle.binding import OracleSim
+
+pytestmark = pytest.mark.gpu
+
+STEPS, WIN, CK = 24, (12, 24), 5
+EPS = 1e-6
+TOL = 1e-4
+
+
+@pytest.fixture(scope="module")
+def channel():
+    return cases.cylinder_channel(bricks_x=2, base_level=1, fine=True, xc=0.5, radius=0.1, refine_lo=0.25,
+                                  refine_hi=1.0, u_in=0.04, ramp_end_t=8.0)
+
+
+def _u_in(case):
+    for s in case.specs:
+        if s.inlet != abi.INLET_NONE:
+            return np.array([s.inlet_u[0], s.inlet_u[1], s.inlet_u[2]])
+    raise AssertionError("no inlet")
+
+
+def _oracle_J(case, q=None, u_in=None):
+    adj = DragAdjoint(case, STEPS, WIN)
+    p = adj._params(u_in)
+    s = OracleSim(case.topo, p)
+    s.set_bc_kinds(case.kinds)
+    s.set_links(case.link, case.q if q is None else q)
+    s.init_equilibrium(1.0, case.init_u0, case.init_mask)
+    s.set_threads(8)
+    J = 0.0
+    for st in range(1, STEPS + 1):
+        s.advance(1)
+        if WIN[0] < st <= WIN[1]:
+            J += s.drag(case.drag_bbox)[0]
+    s.close()
+    return J / (WIN[1] - WIN[0])
+
+
+@pytest.fixture(scope="module")
+def device(channel):
+    print("multi-level adjoint: 72 blocks, 24 coarse steps, checkpoint 5", flush=True)
+    adj = DragAdjoint(channel, STEPS, WIN, checkpoint=CK)
+    J, du0, dq, core = adj.evaluate(final_core=True)
+    return J, dq, adj.inlet_grad.copy(), core
+
+
+def test_forward_is_the_production_step(channel, device):
+    J, _, _, core = device
+    sol = GpuSolver(channel.topo, channel.params(64))
+    sol.set_bc_kinds(channel.kinds)
+    sol.set_links(channel.link, channel.q)
+    sol.init_equilibrium(1.0, channel.init_u0, channel.init_mask)
+    Jp = 0.0
+    for st in range(1, STEPS + 1):
+        sol.advance(1)
+        if WIN[0] < st <= WIN[1]:
+            Jp += sol.drag(channel.drag_bbox)[0]
+    Jp /= WIN[1] - WIN[0]
+    ref = sol.download_core()
+    sol.close()
+    print(f"J {J:.15e} vs production {Jp:.15e}", flush=True)
+    np.testing.assert_array_equal(core, ref)
+    assert J == pytest.approx(Jp, rel=1e-12)
+
+
+def test_inlet_velocity_gradient_matches_central_differences(channel, device):
+    _, _, gi, _ = device
+    u = _u_in(channel)
+    e = np.array([EPS, 0.0, 0.0])
+    fd = (_oracle_J(channel, u_in=u + e) - _oracle_J(channel, u_in=u - e)) / (2 * EPS)
+    print(f"dJ/du_in,x adjoint {gi[0]:.10e} vs central difference {fd:.10e}", flush=True)
+    assert abs(fd) > 0
+    assert abs(gi[0] - fd) <= TOL * abs(fd)
+
+
+def test_wall_fraction_gradients_match_central_differences(channel, device):
+    _, dq, _, _ = device
+    q = np.array(channel.q, dtype=np.float64)
+    top = np.argsort(-np.abs(dq))[:3]
+    for k in top:
+        qp, qm = q.copy(), q.copy()
+        qp[k] += EPS
+        qm[k] -= EPS
+        fd = (_oracle_J(channel, q=qp) - _oracle_J(channel, q=qm)) / (2 * EPS)
+        print(f"link {k}: dJ/dq adjoint {dq[k]:.10e} vs central difference {fd:.10e}", flush=True)
+        assert abs(dq[k] - fd) <= TOL * abs(fd)
+    rng = np.random.default_rng(5)
+    v = rng.normal(size=q.shape)
+    v /= np.abs(v).max()
+    fd = (_oracle_J(channel, q=q + EPS * v) - _oracle_J(channel, q=q - EPS * v)) / (2 * EPS)
+    ad = float(dq @ v)
+    print(f"random direction: adjoint {ad:.10e} vs central difference {fd:.10e}", flush=True)
+    assert abs(ad - fd) <= TOL * abs(fd)
+
+
+def test_checkpoint_interval_does_not_change_the_gradients(channel, device):
+    J, dq, gi, _ = device
+    adj = DragAdjoint(channel, STEPS, WIN, checkpoint=STEPS)
+    J2, _, dq2 = adj.evaluate()
+    assert J2 == J
+    np.testing.assert_array_equal(dq2, dq)
+    np.testing.assert_array_equal(adj.inlet_grad, gi)
