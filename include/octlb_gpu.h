@@ -300,6 +300,23 @@ int octlb_build_links(int32_t num_blocks, int32_t n, const double* bounds, const
                       int64_t num_triangles, const double* triangles, int64_t capacity,
                       int64_t* num_links, int32_t* links, double* q);
 
+/* No counterpart in the reference: octlb_build_links for callers that
+ * cannot allocate device memory (code under HIP graph capture, serving
+ * loops, harnesses that forbid hipMalloc).  Same arguments and results,
+ * but every array (bounds, material, triangles in; links, q out) is a
+ * device pointer the caller owns, the work is queued on `stream` (null: the
+ * default stream) and the call allocates nothing: its scratch is a static
+ * device array of about 10.9 MB per GPU, held until the process ends.
+ * Limits: num_blocks <= 16384 and at most 65536 candidate cells (fluid cells
+ * with a solid-like in-block neighbour); beyond either, OCTLB_EINVAL (the
+ * block limit before any device work, the candidate limit after the
+ * counting pass, before the scratch could overflow).  The call synchronizes `stream` twice
+ * (the candidate and link counts come back to the host).  Calls that
+ * overlap on one GPU share the scratch and race: serialize them. */
+int octlb_build_links_noalloc(int32_t num_blocks, int32_t n, const double* bounds, const uint8_t* material,
+                              int64_t num_triangles, const double* triangles, int64_t capacity,
+                              int64_t* num_links, int32_t* links, double* q, void* stream);
+
 /* Re-adaptation (the survey's dynamic-AMR contract; the reference has
  * no migration code): after the forest changes and a new handle dst is
  * built for it, move src's state onto dst.  Same block edge, halo,

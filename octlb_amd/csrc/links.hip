@@ -22,6 +22,7 @@
 
 #include <cfloat>
 #include <cstdint>
+#include <mutex>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -380,8 +381,116 @@ void BuildLinks(int32_t nb, int32_t n, const double* bounds, const uint8_t* mate
   LINK_CHECK(hipStreamSynchronize(st.s));
 }
 
+// ---- no-allocation form: every scratch buffer of the build lives in one
+// file-scope device array per GPU, sized for at most kNoAllocMaxBlocks
+// blocks and kNoAllocMaxCandidates candidate cells.
+constexpr int64_t kNoAllocMaxBlocks = 16384;
+constexpr int64_t kNoAllocMaxCandidates = 65536;
+
+constexpr size_t Align256(size_t b) { return (b + 255) & ~static_cast<size_t>(255); }
+constexpr size_t kScrCounts = 0;
+constexpr size_t kScrBoff = kScrCounts + Align256(sizeof(int) * kNoAllocMaxBlocks);
+constexpr size_t kScrCand = kScrBoff + Align256(sizeof(int64_t) * (kNoAllocMaxBlocks + 1));
+constexpr size_t kScrQd = kScrCand + Align256(sizeof(int2) * kNoAllocMaxCandidates);
+constexpr size_t kScrCnt = kScrQd + Align256(sizeof(double) * (kQ - 1) * kNoAllocMaxCandidates);
+constexpr size_t kScrLoff = kScrCnt + Align256(sizeof(int) * kNoAllocMaxCandidates);
+constexpr size_t kScratchBytes = kScrLoff + Align256(sizeof(int64_t) * (kNoAllocMaxCandidates + 1));
+
+}  // namespace
+
+// ~10.9 MB per GPU, held until the process ends.  In host code this name is
+// only a symbol handle: its device address comes from hipGetSymbolAddress.
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+
+namespace {
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+unsigned char* g_scratch_ptr[kMaxDevices] = {};
+
+// Device address of `scratch` on the current GPU, looked up once per device.
+unsigned char* ScratchOnCurrentDevice() {
+  int dev = 0;
+  LINK_CHECK(hipGetDevice(&dev));
+  if (dev < 0 || dev >= kMaxDevices) throw std::runtime_error("device index out of range");
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void* p = nullptr;
+    LINK_CHECK(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    g_scratch_ptr[dev] = static_cast<unsigned char*>(p);
+  }
+  return g_scratch_ptr[dev];
+}
+
+void BuildLinksNoAlloc(int32_t nb, int32_t n, const double* d_bounds, const uint8_t* d_mat, int64_t ntri,
+                       const double* d_tris, int64_t capacity, int64_t* num_links, int32_t* d_links, double* d_q,
+                       hipStream_t s) {
+  *num_links = 0;
+  if (nb > kNoAllocMaxBlocks) throw std::invalid_argument("more blocks than the no-allocation scratch holds (16384)");
+  if (ntri == 0 || nb == 0) return;  // soup.empty(): no links (bouzidi_link_data.cpp:147-149)
+  unsigned char* base = ScratchOnCurrentDevice();
+  int* d_counts = reinterpret_cast<int*>(base + kScrCounts);
+  int64_t* d_boff = reinterpret_cast<int64_t*>(base + kScrBoff);
+  int2* d_cand = reinterpret_cast<int2*>(base + kScrCand);
+  double* d_qd = reinterpret_cast<double*>(base + kScrQd);
+  int* d_cnt = reinterpret_cast<int*>(base + kScrCnt);
+  int64_t* d_loff = reinterpret_cast<int64_t*>(base + kScrLoff);
+  // every scratch word read below is written first in this call
+  KCandidates<false><<<nb, kLinkThreads, 0, s>>>(n, d_mat, d_counts, nullptr, nullptr);
+  LINK_CHECK(hipGetLastError());
+  KScan<<<1, 1024, 0, s>>>(d_counts, nb, d_boff);
+  LINK_CHECK(hipGetLastError());
+  int64_t ncand = 0;
+  LINK_CHECK(hipMemcpyAsync(&ncand, d_boff + nb, sizeof(int64_t), hipMemcpyDeviceToHost, s));
+  LINK_CHECK(hipStreamSynchronize(s));
+  if (ncand == 0) return;
+  if (ncand > kNoAllocMaxCandidates)
+    throw std::invalid_argument("more candidate cells than the no-allocation scratch holds (65536)");
+  KCandidates<true><<<nb, kLinkThreads, 0, s>>>(n, d_mat, nullptr, d_boff, d_cand);
+  LINK_CHECK(hipGetLastError());
+  const int64_t waves_per_wg = kLinkThreads / 64;
+  const unsigned grid = static_cast<unsigned>((ncand + waves_per_wg - 1) / waves_per_wg);
+  KNearest<<<grid, kLinkThreads, 0, s>>>(d_cand, ncand, n, d_bounds, d_mat, d_tris, ntri, d_qd, d_cnt);
+  LINK_CHECK(hipGetLastError());
+  KScan<<<1, 1024, 0, s>>>(d_cnt, ncand, d_loff);
+  LINK_CHECK(hipGetLastError());
+  int64_t total = 0;
+  LINK_CHECK(hipMemcpyAsync(&total, d_loff + ncand, sizeof(int64_t), hipMemcpyDeviceToHost, s));
+  LINK_CHECK(hipStreamSynchronize(s));
+  *num_links = total;
+  if (d_links == nullptr && d_q == nullptr) return;  // count query
+  if (capacity < total) throw std::invalid_argument("link capacity smaller than the number of links");
+  if (total == 0) return;
+  KEmit<<<grid, kLinkThreads, 0, s>>>(d_cand, ncand, d_qd, d_loff, d_links, d_q);
+  LINK_CHECK(hipGetLastError());
+  LINK_CHECK(hipStreamSynchronize(s));
+}
+
 }  // namespace
 }  // namespace octlb
+
+extern "C" int octlb_build_links_noalloc(int32_t num_blocks, int32_t n, const double* bounds,
+                                         const uint8_t* material, int64_t num_triangles, const double* triangles,
+                                         int64_t capacity, int64_t* num_links, int32_t* links, double* q,
+                                         void* stream) {
+  using namespace octlb;
+  if (!num_links) return SetError(OCTLB_EINVAL, "null num_links");
+  *num_links = 0;
+  if (num_blocks < 0 || n < 1 || n > 1024 || num_triangles < 0 || capacity < 0)
+    return SetError(OCTLB_EINVAL, "bad link-build sizes");
+  if ((num_blocks > 0 && (!bounds || !material)) || (num_triangles > 0 && !triangles))
+    return SetError(OCTLB_EINVAL, "null link-build input");
+  if ((links == nullptr) != (q == nullptr)) return SetError(OCTLB_EINVAL, "links and q must both be given");
+  try {
+    BuildLinksNoAlloc(num_blocks, n, bounds, material, num_triangles, triangles, capacity, num_links, links, q,
+                      static_cast<hipStream_t>(stream));
+  } catch (const std::invalid_argument& e) {
+    return SetError(OCTLB_EINVAL, e.what());
+  } catch (const std::exception& e) {
+    return SetError(OCTLB_EHIP, e.what());
+  }
+  return OCTLB_OK;
+}
 
 extern "C" int octlb_build_links(int32_t num_blocks, int32_t n, const double* bounds, const uint8_t* material,
                                  int64_t num_triangles, const double* triangles, int64_t capacity,
