@@ -639,13 +639,24 @@ __global__ __launch_bounds__(kThreads) void KMLDrag(const S* y, const S* x, cons
   if (threadIdx.x == 0) out[blockIdx.x] = red[0];
 }
 
-// acc[0..2] += sum of part[3 j + d] over j in order (one thread)
-__global__ void KMLAccum(const double* part, int n, double* acc) {
-  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+// acc[0..2] += sum of part[3 j + d] over j, in a fixed order: thread t sums
+// j = t, t + kAccumThreads, ... in turn, then a tree over the threads.
+constexpr int kAccumThreads = 256;
+__global__ __launch_bounds__(kAccumThreads) void KMLAccum(const double* part, int n, double* acc) {
+  __shared__ double red[kAccumThreads][3];
+  const int t = threadIdx.x;
   double s[3] = {0.0, 0.0, 0.0};
-  for (int j = 0; j < n; ++j)
+  for (int j = t; j < n; j += kAccumThreads)
     for (int d = 0; d < 3; ++d) s[d] += part[3 * j + d];
-  for (int d = 0; d < 3; ++d) acc[d] += s[d];
+  for (int d = 0; d < 3; ++d) red[t][d] = s[d];
+  __syncthreads();
+  for (int w = kAccumThreads / 2; w > 0; w >>= 1) {
+    if (t < w)
+      for (int d = 0; d < 3; ++d) red[t][d] += red[t + w][d];
+    __syncthreads();
+  }
+  if (t == 0)
+    for (int d = 0; d < 3; ++d) acc[d] += red[0][d];
 }
 
 // copies the entries of the listed blocks (one workgroup per block)
@@ -715,6 +726,8 @@ struct octlb_adj {
   short* d_fd_slot = nullptr;
   short* d_fd_list = nullptr;
   int4* d_fd_count = nullptr;
+  int* d_fd_rbeg = nullptr;           // reverse FD stencils: CSR offsets per FD block
+  FdRevEntry* d_fd_rent = nullptr;    // and entries (kFdRevMax per FD block)
   int* d_plan = nullptr;
   std::vector<int*> d_plist, d_ppts, d_pbeg, d_rlist, d_rebeg, d_recell;
   std::vector<int2*> d_rent;
@@ -1124,6 +1137,12 @@ struct octlb_adj {
     d_fd_slot = Upload(m.fd_slot);
     d_fd_list = Upload(m.fd_list);
     d_fd_count = Upload(m.fd_count);
+    if (nfd > 0) {
+      d_fd_rbeg = Alloc<int>(static_cast<size_t>(nfd) * kFdRevBeg);
+      d_fd_rent = Alloc<FdRevEntry>(static_cast<size_t>(nfd) * kFdRevMax);
+      LaunchMLFdRevBuild(d_fd_list, d_fd_count, d_fd_slot, nfd, d_fd_rbeg, d_fd_rent, stream);
+      ADJ_CHECK(hipGetLastError());
+    }
     npts = static_cast<int>(m.plan.size() / 12);
     d_plan = Upload(m.plan);
     // prolongation: per coarse level, the active points and their grouping by
@@ -1312,13 +1331,14 @@ struct octlb_adj {
         ADJ_CHECK(hipMemcpyAsync(d_lx + lv_entry[l], d_ln + lv_entry[l],
                                  static_cast<size_t>(lv_entry[l + 1] - lv_entry[l]) * sizeof(double),
                                  hipMemcpyDeviceToDevice, stream));
-        hipLaunchKernelGGL(KMLAccum, dim3(1), dim3(1), 0, stream, d_dui_part, static_cast<int>(nwg), d_dui);
+        hipLaunchKernelGGL(KMLAccum, dim3(1), dim3(kAccumThreads), 0, stream, d_dui_part, static_cast<int>(nwg),
+                           d_dui);
         break;
       }
       case kOpFd:
-        LaunchMLFdVjp<S>(MLFdArgs<S>(), d_fd_blocks, d_fd_slot, nfd, tape, d_lx, d_boff, d_cnt, t, d_dui_part,
-                         stream);
-        hipLaunchKernelGGL(KMLAccum, dim3(1), dim3(1), 0, stream, d_dui_part, nfd, d_dui);
+        LaunchMLFdVjp<S>(MLFdArgs<S>(), d_fd_blocks, d_fd_slot, nfd, tape, d_lx, d_boff, d_cnt, t, d_fd_rbeg,
+                         d_fd_rent, d_dui_part, stream);
+        hipLaunchKernelGGL(KMLAccum, dim3(1), dim3(kAccumThreads), 0, stream, d_dui_part, nfd, d_dui);
         break;
       case kOpHalf:
       case kOpFull:

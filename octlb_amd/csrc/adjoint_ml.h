@@ -8,12 +8,29 @@
 
 namespace octlb {
 
+// Reverse FD stencils (built once per adjoint handle): for each FD block,
+// phase and block cell, the reads of that cell by the phase's FD cells, in
+// the order the FD VJP sums them.  code = kind | gradient row << 2 | mirror
+// flips of x, y, z << 4.
+struct FdRevEntry {
+  int ee;       // the reading FD cell (index into the block's FD list)
+  int code;
+  double coef;  // stencil coefficient
+};
+constexpr int kFdRevOwnU = 0, kFdRevGrad = 1, kFdRevRhoPost = 2, kFdRevRhoPre = 3;
+constexpr int kFdRevMax = 296 * 16;           // per FD block: <= 16 reads per FD cell
+constexpr int kFdRevBeg = 3 * 512 + 1;        // per FD block: CSR offsets over (phase, cell)
+
+void LaunchMLFdRevBuild(const short* fd_list, const int4* fd_count, const short* fd_slot, int nfd, int* rbeg,
+                        FdRevEntry* rent, hipStream_t s);
+
 template <typename S>
 void LaunchMLFd(const FdArgs<S>& a, const int* fd_blocks, const short* fd_slot, int nfd, S* x, const int* boff,
                 const int* cnt, double t, hipStream_t s);
 template <typename S>
 void LaunchMLFdVjp(const FdArgs<S>& a, const int* fd_blocks, const short* fd_slot, int nfd, const S* xpre,
-                   double* lam, const int* boff, const int* cnt, double t, double* dui, hipStream_t s);
+                   double* lam, const int* boff, const int* cnt, double t, const int* rbeg, const FdRevEntry* rent,
+                   double* dui, hipStream_t s);
 template <typename S>
 void LaunchMLProlong(const int* plan, const int* list, int count, int half, S* x, const int* boff, const int* cnt,
                      S* prev, double scale, hipStream_t s);

@@ -13,6 +13,7 @@
 // from the block's shell store), pull-streams through an LDS copy of the
 // whole 10^3 box and writes S of step n to the other buffer.
 #include "lbm_device.h"
+#include "adjoint_ml.h"
 
 namespace octlb {
 
