@@ -403,6 +403,14 @@ int octlb_adj_final_core(octlb_adj* adj, double* out);
 int octlb_adj_inlet_grad(const octlb_adj* adj, double out[3]);
 /* Device time of the last run's forward sweep and reverse sweep (ms). */
 int octlb_adj_timing(const octlb_adj* adj, double* forward_ms, double* reverse_ms);
+/* Multi-level engine: device memory allowed for one checkpoint segment's
+ * tapes (bytes; -1, the default: 60 % of the free device memory).  When a
+ * segment's K steps of tapes fit, the reverse sweep runs one taping pass per
+ * segment (two forward passes per step in all); otherwise it recomputes the
+ * segment's states and re-runs each step before reversing it (three).  Both
+ * give the same gradients bit for bit.  *packed_last_run (may be null)
+ * reports which mode the last run used. */
+int octlb_adj_set_tape_budget(octlb_adj* adj, int64_t bytes, int32_t* packed_last_run);
 
 #ifdef __cplusplus
 }

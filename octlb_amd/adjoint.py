@@ -28,7 +28,7 @@ from .solver import GpuSolver
 
 class DragAdjoint:
     def __init__(self, case, steps: int, window: tuple, checkpoint: int = 100, direction=(1.0, 0.0, 0.0),
-                 rho0: float = 1.0, precision: int = 64):
+                 rho0: float = 1.0, precision: int = 64, tape_budget: int = -1):
         self.case = case
         self.steps = int(steps)
         self.window = (int(window[0]), int(window[1]))
@@ -38,6 +38,10 @@ class DragAdjoint:
         self.precision = int(precision)  # forward states: 64, or 32 with the adjoint still in FP64
         self.timing = (0.0, 0.0)
         self.inlet_grad = np.zeros(3)
+        # multi-level engine: bytes for one segment's tapes (-1: automatic,
+        # 0: always recompute); ``packed`` tells which mode the last run used
+        self.tape_budget = int(tape_budget)
+        self.packed = False
 
     def _params(self, u_in):
         p = self.case.params(self.precision)
@@ -67,6 +71,7 @@ class DragAdjoint:
         link = np.ascontiguousarray(self.case.link, dtype=np.int32).reshape(-1, 3)
         sol, h = self._handles(q, u_in)
         try:
+            abi.check(lib.octlb_adj_set_tape_budget(h, self.tape_budget, None), "adjoint tape budget")
             J = np.zeros(1)
             du = np.zeros(3)
             dq = np.zeros(max(len(link), 1))
@@ -81,6 +86,9 @@ class DragAdjoint:
             abi.check(lib.octlb_adj_inlet_grad(h, abi.f64p(gi)), "adjoint inlet gradient")
             self.inlet_grad = gi
             self.timing = (float(fw[0]), float(bw[0]))
+            packed = C.c_int32(0)
+            abi.check(lib.octlb_adj_set_tape_budget(h, self.tape_budget, C.byref(packed)), "adjoint tape budget")
+            self.packed = bool(packed.value)
             core = None
             if final_core:
                 core = np.empty(sol.nb * sol.ncore * 19)

@@ -424,7 +424,8 @@ __device__ __forceinline__ Origin CellOrigin(const double* ocw, int b) {
 template <typename S>
 __global__ __launch_bounds__(kThreads) void KMLCollide(const S* x, S* y, const Cell* cells, const uint8_t* ckind,
                                                        const int* cblock, const int* boff, const double* ocw,
-                                                       SimConst sc, double t, int c0, int c1) {
+                                                       const SimConst* scp, double t, int c0, int c1) {
+  const SimConst& sc = *scp;  // by pointer: a by-value copy would be staged in scratch per lane
   const int cl = c0 + static_cast<int>(blockIdx.x) * kThreads + static_cast<int>(threadIdx.x);
   if (cl >= c1) return;
   const Cell c = cells[cl];
@@ -507,10 +508,11 @@ template <typename S>
 __global__ __launch_bounds__(kThreads) void KMLVisitVjp(const double* lx, double* lxn, const S* xp, const int* rd,
                                                         const int* ovo, const int2* ove, const AdjLink* links,
                                                         const Cell* cells, const uint8_t* ckind, const int* cblock,
-                                                        const int* boff, const double* ocw, SimConst sc, double t,
+                                                        const int* boff, const double* ocw, const SimConst* scp, double t,
                                                         const int* cseed, const double* seed_w, double snap_scale,
                                                         int c0, int c1, double* dui) {
   __shared__ double red[3][kThreads];
+  const SimConst& sc = *scp;
   const int cl = c0 + static_cast<int>(blockIdx.x) * kThreads + static_cast<int>(threadIdx.x);
   double g[3] = {0.0, 0.0, 0.0};
   if (cl < c1) {
@@ -606,10 +608,11 @@ __device__ double MLPostCollide(const S* x, const Cell* cells, const uint8_t* ck
 template <typename S>
 __global__ __launch_bounds__(kThreads) void KMLLinkGrad(const double* lx, const S* xp, const Cell* cells,
                                                         const uint8_t* ckind, const int* cblock, const int* boff,
-                                                        const double* ocw, SimConst sc, double t,
+                                                        const double* ocw, const SimConst* scp, double t,
                                                         const AdjLink* links, int k0, int k1, double* dq) {
   const int k = k0 + static_cast<int>(blockIdx.x) * kThreads + static_cast<int>(threadIdx.x);
   if (k >= k1) return;
+  const SimConst& sc = *scp;
   const AdjLink L = links[k];
   const double fbb = MLPostCollide(xp, cells, ckind, cblock, boff, ocw, sc, t, L.bb_cell, L.bb_pop);
   const double fo = MLPostCollide(xp, cells, ckind, cblock, boff, ocw, sc, t, L.ot_cell, L.ot_pop);
@@ -659,16 +662,6 @@ __global__ __launch_bounds__(kAccumThreads) void KMLAccum(const double* part, in
     for (int d = 0; d < 3; ++d) acc[d] += red[0][d];
 }
 
-// copies the entries of the listed blocks (one workgroup per block)
-template <typename S>
-__global__ __launch_bounds__(256) void KMLCopyBlocks(S* dst, const S* src, const int* blocks, const int* boff,
-                                                     const int* cnt) {
-  const int b = blocks[blockIdx.x];
-  const int n = kQ * cnt[b];
-  for (int e = threadIdx.x; e < n; e += 256) dst[boff[b] + e] = src[boff[b] + e];
-}
-
-
 }  // namespace
 }  // namespace octlb
 
@@ -688,6 +681,9 @@ struct octlb_adj {
   std::unordered_map<int64_t, int> link_index;  // (block*512 + cell)*19 + pop -> Bouzidi row
   std::vector<int> border;                      // blocks in level order
   std::vector<int> lv_cell, lv_entry, lv_link;  // per level: [begin, end) of cells, entries, links
+  // multi-level: per operation, the window [win_lo, win_hi) of state entries
+  // its adjoint reads, and the windows' offsets in a packed per-step tape
+  std::vector<long long> win_lo, win_hi, win_off;
   std::vector<int> block_off, block_cnt;        // per block: first entry, plane length
   hipStream_t stream = nullptr;
   Cell* d_cells = nullptr;
@@ -709,6 +705,8 @@ struct octlb_adj {
   size_t selem = 8;
   std::vector<void*> owned;
   double fwd_ms = 0.0, bwd_ms = 0.0;
+  bool packed_mode = false;  // multi-level: the last run kept a segment's tapes for all its steps
+  long long tape_budget = -1;  // bytes allowed for a segment's packed tapes (-1: 60 % of the free memory)
 
   // ---- multi-level step (levels, FD cells, coupling); see KMLCollide
   enum MLOp { kOpVisit = 0, kOpFd = 1, kOpHalf = 2, kOpFull = 3, kOpRestrict = 4 };
@@ -1251,6 +1249,25 @@ struct octlb_adj {
       ops.push_back(make_int2(kOpRestrict, l + 1));
     };
     adv(0);
+    // what each operation's adjoint reads of its input state: a contiguous
+    // window of entries (its level; for the FD pass the span of the FD
+    // blocks), plus the prev macros for the prolongations
+    long long fd_lo = 0, fd_hi = 0;
+    for (size_t k = 0; k < m.fd_blocks.size(); ++k) {
+      const int b = m.fd_blocks[k];
+      const long long lo = block_off[b], hi = lo + static_cast<long long>(kQ) * block_cnt[b];
+      fd_lo = k == 0 ? lo : std::min(fd_lo, lo);
+      fd_hi = k == 0 ? hi : std::max(fd_hi, hi);
+    }
+    win_lo.assign(ops.size(), 0);
+    win_hi.assign(ops.size(), 0);
+    win_off.assign(ops.size() + 1, 0);
+    for (size_t k = 0; k < ops.size(); ++k) {
+      const int2 op = ops[k];
+      win_lo[k] = op.x == kOpFd ? fd_lo : lv_entry[op.y];
+      win_hi[k] = op.x == kOpFd ? fd_hi : lv_entry[op.y + 1];
+      win_off[k + 1] = win_off[k] + (win_hi[k] - win_lo[k]);
+    }
   }
 
   template <typename S>
@@ -1265,46 +1282,38 @@ struct octlb_adj {
     return a;
   }
 
-  // one operation forward on (x, y, prev) at time t; when tape is given, the
-  // state the operation's adjoint needs is copied there first
+  // operation k forward on (x, y, prev) at time t; when tape is given (in
+  // state coordinates: entry e of the window at tape[e]), the window of the
+  // state that the operation's adjoint reads is copied there first, and for a
+  // prolongation the prev macros into ptape
   template <typename S>
-  void MLApply(int2 op, S* x, S* y, S* prev, double t, S* tape, S* ptape) {
+  void MLApply(size_t k, S* x, S* y, S* prev, double t, S* tape, S* ptape) {
     const size_t pm = static_cast<size_t>(npts) * kMacro;
+    const int2 op = ops[k];
     const int l = op.y;
+    if (tape && win_hi[k] > win_lo[k])
+      ADJ_CHECK(hipMemcpyAsync(tape + win_lo[k], x + win_lo[k], static_cast<size_t>(win_hi[k] - win_lo[k]) * sizeof(S),
+                               hipMemcpyDeviceToDevice, stream));
     switch (op.x) {
       case kOpVisit: {
         const int c0 = lv_cell[l], c1 = lv_cell[l + 1];
         if (c1 <= c0) break;
-        if (tape)
-          ADJ_CHECK(hipMemcpyAsync(tape + lv_entry[l], x + lv_entry[l],
-                                   static_cast<size_t>(lv_entry[l + 1] - lv_entry[l]) * sizeof(S),
-                                   hipMemcpyDeviceToDevice, stream));
         hipLaunchKernelGGL(KMLCollide<S>, dim3(Blocks(c1 - c0)), dim3(kThreads), 0, stream, x, y, d_cells, d_ckind,
-                           d_cblock, d_boff, d_ocw, mtopo.sc, t, c0, c1);
+                           d_cblock, d_boff, d_ocw, d_scm, t, c0, c1);
         hipLaunchKernelGGL(KMLGather<S>, dim3(Blocks(c1 - c0)), dim3(kThreads), 0, stream, y, x, d_src, d_links,
                            d_cells, c0, c1);
         break;
       }
       case kOpFd:
-        if (tape) hipLaunchKernelGGL(KMLCopyBlocks<S>, dim3(nfd), dim3(256), 0, stream, tape, x, d_fd_blocks, d_boff, d_cnt);
         LaunchMLFd<S>(MLFdArgs<S>(), d_fd_blocks, d_fd_slot, nfd, x, d_boff, d_cnt, t, stream);
         break;
       case kOpHalf:
       case kOpFull:
-        if (tape) {
-          ADJ_CHECK(hipMemcpyAsync(tape + lv_entry[l], x + lv_entry[l],
-                                   static_cast<size_t>(lv_entry[l + 1] - lv_entry[l]) * sizeof(S),
-                                   hipMemcpyDeviceToDevice, stream));
-          if (pm) ADJ_CHECK(hipMemcpyAsync(ptape, prev, pm * sizeof(S), hipMemcpyDeviceToDevice, stream));
-        }
+        if (tape && pm) ADJ_CHECK(hipMemcpyAsync(ptape, prev, pm * sizeof(S), hipMemcpyDeviceToDevice, stream));
         LaunchMLProlong<S>(d_plan, d_plist[l], np_l[l], op.x == kOpHalf ? 1 : 0, x, d_boff, d_cnt, prev,
                            mtopo.prolong_scale, stream);
         break;
       case kOpRestrict:
-        if (tape)
-          ADJ_CHECK(hipMemcpyAsync(tape + lv_entry[l], x + lv_entry[l],
-                                   static_cast<size_t>(lv_entry[l + 1] - lv_entry[l]) * sizeof(S),
-                                   hipMemcpyDeviceToDevice, stream));
         LaunchMLRestrict<S>(d_plan, d_rlist[l], nr_l[l], x, d_boff, d_cnt, mtopo.restrict_scale, stream);
         break;
     }
@@ -1323,10 +1332,10 @@ struct octlb_adj {
         const int k0 = lv_link[l], k1 = lv_link[l + 1];
         if (k1 > k0)
           hipLaunchKernelGGL(KMLLinkGrad<S>, dim3(Blocks(k1 - k0)), dim3(kThreads), 0, stream, d_lx, tape, d_cells,
-                             d_ckind, d_cblock, d_boff, d_ocw, mtopo.sc, t, d_links, k0, k1, d_dq);
+                             d_ckind, d_cblock, d_boff, d_ocw, d_scm, t, d_links, k0, k1, d_dq);
         const unsigned nwg = Blocks(c1 - c0);
         hipLaunchKernelGGL(KMLVisitVjp<S>, dim3(nwg), dim3(kThreads), 0, stream, d_lx, d_ln, tape, d_rd, d_ovo,
-                           d_ove, d_links, d_cells, d_ckind, d_cblock, d_boff, d_ocw, mtopo.sc, t, cseed, seed_w,
+                           d_ove, d_links, d_cells, d_ckind, d_cblock, d_boff, d_ocw, d_scm, t, cseed, seed_w,
                            snap_scale, c0, c1, d_dui_part);
         ADJ_CHECK(hipMemcpyAsync(d_lx + lv_entry[l], d_ln + lv_entry[l],
                                  static_cast<size_t>(lv_entry[l + 1] - lv_entry[l]) * sizeof(double),
@@ -1390,11 +1399,34 @@ struct octlb_adj {
       S* pv = static_cast<S*>(alloc(pmn * sizeof(S)));
       S* ck = static_cast<S*>(alloc(static_cast<size_t>(nck) * st * sizeof(S)));
       S* ckp = static_cast<S*>(alloc(static_cast<size_t>(nck) * pmn * sizeof(S)));
-      S* seg = static_cast<S*>(alloc(static_cast<size_t>(K) * st * sizeof(S)));
-      S* segp = static_cast<S*>(alloc(static_cast<size_t>(K) * pmn * sizeof(S)));
       const size_t nops = ops.size();
-      S* tape = static_cast<S*>(alloc(nops * st * sizeof(S)));
-      S* ptape = static_cast<S*>(alloc(nops * pmn * sizeof(S)));
+      // Reverse sweep, per checkpoint segment.  Packed mode (when K steps of
+      // tapes fit in a share of the free device memory): one forward pass
+      // over the segment writes every step's tapes, then the steps are
+      // reversed.  Otherwise the segment's states are recomputed first and
+      // each step is re-run from its state to tape it just before its reverse.
+      const size_t step_tape = static_cast<size_t>(win_off[nops]);
+      const size_t step_ptape = nops * pmn;
+      size_t free_b = 0, total_b = 0;
+      ADJ_CHECK(hipMemGetInfo(&free_b, &total_b));
+      const double packed_bytes =
+          static_cast<double>(K) * static_cast<double>(step_tape + step_ptape) * static_cast<double>(sizeof(S));
+      const double budget = tape_budget < 0 ? 0.6 * static_cast<double>(free_b) : static_cast<double>(tape_budget);
+      const bool packed = packed_bytes <= budget;
+      S* seg = packed ? nullptr : static_cast<S*>(alloc(static_cast<size_t>(K) * st * sizeof(S)));
+      S* segp = packed ? nullptr : static_cast<S*>(alloc(static_cast<size_t>(K) * pmn * sizeof(S)));
+      S* tape = static_cast<S*>(alloc((packed ? static_cast<size_t>(K) * step_tape : nops * st) * sizeof(S)));
+      S* ptape = static_cast<S*>(alloc((packed ? static_cast<size_t>(K) : 1) * step_ptape * sizeof(S)));
+      // tape of operation k at step j of the segment, in state coordinates
+      auto tape_of = [&](int j, size_t k) -> S* {
+        if (!packed) return tape + k * st;
+        const uintptr_t at = reinterpret_cast<uintptr_t>(tape + static_cast<size_t>(j) * step_tape + win_off[k]);
+        return reinterpret_cast<S*>(at - static_cast<uintptr_t>(win_lo[k]) * sizeof(S));
+      };
+      auto ptape_of = [&](int j, size_t k) -> S* {
+        return ptape + (packed ? static_cast<size_t>(j) * step_ptape : 0) + k * pmn;
+      };
+      packed_mode = packed;
       double* lp = static_cast<double*>(alloc(pmn * sizeof(double)));
       const unsigned ncb = Blocks(ncells);
       double* d_part = static_cast<double*>(alloc(static_cast<size_t>(ncb) * 3 * sizeof(double)));
@@ -1413,7 +1445,7 @@ struct octlb_adj {
       for (auto& e : ev) ADJ_CHECK(hipEventCreate(&e));
       ADJ_CHECK(hipEventRecord(ev[0], stream));
       auto step_fwd = [&](S* xs, S* ps, double t) {
-        for (const int2 op : ops) MLApply<S>(op, xs, y, ps, t, nullptr, nullptr);
+        for (size_t k = 0; k < ops.size(); ++k) MLApply<S>(k, xs, y, ps, t, nullptr, nullptr);
       };
       // ---- forward sweep: initial state, coupler init, checkpoints, J
       hipLaunchKernelGGL(KAdjInit<S>, dim3(ncb), dim3(kThreads), 0, stream, x, d_cells, d_ckind, ncells, rho0, u0[0],
@@ -1439,8 +1471,35 @@ struct octlb_adj {
       ADJ_CHECK(hipMemsetAsync(d_lx, 0, st * sizeof(double), stream));
       ADJ_CHECK(hipMemsetAsync(d_ln, 0, st * sizeof(double), stream));
       ADJ_CHECK(hipMemsetAsync(lp, 0, pmn * sizeof(double), stream));
+      auto reverse_step = [&](int s, int j) {
+        const double t = static_cast<double>(s - 1);
+        const bool win = s > w0 && s <= w1 && nterms > 0;
+        if (win) hipLaunchKernelGGL(KAdjSeedLive, dim3(Blocks(nterms)), dim3(kThreads), 0, stream, d_lx, d_terms,
+                                    nterms, scale);
+        std::vector<char> seeded(nlev, 0);
+        for (size_t k = nops; k-- > 0;) {
+          double ss = 0.0;
+          if (ops[k].x == kOpVisit && win && !seeded[ops[k].y]) {
+            ss = scale;  // the level's last visit took the drag snapshot
+            seeded[ops[k].y] = 1;
+          }
+          MLReverse<S>(ops[k], t, tape_of(j, k), ptape_of(j, k), lp, ss, d_cseed, d_seed_w);
+        }
+      };
       for (int c = nck - 1; c >= 0; --c) {
         const int s0 = c * K, s1 = std::min(steps, s0 + K);
+        if (packed) {
+          // one pass over the segment from its checkpoint, taping every step
+          ADJ_CHECK(hipMemcpyAsync(x, ck + static_cast<size_t>(c) * st, st * sizeof(S), hipMemcpyDeviceToDevice, stream));
+          if (pm)
+            ADJ_CHECK(hipMemcpyAsync(pv, ckp + static_cast<size_t>(c) * pmn, pm * sizeof(S), hipMemcpyDeviceToDevice,
+                                     stream));
+          for (int s = s0 + 1; s <= s1; ++s)
+            for (size_t k = 0; k < nops; ++k)
+              MLApply<S>(k, x, y, pv, static_cast<double>(s - 1), tape_of(s - 1 - s0, k), ptape_of(s - 1 - s0, k));
+          for (int s = s1; s > s0; --s) reverse_step(s, s - 1 - s0);
+          continue;
+        }
         // segment states x_{s0} .. x_{s1 - 1} (and their prev macros)
         ADJ_CHECK(hipMemcpyAsync(seg, ck + static_cast<size_t>(c) * st, st * sizeof(S), hipMemcpyDeviceToDevice, stream));
         if (pm)
@@ -1456,7 +1515,6 @@ struct octlb_adj {
                    static_cast<double>(s0 + j - 1));
         }
         for (int s = s1; s > s0; --s) {
-          const double t = static_cast<double>(s - 1);
           // re-run step s from x_{s-1}, taping what each operation's adjoint reads
           ADJ_CHECK(hipMemcpyAsync(x, seg + static_cast<size_t>(s - 1 - s0) * st, st * sizeof(S),
                                    hipMemcpyDeviceToDevice, stream));
@@ -1464,19 +1522,8 @@ struct octlb_adj {
             ADJ_CHECK(hipMemcpyAsync(pv, segp + static_cast<size_t>(s - 1 - s0) * pmn, pm * sizeof(S),
                                      hipMemcpyDeviceToDevice, stream));
           for (size_t k = 0; k < nops; ++k)
-            MLApply<S>(ops[k], x, y, pv, t, tape + k * st, ptape + k * pmn);
-          const bool win = s > w0 && s <= w1 && nterms > 0;
-          if (win) hipLaunchKernelGGL(KAdjSeedLive, dim3(Blocks(nterms)), dim3(kThreads), 0, stream, d_lx, d_terms,
-                                      nterms, scale);
-          std::vector<char> seeded(nlev, 0);
-          for (size_t k = nops; k-- > 0;) {
-            double ss = 0.0;
-            if (ops[k].x == kOpVisit && win && !seeded[ops[k].y]) {
-              ss = scale;  // the level's last visit took the drag snapshot
-              seeded[ops[k].y] = 1;
-            }
-            MLReverse<S>(ops[k], t, tape + k * st, ptape + k * pmn, lp, ss, d_cseed, d_seed_w);
-          }
+            MLApply<S>(k, x, y, pv, static_cast<double>(s - 1), tape_of(0, k), ptape_of(0, k));
+          reverse_step(s, 0);
         }
       }
       // the initial prev macros came from the initial state
@@ -1775,6 +1822,15 @@ int octlb_adj_timing(const octlb_adj* adj, double* forward_ms, double* reverse_m
     if (!adj) throw std::invalid_argument("null argument");
     if (forward_ms) *forward_ms = adj->fwd_ms;
     if (reverse_ms) *reverse_ms = adj->bwd_ms;
+  });
+}
+
+int octlb_adj_set_tape_budget(octlb_adj* adj, int64_t bytes, int32_t* packed_last_run) {
+  return AdjGuard([&] {
+    if (!adj) throw std::invalid_argument("null argument");
+    if (bytes < -1) throw std::invalid_argument("tape budget must be -1 (automatic) or >= 0 bytes");
+    adj->tape_budget = bytes;
+    if (packed_last_run) *packed_last_run = adj->packed_mode ? 1 : 0;
   });
 }
 

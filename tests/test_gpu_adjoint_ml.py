@@ -11,7 +11,9 @@ coarse steps 13..24, checkpoint every 5 steps, inlet ramp over 8 steps.
   FD pass) match central differences of the FP64 oracle's forward
   (eps = 1e-6) to 1e-4: u_in's x component, the three largest |dJ/dq|, and
   a random direction over all links.
-* The gradients do not depend on the checkpoint interval.
+* The gradients do not depend on the checkpoint interval, nor on whether a
+  segment's tapes are kept for all its steps (the default here) or each step
+  is re-run just before its reverse (a zero tape budget).
 """
 import numpy as np
 import pytest
@@ -63,6 +65,7 @@ def device(channel):
     print("multi-level adjoint: 72 blocks, 24 coarse steps, checkpoint 5", flush=True)
     adj = DragAdjoint(channel, STEPS, WIN, checkpoint=CK)
     J, du0, dq, core = adj.evaluate(final_core=True)
+    assert adj.packed, "the default tape budget should hold this case's segments"
     return J, dq, adj.inlet_grad.copy(), core
 
 
@@ -119,6 +122,16 @@ def test_checkpoint_interval_does_not_change_the_gradients(channel, device):
     J, dq, gi, _ = device
     adj = DragAdjoint(channel, STEPS, WIN, checkpoint=STEPS)
     J2, _, dq2 = adj.evaluate()
+    assert J2 == J
+    np.testing.assert_array_equal(dq2, dq)
+    np.testing.assert_array_equal(adj.inlet_grad, gi)
+
+
+def test_recompute_mode_matches_the_packed_tapes(channel, device):
+    J, dq, gi, _ = device
+    adj = DragAdjoint(channel, STEPS, WIN, checkpoint=CK, tape_budget=0)
+    J2, _, dq2 = adj.evaluate()
+    assert not adj.packed
     assert J2 == J
     np.testing.assert_array_equal(dq2, dq)
     np.testing.assert_array_equal(adj.inlet_grad, gi)
