@@ -507,67 +507,82 @@ __device__ inline void VelocityCollideVjp(const double* f, double* l, double ome
 template <typename S>
 __global__ __launch_bounds__(kThreads) void KMLVisitVjp(const double* lx, double* lxn, const S* xp, const int* rd,
                                                         const int* ovo, const int2* ove, const AdjLink* links,
-                                                        const Cell* cells, const uint8_t* ckind, const int* cblock,
-                                                        const int* boff, const double* ocw, const SimConst* scp, double t,
-                                                        const int* cseed, const double* seed_w, double snap_scale,
-                                                        int c0, int c1, double* dui) {
+                                                        const Cell* cells, const uint8_t* ckind,
+                                                        const SimConst* scp, const int* cseed, const double* seed_w,
+                                                        double snap_scale, int c0, int c1) {
+  const int cl = c0 + static_cast<int>(blockIdx.x) * kThreads + static_cast<int>(threadIdx.x);
+  if (cl >= c1) return;
+  const Cell c = cells[cl];
+  double l[kQ];
+#pragma unroll
+  for (int i = 0; i < kQ; ++i) {
+    const int r = rd[c.base + i * c.stride];
+    double v = 0.0;
+    if (r >= 0) {
+      v = lx[r];
+    } else if (r < -1) {
+      const int k = -2 - r;
+      for (int j = ovo[k]; j < ovo[k + 1]; ++j) {
+        const int2 o = ove[j];
+        v += (o.y < 0 ? 1.0 : ReaderCoef(links[o.y >> 1], o.y & 1)) * lx[o.x];
+      }
+    }
+    l[i] = v;
+  }
+  const int sk = cseed[cl];
+  if (sk >= 0 && snap_scale != 0.0) {
+#pragma unroll
+    for (int i = 0; i < kQ; ++i) l[i] += snap_scale * seed_w[static_cast<size_t>(sk) * 2 * kQ + i];
+  }
+  const uint8_t k = ckind[cl];
+  if (k == kBulk) {
+    double f[kQ];
+    LoadCell(xp, c, f);
+    BgkVjp(f, l, scp->omega);
+  } else if (k == kBounceBack) {
+    double tmp[kQ];
+#pragma unroll
+    for (int i = 0; i < kQ; ++i) tmp[i] = l[i];
+#pragma unroll
+    for (int i = 0; i < kQ; ++i) l[i] = tmp[Opp(i)];
+  }
+  // velocity cells: KMLVelocityVjp takes their collide next
+#pragma unroll
+  for (int i = 0; i < kQ; ++i) lxn[c.base + i * c.stride] = l[i];
+}
+
+// The velocity-Dirichlet collide VJP of one level's velocity cells (list
+// vc[0, nv)), in place on the adjoint KMLVisitVjp left for them; d/d(inlet
+// velocity) as one partial per workgroup (fixed tree) in dui.
+template <typename S>
+__global__ __launch_bounds__(kThreads) void KMLVelocityVjp(double* lxn, const S* xp, const int* vc, int nv,
+                                                           const Cell* cells, const int* cblock, const int* boff,
+                                                           const double* ocw, const SimConst* scp, double t,
+                                                           double* dui) {
   __shared__ double red[3][kThreads];
   const SimConst& sc = *scp;
-  const int cl = c0 + static_cast<int>(blockIdx.x) * kThreads + static_cast<int>(threadIdx.x);
+  const int j = static_cast<int>(blockIdx.x) * kThreads + static_cast<int>(threadIdx.x);
   double g[3] = {0.0, 0.0, 0.0};
-  if (cl < c1) {
+  if (j < nv) {
+    const int cl = vc[j];
     const Cell c = cells[cl];
-    double l[kQ];
-#pragma unroll
-    for (int i = 0; i < kQ; ++i) {
-      const int r = rd[c.base + i * c.stride];
-      double v = 0.0;
-      if (r >= 0) {
-        v = lx[r];
-      } else if (r < -1) {
-        const int k = -2 - r;
-        for (int j = ovo[k]; j < ovo[k + 1]; ++j) {
-          const int2 o = ove[j];
-          v += (o.y < 0 ? 1.0 : ReaderCoef(links[o.y >> 1], o.y & 1)) * lx[o.x];
-        }
-      }
-      l[i] = v;
-    }
-    const int sk = cseed[cl];
-    if (sk >= 0 && snap_scale != 0.0) {
-#pragma unroll
-      for (int i = 0; i < kQ; ++i) l[i] += snap_scale * seed_w[static_cast<size_t>(sk) * 2 * kQ + i];
-    }
-    const uint8_t k = ckind[cl];
-    if (k == kBulk) {
-      double f[kQ];
-      LoadCell(xp, c, f);
-      BgkVjp(f, l, sc.omega);
-    } else if (k == kBounceBack) {
-      double tmp[kQ];
-#pragma unroll
-      for (int i = 0; i < kQ; ++i) tmp[i] = l[i];
-#pragma unroll
-      for (int i = 0; i < kQ; ++i) l[i] = tmp[Opp(i)];
-    } else if (k == kVelocity) {
-      double f[kQ];
-      LoadCell(xp, c, f);
-      const int b = cblock[cl];
-      const int rank = c.base - boff[b];
-      const int ix = rank >> 6, iy = (rank >> 3) & 7, iz = rank & 7;
-      const Origin og = CellOrigin(ocw, b);
-      const double org[3] = {og.x, og.y, og.z};
-      double ud[3];
-      PrescribedU(sc, org, og.cw, ix, iy, iz, t, ud);
-      // the forward rounds the prescribed velocity to the state's precision
-      const double u[3] = {static_cast<double>(static_cast<S>(ud[0])), static_cast<double>(static_cast<S>(ud[1])),
-                           static_cast<double>(static_cast<S>(ud[2]))};
-      double du[3];
-      VelocityCollideVjp(f, l, sc.omega, u, kN, ix, iy, iz, du);
-      const double fct = InletFactor(sc, org, og.cw, ix, iy, iz, t);
-      for (int d = 0; d < 3; ++d) g[d] = du[d] * fct;
-    }
-#pragma unroll
+    double l[kQ], f[kQ];
+    for (int i = 0; i < kQ; ++i) l[i] = lxn[c.base + i * c.stride];
+    LoadCell(xp, c, f);
+    const int b = cblock[cl];
+    const int rank = c.base - boff[b];
+    const int ix = rank >> 6, iy = (rank >> 3) & 7, iz = rank & 7;
+    const Origin og = CellOrigin(ocw, b);
+    const double org[3] = {og.x, og.y, og.z};
+    double ud[3];
+    PrescribedU(sc, org, og.cw, ix, iy, iz, t, ud);
+    // the forward rounds the prescribed velocity to the state's precision
+    const double u[3] = {static_cast<double>(static_cast<S>(ud[0])), static_cast<double>(static_cast<S>(ud[1])),
+                         static_cast<double>(static_cast<S>(ud[2]))};
+    double du[3];
+    VelocityCollideVjp(f, l, sc.omega, u, kN, ix, iy, iz, du);
+    const double fct = InletFactor(sc, org, og.cw, ix, iy, iz, t);
+    for (int d = 0; d < 3; ++d) g[d] = du[d] * fct;
     for (int i = 0; i < kQ; ++i) lxn[c.base + i * c.stride] = l[i];
   }
   for (int d = 0; d < 3; ++d) red[d][threadIdx.x] = g[d];
@@ -684,8 +699,18 @@ struct octlb_adj {
   // multi-level: per operation, the window [win_lo, win_hi) of state entries
   // its adjoint reads, and the windows' offsets in a packed per-step tape
   std::vector<long long> win_lo, win_hi, win_off;
+  std::vector<int> lv_vcell;  // per level: [begin, end) in d_vcell of its velocity-Dirichlet cells
+  int* d_vcell = nullptr;
+  double* d_pcm = nullptr;
+  unsigned fd_level_mask = 0;      // levels holding FD blocks
+  double* d_dui_fd = nullptr;      // d/d(inlet velocity) from the FD VJPs (their own accumulator)
+  double* d_dui_fd_part = nullptr;
   std::vector<int> block_off, block_cnt;        // per block: first entry, plane length
   hipStream_t stream = nullptr;
+  // multi-level: the FD VJP runs on `side` beside a level visit's VJP when
+  // they touch different levels (fork / join events)
+  hipStream_t side = nullptr;
+  hipEvent_t ev_fork = nullptr, ev_join = nullptr;
   Cell* d_cells = nullptr;
   int* d_cblock = nullptr;
   uint8_t* d_cflag = nullptr;
@@ -705,6 +730,7 @@ struct octlb_adj {
   size_t selem = 8;
   std::vector<void*> owned;
   double fwd_ms = 0.0, bwd_ms = 0.0;
+  bool fd_side = false;  // MLReverse: launch the FD VJP on the side stream
   bool packed_mode = false;  // multi-level: the last run kept a segment's tapes for all its steps
   long long tape_budget = -1;  // bytes allowed for a segment's packed tapes (-1: 60 % of the free memory)
 
@@ -748,7 +774,12 @@ struct octlb_adj {
   void FreeAll() {
     for (void* p : owned) (void)hipFree(p);  // teardown: nothing to report to
     owned.clear();
+    if (ev_fork) (void)hipEventDestroy(ev_fork);
+    if (ev_join) (void)hipEventDestroy(ev_join);
+    if (side) (void)hipStreamDestroy(side);
     if (stream) (void)hipStreamDestroy(stream);
+    ev_fork = ev_join = nullptr;
+    side = nullptr;
     stream = nullptr;
   }
 
@@ -1182,6 +1213,11 @@ struct octlb_adj {
       d_pbeg[l] = Upload(beg);
     }
     {
+      int most = 1;
+      for (int l = 0; l < nlev; ++l) most = std::max(most, np_l[l]);
+      d_pcm = Alloc<double>(static_cast<size_t>(most) * kMacro);  // per-point macro adjoints of a prolongation VJP
+    }
+    {
       std::vector<int> all(npts), pts, beg;
       for (int p = 0; p < npts; ++p) all[p] = p;
       nig = group(all, &pts, &beg);
@@ -1235,6 +1271,25 @@ struct octlb_adj {
     }
     d_rbuf = Alloc<double>(rmax * kQ);
     d_dui = Alloc<double>(3);
+    d_dui_fd = Alloc<double>(3);
+    d_dui_fd_part = Alloc<double>(3 * static_cast<size_t>(std::max(nfd, 1)));
+    fd_level_mask = 0;
+    for (int b : m.fd_blocks) fd_level_mask |= 1u << m.level[b];
+    ADJ_CHECK(hipStreamCreateWithFlags(&side, hipStreamNonBlocking));
+    ADJ_CHECK(hipEventCreateWithFlags(&ev_fork, hipEventDisableTiming));
+    ADJ_CHECK(hipEventCreateWithFlags(&ev_join, hipEventDisableTiming));
+    // per level, its velocity-Dirichlet cells (their collide VJP runs apart)
+    {
+      std::vector<int> vc;
+      lv_vcell.assign(nlev + 1, 0);
+      for (int l = 0; l < nlev; ++l) {
+        lv_vcell[l] = static_cast<int>(vc.size());
+        for (int cl = lv_cell[l]; cl < lv_cell[l + 1]; ++cl)
+          if (ckind[cl] == kVelocity) vc.push_back(cl);
+      }
+      lv_vcell[nlev] = static_cast<int>(vc.size());
+      d_vcell = Upload(vc);
+    }
     d_dui_part = Alloc<double>(3 * static_cast<size_t>(std::max<long long>(Blocks(ncells), std::max(nfd, 1))));
     // the coarse step as operations (TimeLoop::advance, time_loop.h:201-240)
     ops.clear();
@@ -1333,26 +1388,32 @@ struct octlb_adj {
         if (k1 > k0)
           hipLaunchKernelGGL(KMLLinkGrad<S>, dim3(Blocks(k1 - k0)), dim3(kThreads), 0, stream, d_lx, tape, d_cells,
                              d_ckind, d_cblock, d_boff, d_ocw, d_scm, t, d_links, k0, k1, d_dq);
-        const unsigned nwg = Blocks(c1 - c0);
-        hipLaunchKernelGGL(KMLVisitVjp<S>, dim3(nwg), dim3(kThreads), 0, stream, d_lx, d_ln, tape, d_rd, d_ovo,
-                           d_ove, d_links, d_cells, d_ckind, d_cblock, d_boff, d_ocw, d_scm, t, cseed, seed_w,
-                           snap_scale, c0, c1, d_dui_part);
+        hipLaunchKernelGGL(KMLVisitVjp<S>, dim3(Blocks(c1 - c0)), dim3(kThreads), 0, stream, d_lx, d_ln, tape, d_rd,
+                           d_ovo, d_ove, d_links, d_cells, d_ckind, d_scm, cseed, seed_w, snap_scale, c0, c1);
+        const int nv = lv_vcell[l + 1] - lv_vcell[l];
+        if (nv > 0) {
+          const unsigned nwg = Blocks(nv);
+          hipLaunchKernelGGL(KMLVelocityVjp<S>, dim3(nwg), dim3(kThreads), 0, stream, d_ln, tape,
+                             d_vcell + lv_vcell[l], nv, d_cells, d_cblock, d_boff, d_ocw, d_scm, t, d_dui_part);
+          hipLaunchKernelGGL(KMLAccum, dim3(1), dim3(kAccumThreads), 0, stream, d_dui_part, static_cast<int>(nwg),
+                             d_dui);
+        }
         ADJ_CHECK(hipMemcpyAsync(d_lx + lv_entry[l], d_ln + lv_entry[l],
                                  static_cast<size_t>(lv_entry[l + 1] - lv_entry[l]) * sizeof(double),
                                  hipMemcpyDeviceToDevice, stream));
-        hipLaunchKernelGGL(KMLAccum, dim3(1), dim3(kAccumThreads), 0, stream, d_dui_part, static_cast<int>(nwg),
-                           d_dui);
         break;
       }
-      case kOpFd:
+      case kOpFd: {
+        hipStream_t fs = fd_side ? side : stream;
         LaunchMLFdVjp<S>(MLFdArgs<S>(), d_fd_blocks, d_fd_slot, nfd, tape, d_lx, d_boff, d_cnt, t, d_fd_rbeg,
-                         d_fd_rent, d_dui_part, stream);
-        hipLaunchKernelGGL(KMLAccum, dim3(1), dim3(kAccumThreads), 0, stream, d_dui_part, nfd, d_dui);
+                         d_fd_rent, d_dui_fd_part, fs);
+        hipLaunchKernelGGL(KMLAccum, dim3(1), dim3(kAccumThreads), 0, fs, d_dui_fd_part, nfd, d_dui_fd);
         break;
+      }
       case kOpHalf:
       case kOpFull:
-        LaunchMLProlongVjp<S>(d_plan, d_ppts[l], d_pbeg[l], npg_l[l], op.x == kOpHalf ? 1 : 0, tape, d_lx, d_boff,
-                              d_cnt, ptape, lp, mtopo.prolong_scale, stream);
+        LaunchMLProlongVjp<S>(d_plan, d_ppts[l], d_pbeg[l], npg_l[l], np_l[l], op.x == kOpHalf ? 1 : 0, tape, d_lx,
+                              d_boff, d_cnt, ptape, lp, mtopo.prolong_scale, d_pcm, stream);
         break;
       case kOpRestrict:
         LaunchMLRestrictTake(d_plan, d_rlist[l], nr_l[l], d_lx, d_boff, d_cnt, d_rbuf, stream);
@@ -1439,6 +1500,7 @@ struct octlb_adj {
       ADJ_CHECK(hipMemsetAsync(d_jv, 0, static_cast<size_t>(steps + 1) * ndg * sizeof(double), stream));
       ADJ_CHECK(hipMemsetAsync(d_dq, 0, static_cast<size_t>(nlinks > 0 ? nlinks : 1) * sizeof(double), stream));
       ADJ_CHECK(hipMemsetAsync(d_dui, 0, 3 * sizeof(double), stream));
+      if (d_dui_fd) ADJ_CHECK(hipMemsetAsync(d_dui_fd, 0, 3 * sizeof(double), stream));
       ADJ_CHECK(hipMemsetAsync(x, 0, st * sizeof(S), stream));
       ADJ_CHECK(hipMemsetAsync(y, 0, st * sizeof(S), stream));
       hipEvent_t ev[3];
@@ -1477,13 +1539,31 @@ struct octlb_adj {
         if (win) hipLaunchKernelGGL(KAdjSeedLive, dim3(Blocks(nterms)), dim3(kThreads), 0, stream, d_lx, d_terms,
                                     nterms, scale);
         std::vector<char> seeded(nlev, 0);
+        bool joined = true;
         for (size_t k = nops; k-- > 0;) {
           double ss = 0.0;
           if (ops[k].x == kOpVisit && win && !seeded[ops[k].y]) {
             ss = scale;  // the level's last visit took the drag snapshot
             seeded[ops[k].y] = 1;
           }
+          // an FD VJP followed (in reverse order) by a visit of a level without
+          // FD blocks: the two touch disjoint adjoint entries and run side by side
+          const bool fork = ops[k].x == kOpFd && k > 0 && ops[k - 1].x == kOpVisit &&
+                            !((fd_level_mask >> ops[k - 1].y) & 1u);
+          if (fork) {
+            ADJ_CHECK(hipEventRecord(ev_fork, stream));
+            ADJ_CHECK(hipStreamWaitEvent(side, ev_fork, 0));
+            fd_side = true;
+          }
           MLReverse<S>(ops[k], t, tape_of(j, k), ptape_of(j, k), lp, ss, d_cseed, d_seed_w);
+          fd_side = false;
+          if (fork) {
+            ADJ_CHECK(hipEventRecord(ev_join, side));
+            joined = false;
+          } else if (!joined) {
+            ADJ_CHECK(hipStreamWaitEvent(stream, ev_join, 0));
+            joined = true;
+          }
         }
       };
       for (int c = nck - 1; c >= 0; --c) {
@@ -1535,8 +1615,11 @@ struct octlb_adj {
       std::vector<double> jv(static_cast<size_t>(steps + 1) * ndg), part(static_cast<size_t>(ncb) * 3);
       ADJ_CHECK(hipMemcpyAsync(jv.data(), d_jv, jv.size() * sizeof(double), hipMemcpyDeviceToHost, stream));
       ADJ_CHECK(hipMemcpyAsync(part.data(), d_part, part.size() * sizeof(double), hipMemcpyDeviceToHost, stream));
+      double dui_fd[3] = {0.0, 0.0, 0.0};
       ADJ_CHECK(hipMemcpyAsync(dui, d_dui, 3 * sizeof(double), hipMemcpyDeviceToHost, stream));
+      if (d_dui_fd) ADJ_CHECK(hipMemcpyAsync(dui_fd, d_dui_fd, 3 * sizeof(double), hipMemcpyDeviceToHost, stream));
       ADJ_CHECK(hipStreamSynchronize(stream));
+      for (int d = 0; d < 3; ++d) dui[d] += dui_fd[d];
       float a = 0.f, b = 0.f;
       ADJ_CHECK(hipEventElapsedTime(&a, ev[0], ev[1]));
       ADJ_CHECK(hipEventElapsedTime(&b, ev[1], ev[2]));

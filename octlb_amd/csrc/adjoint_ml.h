@@ -41,9 +41,9 @@ template <typename S>
 void LaunchMLRestrict(const int* plan, const int* list, int count, S* x, const int* boff, const int* cnt,
                       double scale, hipStream_t s);
 template <typename S>
-void LaunchMLProlongVjp(const int* plan, const int* pts, const int* pbeg, int ngroups, int half, const S* xc,
-                        double* lam, const int* boff, const int* cnt, const S* prev, double* lprev, double scale,
-                        hipStream_t s);
+void LaunchMLProlongVjp(const int* plan, const int* pts, const int* pbeg, int ngroups, int count, int half,
+                        const S* xc, double* lam, const int* boff, const int* cnt, const S* prev, double* lprev,
+                        double scale, double* cm, hipStream_t s);
 template <typename S>
 void LaunchMLCouplerInitVjp(const int* plan, const int* pts, const int* pbeg, int ngroups, const S* x, double* lam,
                             const int* boff, const int* cnt, const double* lprev, hipStream_t s);
