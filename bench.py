@@ -62,13 +62,25 @@ def traffic_bytes(workload, precision):
     return round(kib * 1024.0)
 
 
-def run_gpu(case, precision, steps, warmup, timing_steps):
+SCHEMES = {"auto": None, "pull": 0, "push": 1}
+
+
+def choose_scheme(sim, scheme, reps):
+    """The fused kernel's update scheme: the autotuner's choice, or the one
+    asked for (the autotuner's timings are then not taken)."""
+    if SCHEMES[scheme] is None:
+        return sim.autotune(reps=reps)
+    sim.set_kernel_variant(SCHEMES[scheme])
+    return SCHEMES[scheme], None
+
+
+def run_gpu(case, precision, steps, warmup, timing_steps, scheme="auto"):
     from octlb_amd.solver import GpuSolver
 
     sim = GpuSolver(case.topo, case.params(precision))
     case.apply(sim)
     actual, feq = sim.update_counts()
-    variant, tuned_ms = sim.autotune(reps=10)  # update scheme of the fused kernel, state unchanged
+    variant, tuned_ms = choose_scheme(sim, scheme, 10)  # update scheme of the fused kernel, state unchanged
     sim.prepare_graphs()  # every graph the timed region replays exists before the timer
     sim.advance_graph(max(warmup, 1))
     sim.synchronize()
@@ -77,6 +89,13 @@ def run_gpu(case, precision, steps, warmup, timing_steps):
     sim.synchronize()
     dt = time.perf_counter() - t0
     drag = sim.drag(case.drag_bbox) if case.drag_bbox is not None else np.zeros(3)
+    drag_us = None
+    if case.drag_bbox is not None:
+        # one drag query as a caller sees it: kernels, 3 doubles back, host sync
+        t0 = time.perf_counter()
+        for _ in range(20):
+            sim.drag(case.drag_bbox)
+        drag_us = 1e6 * (time.perf_counter() - t0) / 20
     mass = sim.mass()
     # per-launch duration of the fused kernel of every level: back-to-back
     # replays between HIP events on the solver's stream (state unchanged)
@@ -91,18 +110,20 @@ def run_gpu(case, precision, steps, warmup, timing_steps):
         "mlups_actual": actual * steps / dt / 1e6,
         "kernel_levels": levels,
         "drag": drag.tolist(),
+        "drag_query_us": drag_us,
         "mass": mass,
         "actual_per_step": actual,
         "feq_per_step": feq,
         "autotune": {"variant": {0: "pull", 1: "push where plain"}[variant], "push_blocks": sim.kernel_variant()[1],
-                     "fused_ms_per_coarse_step": {"pull": round(float(tuned_ms[0]), 4),
-                                                  "push where plain": round(float(tuned_ms[1]), 4)}},
+                     "fused_ms_per_coarse_step": None if tuned_ms is None else
+                     {"pull": round(float(tuned_ms[0]), 4), "push where plain": round(float(tuned_ms[1]), 4)},
+                     "chosen_by": "fixed" if tuned_ms is None else "autotuner"},
     }
     sim.close()
     return res
 
 
-def run_ensemble(precs, steps, warmup, timing_steps, count=64):
+def run_ensemble(precs, steps, warmup, timing_steps, count=64, scheme="auto"):
     """64 independent lid-driven cavities (cells per side drawn from
     {32, 48, 64, 96}, tau uniform in [0.52, 0.8], lid speed uniform in
     [0.02, 0.08], one seed) stepped as one device state: one fused-kernel
@@ -122,7 +143,7 @@ def run_ensemble(precs, steps, warmup, timing_steps, count=64):
         sim = GpuSolver.ensemble(ens, p)
         sim.set_bc_kinds(ens.kinds)
         sim.init_equilibrium(1.0, (0.0, 0.0, 0.0), 0xFF)
-        variant, tuned_ms = sim.autotune(reps=5)
+        variant, tuned_ms = choose_scheme(sim, scheme, 5)
         sim.prepare_graphs()
         sim.advance_graph(max(warmup, 1))
         sim.synchronize()
@@ -135,7 +156,8 @@ def run_ensemble(precs, steps, warmup, timing_steps, count=64):
         out[f"fp{p}"] = {"mlups": round(cells * steps / dt / 1e6, 3), "ms_per_step": round(1e3 * dt / steps, 6),
                          "steps": steps,
                          "autotune": {"variant": {0: "pull", 1: "push where plain"}[variant],
-                                      "fused_ms": [round(float(x), 4) for x in tuned_ms]},
+                                      "fused_ms": None if tuned_ms is None else [round(float(x), 4) for x in tuned_ms],
+                                      "chosen_by": "fixed" if tuned_ms is None else "autotuner"},
                          "roofline": {"bound": "hbm", "achieved": round(ach, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                                       "frac": round(ach / HBM_PEAK_GBS, 4), "kernel": "fused collide+stream (KBlock)",
                                       "cells_per_launch": upd, "avg_launch_us": round(1e3 * ms, 3)}}
@@ -182,6 +204,7 @@ def run_adjoint_channel(steps, checkpoint):
             "coarse_steps": steps, "window": list(win), "checkpoint": checkpoint, "links": int(len(case.q)),
             "J": J, "dJ_du_inlet": [float(v) for v in gi], "max_abs_dJ_dq": float(np.abs(dq).max()),
             "adjoint_ms": round(fw + bw, 3), "forward_sweep_ms": round(fw, 3), "reverse_sweep_ms": round(bw, 3),
+            "tapes": "one taping pass per segment" if adj.packed else "per-step re-run",
             "forward_fp64_ms_per_coarse_step": round(fwd_ms, 5),
             "adjoint_over_forward_fp64": round((fw + bw) / (steps * fwd_ms), 3),
             "fp32_forward": {"adjoint_ms": round(fw32 + bw32, 3),
@@ -401,6 +424,8 @@ def main():
     ap.add_argument("--channel-adjoint-steps", type=int, default=4000,
                     help="coarse steps of the 2-level channel's adjoint run (J over the second half)")
     ap.add_argument("--precision", default="both", choices=["both", "32", "64"])
+    ap.add_argument("--scheme", default="auto", choices=sorted(SCHEMES),
+                    help="fused-kernel update scheme: the autotuner's choice (default) or a fixed one")
     ap.add_argument("--calibrate-copy-mb", type=int, default=0,
                     help="run one device-to-device copy of this many MiB first (profiler "
                          "counter scale: a plain copy reads and writes exactly that many bytes)")
@@ -428,7 +453,7 @@ def main():
     precs = [32, 64] if args.precision == "both" else [int(args.precision)]
     for p in precs:
         log(f"gpu fp{p}: warmup {args.warmup}, timed {args.steps} coarse steps")
-        res[p] = run_gpu(case, p, args.steps, args.warmup, args.timing_steps)
+        res[p] = run_gpu(case, p, args.steps, args.warmup, args.timing_steps, args.scheme)
         log(f"  fp{p}: {res[p]['mlups_feq']:.1f} MLUPS feq, {res[p]['ms_per_step']:.4f} ms/step")
     head = 32 if 32 in res else precs[0]
     r = res[head]
@@ -445,7 +470,7 @@ def main():
         sres = {}
         for p in precs:
             log(f"gpu fp{p} synthetic: {args.synthetic_steps} coarse steps")
-            sres[p] = run_gpu(sc, p, args.synthetic_steps, 5, min(args.timing_steps, 20))
+            sres[p] = run_gpu(sc, p, args.synthetic_steps, 5, min(args.timing_steps, 20), args.scheme)
             log(f"  fp{p}: {sres[p]['mlups_feq']:.1f} MLUPS feq, {sres[p]['ms_per_step']:.4f} ms/step")
         sh = sres[head]
         sl = max(sh["kernel_levels"], key=lambda lv: sh["kernel_levels"][lv]["cells_per_launch"])
@@ -473,7 +498,7 @@ def main():
     ens = None
     if not args.no_ensemble and args.workload == "cylinder":
         log("cavity ensemble")
-        ens = run_ensemble(precs, args.ensemble_steps, 5, min(args.timing_steps, 10))
+        ens = run_ensemble(precs, args.ensemble_steps, 5, min(args.timing_steps, 10), scheme=args.scheme)
         for p in precs:
             log(f"  fp{p}: {ens[f'fp{p}']['mlups']:.1f} MLUPS, {ens[f'fp{p}']['ms_per_step']:.4f} ms/step")
     links = None
@@ -552,6 +577,8 @@ def main():
         "link_build": links,
         "adjoint": adjc,
         "adjoint_single_level": adjo,
+        "drag_query_us": {f"fp{p}": (round(res[p]["drag_query_us"], 2) if res[p]["drag_query_us"] else None)
+                          for p in res},
         "drag_fp32": res[32]["drag"] if 32 in res else None,
         "drag_fp64": res[64]["drag"] if 64 in res else None,
     }
