@@ -1577,16 +1577,20 @@ __global__ __launch_bounds__(512) void KFdCouple(FdArgs<T> fa, const int* fd_blo
 // (iz, iy, ix, iPop) and over blocks in block order, all in FP64.
 // MomentumExchangeDrag::force_on_fluid_if of one block per workgroup
 // (drag_postprocessor.h:66-115): the threads evaluate the block's link terms
-// (f_out + f_bar, the bbox predicate of cylinder3d_case.h:483-490) into LDS,
-// then one thread sums them in the reference's loop order -- bit-equal to the
-// sequential loop, with no global load on the dependent summation chain.
+// (f_out + f_bar times c_i, the bbox predicate of cylinder3d_case.h:483-490)
+// into LDS, then one thread sums them in the reference's loop order --
+// bit-equal to the sequential loop, with only LDS reads (no table lookup)
+// on the dependent summation chain.
 constexpr int kDragThreads = 256;
 constexpr int kDragChunk = 2048;
 
 template <typename T>
 __global__ __launch_bounds__(kDragThreads) void KDragBlock(DragArgs<T> a, const int* block_begin, double* out) {
-  __shared__ double term[kDragChunk];
-  __shared__ int8_t pop[kDragChunk];  // -1: link outside the predicate
+  // per link: the three products t * c_i (the reference's operands, so the
+  // sequential sum below adds exactly what its loop adds) and whether the
+  // link passes the bbox predicate
+  __shared__ double tx[kDragChunk], ty[kDragChunk], tz[kDragChunk];
+  __shared__ uint8_t in[kDragChunk];
   const int j = blockIdx.x, tid = threadIdx.x;
   const int begin = block_begin[j], end = block_begin[j + 1];
   const SimConst& sc = a.sc;
@@ -1603,26 +1607,27 @@ __global__ __launch_bounds__(kDragThreads) void KDragBlock(DragArgs<T> a, const 
       const double px = org[0] + (ix + kCxT[i] + 0.5) * cw;
       const double py = org[1] + (iy + kCyT[i] + 0.5) * cw;
       const double pz = org[2] + (iz + kCzT[i] + 0.5) * cw;
-      const bool in = px >= a.bbox[0] && px <= a.bbox[3] && py >= a.bbox[1] && py <= a.bbox[4] &&
-                      pz >= a.bbox[2] && pz <= a.bbox[5];
+      const bool inside = px >= a.bbox[0] && px <= a.bbox[3] && py >= a.bbox[1] && py <= a.bbox[4] &&
+                          pz >= a.bbox[2] && pz <= a.bbox[5];
       const T* live = (a.par.v[a.bt[b].level] ? a.pops1 : a.pops0) + static_cast<size_t>(b) * kQ * sc.plane;
       const int sl = CoreSlot(sc, c);
       const int opp = i == 0 ? 0 : (i <= 9 ? i + 9 : i - 9);
       const double fbar = static_cast<double>(live[static_cast<size_t>(opp) * sc.plane + sl]);
       const double fout =
           a.snap ? static_cast<double>(a.snap[L.w]) : static_cast<double>(live[static_cast<size_t>(i) * sc.plane + sl]);
-      term[e] = fout + fbar;
-      pop[e] = in ? static_cast<int8_t>(i) : static_cast<int8_t>(-1);
+      const double t = fout + fbar;
+      tx[e] = t * static_cast<double>(kCxT[i]);
+      ty[e] = t * static_cast<double>(kCyT[i]);
+      tz[e] = t * static_cast<double>(kCzT[i]);
+      in[e] = inside ? 1 : 0;
     }
     __syncthreads();
     if (tid == 0) {
       for (int e = 0; e < m; ++e) {
-        const int i = pop[e];
-        if (i < 0) continue;
-        const double t = term[e];
-        fo0 += t * static_cast<double>(kCxT[i]);
-        fo1 += t * static_cast<double>(kCyT[i]);
-        fo2 += t * static_cast<double>(kCzT[i]);
+        if (!in[e]) continue;
+        fo0 += tx[e];
+        fo1 += ty[e];
+        fo2 += tz[e];
       }
     }
     __syncthreads();
