@@ -730,7 +730,8 @@ struct octlb_adj {
   size_t selem = 8;
   std::vector<void*> owned;
   double fwd_ms = 0.0, bwd_ms = 0.0;
-  bool fd_side = false;  // MLReverse: launch the FD VJP on the side stream
+  bool fd_side = false;      // MLReverse: launch the FD VJP on the side stream
+  bool fd_inflight = false;  // an FD VJP on the side stream is not joined yet
   bool packed_mode = false;  // multi-level: the last run kept a segment's tapes for all its steps
   long long tape_budget = -1;  // bytes allowed for a segment's packed tapes (-1: 60 % of the free memory)
 
@@ -1398,9 +1399,27 @@ struct octlb_adj {
           hipLaunchKernelGGL(KMLAccum, dim3(1), dim3(kAccumThreads), 0, stream, d_dui_part, static_cast<int>(nwg),
                              d_dui);
         }
-        ADJ_CHECK(hipMemcpyAsync(d_lx + lv_entry[l], d_ln + lv_entry[l],
-                                 static_cast<size_t>(lv_entry[l + 1] - lv_entry[l]) * sizeof(double),
-                                 hipMemcpyDeviceToDevice, stream));
+        // the level's new adjoint is in d_ln: copy it back, or, when the other
+        // levels hold fewer entries, copy those across and swap the buffers
+        const size_t mine = static_cast<size_t>(lv_entry[l + 1] - lv_entry[l]);
+        const size_t rest = static_cast<size_t>(ntot) - mine;
+        if (rest < mine) {
+          if (fd_inflight) {  // the FD VJP beside this visit writes entries of the other levels
+            ADJ_CHECK(hipStreamWaitEvent(stream, ev_join, 0));
+            fd_inflight = false;
+          }
+          if (lv_entry[l] > 0)
+            ADJ_CHECK(hipMemcpyAsync(d_ln, d_lx, static_cast<size_t>(lv_entry[l]) * sizeof(double),
+                                     hipMemcpyDeviceToDevice, stream));
+          if (lv_entry[l + 1] < ntot)
+            ADJ_CHECK(hipMemcpyAsync(d_ln + lv_entry[l + 1], d_lx + lv_entry[l + 1],
+                                     static_cast<size_t>(ntot - lv_entry[l + 1]) * sizeof(double),
+                                     hipMemcpyDeviceToDevice, stream));
+          std::swap(d_lx, d_ln);
+        } else {
+          ADJ_CHECK(hipMemcpyAsync(d_lx + lv_entry[l], d_ln + lv_entry[l], mine * sizeof(double),
+                                   hipMemcpyDeviceToDevice, stream));
+        }
         break;
       }
       case kOpFd: {
@@ -1560,9 +1579,11 @@ struct octlb_adj {
           if (fork) {
             ADJ_CHECK(hipEventRecord(ev_join, side));
             joined = false;
+            fd_inflight = true;
           } else if (!joined) {
             ADJ_CHECK(hipStreamWaitEvent(stream, ev_join, 0));
             joined = true;
+            fd_inflight = false;
           }
         }
       };

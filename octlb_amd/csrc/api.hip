@@ -184,6 +184,7 @@ struct octlb_dev {
     return p;
   }
   double* d_out = nullptr;
+  double* h_out = nullptr;  // pinned host copy of d_out: small results come back without a staging copy
   double* d_mass_partial = nullptr;
   int* d_plan = nullptr;
   void* d_prev = nullptr;
@@ -263,6 +264,8 @@ struct octlb_dev {
     for (hipEvent_t e : ev) (void)hipEventDestroy(e);
     for (void* p : owned) (void)hipFree(p);
     owned.clear();
+    if (h_out) (void)hipHostFree(h_out);
+    h_out = nullptr;
     if (stream) (void)hipStreamDestroy(stream);
     stream = nullptr;
   }
@@ -598,6 +601,7 @@ void octlb_dev::Build(const octlb_topology* t, const octlb_params* p) {
   d_step = Alloc<int>(2);
   HIP_CHECK(hipMemsetAsync(d_step, 0, 2 * sizeof(int), stream));
   d_out = Alloc<double>(8);
+  HIP_CHECK(hipHostMalloc(reinterpret_cast<void**>(&h_out), 8 * sizeof(double), hipHostMallocDefault));
   d_mass_partial = Alloc<double>(nb);
   d_plan = Alloc<int>(plan.size());
   Upload(d_plan, plan, stream);
@@ -1480,8 +1484,9 @@ void Drag(octlb_dev* d, const double* bbox, double* out) {
   a.sc = d->sc;
   LaunchDrag<T>(a, d->ndlinks, d->d_dblock_begin, d->ndblocks, d->d_dblock_out, d->d_out, d->stream);
   HIP_CHECK(hipGetLastError());
-  HIP_CHECK(hipMemcpyAsync(out, d->d_out, 3 * sizeof(double), hipMemcpyDeviceToHost, d->stream));
+  HIP_CHECK(hipMemcpyAsync(d->h_out, d->d_out, 3 * sizeof(double), hipMemcpyDeviceToHost, d->stream));
   d->Sync();
+  for (int k = 0; k < 3; ++k) out[k] = d->h_out[k];
 }
 
 template <typename T>
@@ -1489,8 +1494,9 @@ void Mass(octlb_dev* d, double* out) {
   LaunchMass<T>(static_cast<const T*>(d->pops[0]), static_cast<const T*>(d->pops[1]), d->Par(), d->d_bt,
                 d->d_kind, d->sc, d->nb, d->d_mass_partial, d->d_out + 3, d->stream);
   HIP_CHECK(hipGetLastError());
-  HIP_CHECK(hipMemcpyAsync(out, d->d_out + 3, sizeof(double), hipMemcpyDeviceToHost, d->stream));
+  HIP_CHECK(hipMemcpyAsync(d->h_out + 3, d->d_out + 3, sizeof(double), hipMemcpyDeviceToHost, d->stream));
   d->Sync();
+  *out = d->h_out[3];
 }
 
 // rho [nb][ncore], u [nb][ncore][3] in VTK cell order, on the host

@@ -1587,10 +1587,10 @@ constexpr int kDragChunk = 2048;
 template <typename T>
 __global__ __launch_bounds__(kDragThreads) void KDragBlock(DragArgs<T> a, const int* block_begin, double* out) {
   // per link: the three products t * c_i (the reference's operands, so the
-  // sequential sum below adds exactly what its loop adds) and whether the
-  // link passes the bbox predicate
+  // sequential sum below adds exactly what its loop adds), +0.0 for a link
+  // outside the bbox predicate: a sum that starts at +0.0 is never -0.0
+  // under round-to-nearest, so adding +0.0 leaves it exactly as skipping does
   __shared__ double tx[kDragChunk], ty[kDragChunk], tz[kDragChunk];
-  __shared__ uint8_t in[kDragChunk];
   const int j = blockIdx.x, tid = threadIdx.x;
   const int begin = block_begin[j], end = block_begin[j + 1];
   const SimConst& sc = a.sc;
@@ -1616,15 +1616,14 @@ __global__ __launch_bounds__(kDragThreads) void KDragBlock(DragArgs<T> a, const 
       const double fout =
           a.snap ? static_cast<double>(a.snap[L.w]) : static_cast<double>(live[static_cast<size_t>(i) * sc.plane + sl]);
       const double t = fout + fbar;
-      tx[e] = t * static_cast<double>(kCxT[i]);
-      ty[e] = t * static_cast<double>(kCyT[i]);
-      tz[e] = t * static_cast<double>(kCzT[i]);
-      in[e] = inside ? 1 : 0;
+      tx[e] = inside ? t * static_cast<double>(kCxT[i]) : 0.0;
+      ty[e] = inside ? t * static_cast<double>(kCyT[i]) : 0.0;
+      tz[e] = inside ? t * static_cast<double>(kCzT[i]) : 0.0;
     }
     __syncthreads();
     if (tid == 0) {
+#pragma unroll 8
       for (int e = 0; e < m; ++e) {
-        if (!in[e]) continue;
         fo0 += tx[e];
         fo1 += ty[e];
         fo2 += tz[e];

@@ -11,6 +11,8 @@ coarse steps 13..24, checkpoint every 5 steps, inlet ramp over 8 steps.
   FD pass) match central differences of the FP64 oracle's forward
   (eps = 1e-6) to 1e-4: u_in's x component, the three largest |dJ/dq|, and
   a random direction over all links.
+* With FP32 forward states (FP64 adjoint) the same gradients match the same
+  central differences to 1e-4.
 * The gradients do not depend on the checkpoint interval, nor on whether a
   segment's tapes are kept for all its steps (the default here) or each step
   is re-run just before its reverse (a zero tape budget).
@@ -41,6 +43,16 @@ def _u_in(case):
         if s.inlet != abi.INLET_NONE:
             return np.array([s.inlet_u[0], s.inlet_u[1], s.inlet_u[2]])
     raise AssertionError("no inlet")
+
+
+_FD_CACHE = {}
+
+
+def _central_difference(case, key, plus, minus):
+    """(J(plus) - J(minus)) / 2 eps of the FP64 oracle, computed once per key."""
+    if key not in _FD_CACHE:
+        _FD_CACHE[key] = (_oracle_J(case, **plus) - _oracle_J(case, **minus)) / (2 * EPS)
+    return _FD_CACHE[key]
 
 
 def _oracle_J(case, q=None, u_in=None):
@@ -92,10 +104,16 @@ def test_inlet_velocity_gradient_matches_central_differences(channel, device):
     _, _, gi, _ = device
     u = _u_in(channel)
     e = np.array([EPS, 0.0, 0.0])
-    fd = (_oracle_J(channel, u_in=u + e) - _oracle_J(channel, u_in=u - e)) / (2 * EPS)
+    fd = _central_difference(channel, "u_in_x", dict(u_in=u + e), dict(u_in=u - e))
     print(f"dJ/du_in,x adjoint {gi[0]:.10e} vs central difference {fd:.10e}", flush=True)
     assert abs(fd) > 0
     assert abs(gi[0] - fd) <= TOL * abs(fd)
+
+
+def _direction(q):
+    rng = np.random.default_rng(5)
+    v = rng.normal(size=q.shape)
+    return v / np.abs(v).max()
 
 
 def test_wall_fraction_gradients_match_central_differences(channel, device):
@@ -106,13 +124,11 @@ def test_wall_fraction_gradients_match_central_differences(channel, device):
         qp, qm = q.copy(), q.copy()
         qp[k] += EPS
         qm[k] -= EPS
-        fd = (_oracle_J(channel, q=qp) - _oracle_J(channel, q=qm)) / (2 * EPS)
+        fd = _central_difference(channel, ("q", int(k)), dict(q=qp), dict(q=qm))
         print(f"link {k}: dJ/dq adjoint {dq[k]:.10e} vs central difference {fd:.10e}", flush=True)
         assert abs(dq[k] - fd) <= TOL * abs(fd)
-    rng = np.random.default_rng(5)
-    v = rng.normal(size=q.shape)
-    v /= np.abs(v).max()
-    fd = (_oracle_J(channel, q=q + EPS * v) - _oracle_J(channel, q=q - EPS * v)) / (2 * EPS)
+    v = _direction(q)
+    fd = _central_difference(channel, "q_dir", dict(q=q + EPS * v), dict(q=q - EPS * v))
     ad = float(dq @ v)
     print(f"random direction: adjoint {ad:.10e} vs central difference {fd:.10e}", flush=True)
     assert abs(ad - fd) <= TOL * abs(fd)
@@ -135,3 +151,31 @@ def test_recompute_mode_matches_the_packed_tapes(channel, device):
     assert J2 == J
     np.testing.assert_array_equal(dq2, dq)
     np.testing.assert_array_equal(adj.inlet_grad, gi)
+
+
+def test_fp32_forward_gradients_match_central_differences(channel, device):
+    """FP32 states in the forward sweep and the taping pass (the FP32 path's
+    arithmetic), adjoint in FP64: dJ/du_in, the three largest |dJ/dq| and
+    the random direction against the FP64 oracle's central differences."""
+    _, dq64, _, _ = device
+    adj = DragAdjoint(channel, STEPS, WIN, checkpoint=CK, precision=32)
+    _, _, dq = adj.evaluate()
+    gi = adj.inlet_grad
+    u = _u_in(channel)
+    e = np.array([EPS, 0.0, 0.0])
+    fd = _central_difference(channel, "u_in_x", dict(u_in=u + e), dict(u_in=u - e))
+    print(f"FP32 forward: dJ/du_in,x {gi[0]:.10e} vs central difference {fd:.10e}", flush=True)
+    assert abs(gi[0] - fd) <= TOL * abs(fd)
+    q = np.array(channel.q, dtype=np.float64)
+    for k in np.argsort(-np.abs(dq64))[:3]:
+        qp, qm = q.copy(), q.copy()
+        qp[k] += EPS
+        qm[k] -= EPS
+        fd = _central_difference(channel, ("q", int(k)), dict(q=qp), dict(q=qm))
+        print(f"FP32 forward: link {k} dJ/dq {dq[k]:.10e} vs central difference {fd:.10e}", flush=True)
+        assert abs(dq[k] - fd) <= TOL * abs(fd)
+    v = _direction(q)
+    fd = _central_difference(channel, "q_dir", dict(q=q + EPS * v), dict(q=q - EPS * v))
+    ad = float(dq @ v)
+    print(f"FP32 forward: random direction {ad:.10e} vs central difference {fd:.10e}", flush=True)
+    assert abs(ad - fd) <= TOL * abs(fd)
