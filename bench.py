@@ -74,6 +74,14 @@ def choose_scheme(sim, scheme, reps):
     return SCHEMES[scheme], None
 
 
+def epilogue_record(sim):
+    """Which coupling steps run inside the update launches (bit 0: half-time
+    prolongation, bit 1: restriction) and the autotuner's weighted times."""
+    mode, ms = sim.coupling_epilogues()
+    return {"half_time_prolongation": bool(mode & 1), "restriction": bool(mode & 2),
+            "ms_per_coarse_step": {"separate": round(float(ms[0]), 4), "chosen": round(float(ms[1]), 4)}}
+
+
 def run_gpu(case, precision, steps, warmup, timing_steps, scheme="auto"):
     from octlb_amd.solver import GpuSolver
 
@@ -117,7 +125,8 @@ def run_gpu(case, precision, steps, warmup, timing_steps, scheme="auto"):
         "autotune": {"variant": {0: "pull", 1: "push where plain"}[variant], "push_blocks": sim.kernel_variant()[1],
                      "fused_ms_per_coarse_step": None if tuned_ms is None else
                      {"pull": round(float(tuned_ms[0]), 4), "push where plain": round(float(tuned_ms[1]), 4)},
-                     "chosen_by": "fixed" if tuned_ms is None else "autotuner"},
+                     "chosen_by": "fixed" if tuned_ms is None else "autotuner",
+                     "coupling_epilogues": epilogue_record(sim)},
     }
     sim.close()
     return res

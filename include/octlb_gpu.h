@@ -236,8 +236,22 @@ int octlb_dev_kernel_variant(octlb_dev* dev, int32_t* variant, int32_t* push_blo
  * the current state (left unchanged), weights level l by its 2^l launches
  * per coarse step, and keeps the fastest.  step_ms[2] (optional) receives
  * the weighted kernel time per coarse step of variants 0 and 1 (0 when a
- * variant does not apply). */
+ * variant does not apply).  Then, per kind, it decides the coupling
+ * epilogues (below) the same way. */
 int octlb_dev_autotune(octlb_dev* dev, int32_t reps, double* step_ms);
+/* Coupling epilogues (no counterpart in the reference; same results bit for
+ * bit either way): bit 0, the half-time prolongation (LevelCoupler::
+ * apply_half_time, level_coupler.cpp:438-478) that follows an update runs in
+ * that update's launch, in the workgroups of the blocks holding its coarse
+ * cells; bit 1, the restriction (level_coupler.cpp:613-654) after the
+ * finest level's second update runs in the workgroups of the blocks holding
+ * its fine cells.  Default 3; applies only when no FD block is coupled and
+ * the coupled blocks pull (else the separate coupling kernels run).
+ * octlb_dev_coupling_epilogues reports the mode in effect and the
+ * autotuner's weighted times per coarse step of the affected launches,
+ * ms[0] without and ms[1] with its choice. */
+int octlb_dev_set_coupling_epilogues(octlb_dev* dev, int32_t mode);
+int octlb_dev_coupling_epilogues(octlb_dev* dev, int32_t* mode, double ms[2]);
 
 /* MomentumExchangeDrag::force_on_fluid_if summed over blocks with the
  * obstacle predicate "neighbour cell centre inside bbox"

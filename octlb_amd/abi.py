@@ -209,6 +209,8 @@ _SIGNATURES = {
     "octlb_adj_inlet_grad": ([_VP, _F64P], C.c_int),
     "octlb_adj_timing": ([_VP, _F64P, _F64P], C.c_int),
     "octlb_adj_set_tape_budget": ([_VP, C.c_int64, C.POINTER(C.c_int32)], C.c_int),
+    "octlb_dev_set_coupling_epilogues": ([_VP, C.c_int32], C.c_int),
+    "octlb_dev_coupling_epilogues": ([_VP, C.POINTER(C.c_int32), _F64P], C.c_int),
 }
 
 EXPORTED = tuple(_SIGNATURES)

@@ -146,6 +146,23 @@ struct octlb_dev {
   int* d_pfull = nullptr;     // per core cell: active prolongation point targeting it, or -1
   int pfull_fuse_level = -1;  // the next update launch of this level applies the full-time prolongation
   bool pfull_fused = false;
+  // Coupling epilogues (TileArgs::ep_*): the half-time prolongation after an
+  // update of level l and the restriction after the second update of level l
+  // in its parent's visit run in the coupled blocks' own workgroups of that
+  // update launch.  Per level, CSR over all blocks of the active points.
+  bool ep_ok = false;
+  int* d_ep_pro_beg[kMaxLevels] = {};
+  int* d_ep_pro_pts[kMaxLevels] = {};
+  int* d_ep_res_beg[kMaxLevels] = {};
+  int* d_ep_res_pts[kMaxLevels] = {};
+  bool second_visit[kMaxLevels] = {};   // the next update of this level is its parent's second visit
+  bool pro_done = false, res_done = false;  // the coupling step that follows ran in the update launch
+  int ep_mode = 3;                      // bit 0: half-time prolongation epilogue, bit 1: restriction epilogue
+  double ep_ms[2] = {0.0, 0.0};         // autotuner: fused + coupling kernels per coarse step, off / on
+  template <typename T>
+  CoupleArgs<T> EpilogueArgs(int l, bool timing_only, int other_level) const;
+  template <typename T>
+  double TimeLaunches(int l, int reps, int what);
   BlockTab* d_bt = nullptr;
   Tile* d_tiles = nullptr;
   int* d_block_rank = nullptr;  // block -> index among its level's blocks in visiting order
@@ -962,6 +979,33 @@ void octlb_dev::Prepare() {
     fd_phase[0] = fd_phase[1] = fd_phase[2] = fd_phase[3] = 0;
   }
 
+  // ---- coupling epilogues: the coupled blocks must be pull tiles of their
+  // level's launch, and no FD block may be coupled (the FD pass of a level
+  // then commutes with its coupling steps)
+  ep_ok = !padded && npts > 0 && (nfd_blocks == 0 || fd_merge_ok);
+  for (int l = 0; l < kMaxLevels && ep_ok; ++l) {
+    for (int i : prolong_active[l]) ep_ok = ep_ok && !(bt[plan[12 * i]].flags & kFlagPush);
+    for (int i : restrict_active[l]) ep_ok = ep_ok && !(bt[plan[12 * i + 4]].flags & kFlagPush);
+  }
+  for (int l = 0; l < kMaxLevels; ++l) {
+    d_ep_pro_beg[l] = d_ep_pro_pts[l] = d_ep_res_beg[l] = d_ep_res_pts[l] = nullptr;
+    if (!ep_ok) continue;
+    auto csr = [&](const std::vector<int>& pts, int field, int** d_beg, int** d_pts) {
+      if (pts.empty()) return;
+      std::vector<int> beg(static_cast<size_t>(nb) + 1, 0), out(pts.size());
+      for (int i : pts) ++beg[plan[12 * i + field] + 1];
+      for (int b = 0; b < nb; ++b) beg[b + 1] += beg[b];
+      std::vector<int> at(beg.begin(), beg.end() - 1);
+      for (int i : pts) out[at[plan[12 * i + field]]++] = i;
+      *d_beg = PAlloc<int>(beg.size());
+      Upload(*d_beg, beg, stream);
+      *d_pts = PAlloc<int>(out.size());
+      Upload(*d_pts, out, stream);
+    };
+    csr(prolong_active[l], 0, &d_ep_pro_beg[l], &d_ep_pro_pts[l]);   // by coarse block
+    csr(restrict_active[l], 4, &d_ep_res_beg[l], &d_ep_res_pts[l]);  // by fine block
+  }
+
   d_qmask = PAlloc<uint32_t>(qmask.size());
   Upload(d_qmask, qmask, stream);
   d_qoff = PAlloc<int>(qoff.size());
@@ -1125,6 +1169,76 @@ double octlb_dev::TimeKernel(int l, int reps) {
   return static_cast<double>(ms) / reps;
 }
 
+// Coupler arguments for a coupling step run by the update launch of level l
+// (its cells are in that launch's output).  timing_only: the step's writes
+// into level other_level go to that level's idle buffer instead (overwritten
+// by its next update), so a timing replay leaves the state as it was.
+template <typename T>
+CoupleArgs<T> octlb_dev::EpilogueArgs(int l, bool timing_only, int other_level) const {
+  const double tau = 1.0 / sc.omega;  // level_coupler.cpp:226-228
+  CoupleArgs<T> e{};
+  e.plan = d_plan;
+  e.pops0 = static_cast<T*>(pops[0]);
+  e.pops1 = static_cast<T*>(pops[1]);
+  e.prev = static_cast<T*>(d_prev);
+  e.prolong_scale = static_cast<T>((tau - 0.25) / tau);
+  e.restrict_scale = static_cast<T>(tau / (tau - 0.25));
+  e.par = Par();
+  e.par.v[l] ^= 1;
+  if (timing_only) e.par.v[other_level] ^= 1;
+  return e;
+}
+
+// Mean duration (ms) of `reps` back-to-back replays, state unchanged:
+// what = 0: level l's fused kernel with the half-time prolongation epilogue;
+// 1: with the restriction epilogue; 2: the standalone half-time prolongation
+// of coarse level l; 3: the standalone restriction from fine level l.  Every
+// write goes to an idle buffer (the fused kernel's output as in TimeKernel,
+// the coupled level's cells through EpilogueArgs' redirection).
+template <typename T>
+double octlb_dev::TimeLaunches(int l, int reps, int what) {
+  auto launch = [&]() {
+    if (what <= 1) {
+      TileArgs<T> a = MakeTileArgs<T>(l);
+      a.want_stats = 0;
+      a.snap_on = 0;
+      if (what == 0) {
+        a.ep_pro_beg = d_ep_pro_beg[l];
+        a.ep_pro_pts = d_ep_pro_pts[l];
+        a.ep_pro = EpilogueArgs<T>(l, true, l + 1);
+      } else {
+        a.ep_res_beg = d_ep_res_beg[l];
+        a.ep_res_pts = d_ep_res_pts[l];
+        a.ep_res = EpilogueArgs<T>(l, true, l - 1);
+      }
+      LaunchTile<T>(a, stream);
+    } else {
+      // the standalone kernels read the current state; their writes go idle
+      CoupleArgs<T> c = EpilogueArgs<T>(l, true, what == 2 ? l + 1 : l - 1);
+      c.par.v[l] ^= 1;  // reads of level l: the current state, as after its update
+      if (what == 2) {
+        LaunchProlong<T>(c, d_prolong + prolong_off[l], static_cast<int>(prolong_active[l].size()), 1, stream);
+      } else {
+        LaunchRestrict<T>(c, d_restrict + restrict_off[l], static_cast<int>(restrict_active[l].size()), stream);
+      }
+    }
+    HIP_CHECK(hipGetLastError());
+  };
+  hipEvent_t e0, e1;
+  HIP_CHECK(hipEventCreate(&e0));
+  HIP_CHECK(hipEventCreate(&e1));
+  launch();
+  HIP_CHECK(hipEventRecord(e0, stream));
+  for (int r = 0; r < reps; ++r) launch();
+  HIP_CHECK(hipEventRecord(e1, stream));
+  HIP_CHECK(hipEventSynchronize(e1));
+  float ms = 0.0f;
+  HIP_CHECK(hipEventElapsedTime(&ms, e0, e1));
+  HIP_CHECK(hipEventDestroy(e0));
+  HIP_CHECK(hipEventDestroy(e1));
+  return static_cast<double>(ms) / reps;
+}
+
 template <typename T>
 void octlb_dev::Level(int l) {
   const int cnt = tile_count[l];
@@ -1158,6 +1272,22 @@ void octlb_dev::Level(int l) {
         fd_pending = false;
       } else {
         FlushFd<T>();
+      }
+    }
+    if (ep_ok) {
+      if ((ep_mode & 1) && l < max_level && d_ep_pro_beg[l] != nullptr) {
+        a.ep_pro_beg = d_ep_pro_beg[l];
+        a.ep_pro_pts = d_ep_pro_pts[l];
+        a.ep_pro = EpilogueArgs<T>(l, false, l + 1);
+        pro_done = true;
+      }
+      // restriction: only from the finest level, whose second update is the
+      // last thing before it (a coarser level's nested visits come between)
+      if ((ep_mode & 2) && l >= 1 && l == max_level && second_visit[l] && d_ep_res_beg[l] != nullptr) {
+        a.ep_res_beg = d_ep_res_beg[l];
+        a.ep_res_pts = d_ep_res_pts[l];
+        a.ep_res = EpilogueArgs<T>(l, false, l - 1);
+        res_done = true;
       }
     }
     if (!padded && fd_other_cnt[l] > 0) {
@@ -1235,7 +1365,9 @@ void octlb_dev::Level(int l) {
   if (!pfull_fused) Couple<T>(c, d_prolong + prolong_off[l], npro, 0, l + 1);
   pfull_fused = false;
   pfull_fuse_level = -1;
+  second_visit[l + 1] = true;
   Level<T>(l + 1);
+  second_visit[l + 1] = false;
   Couple<T>(c, d_restrict + restrict_off[l + 1], nres, 2, -1);
 }
 
@@ -1281,6 +1413,14 @@ void octlb_dev::Couple(CoupleArgs<T> c, const int* list, int count, int mode, in
   const bool defer = fd_pending && fd_merge_ok && fd_pend_level >= 0 && next_level >= 0 &&
                      next_level != fd_pend_level && tile_count[next_level] > 0 &&
                      fd_other_cnt[next_level] > 0;
+  if ((mode == 1 && pro_done) || (mode == 2 && res_done)) {
+    // the coupling step ran in the update launch before; a pending FD pass
+    // still needs a launch unless the next update carries it
+    pro_done = mode == 1 ? false : pro_done;
+    res_done = mode == 2 ? false : res_done;
+    if (!defer) FlushFd<T>();
+    return;
+  }
   if (defer) {
     c.step_next = TakeInc(count > 0);
     if (mode == 2) {
@@ -2291,6 +2431,53 @@ int octlb_dev_autotune(octlb_dev* d, int32_t reps, double* step_ms) {
       d->prepared = false;
       EnsureReady(d);
     }
+    // Coupling epilogues, per kind: on when the fused launches carrying them
+    // (per coarse step) take less than the plain launches plus the separate
+    // coupling kernels they replace.
+    d->ep_ms[0] = d->ep_ms[1] = 0.0;
+    if (d->ep_ok) {
+      double pro_off = 0.0, pro_on = 0.0, res_off = 0.0, res_on = 0.0;
+      for (int l = 0; l <= d->max_level; ++l) {
+        if (d->tile_count[l] == 0) continue;
+        auto time = [&](int what) {
+          return d->prec == 64 ? d->TimeLaunches<double>(l, reps, what) : d->TimeLaunches<float>(l, reps, what);
+        };
+        const double t = d->prec == 64 ? d->TimeKernel<double>(l, reps) : d->TimeKernel<float>(l, reps);
+        const double w = static_cast<double>(1 << l);
+        if (l < d->max_level && d->d_ep_pro_beg[l] != nullptr) {
+          pro_off += w * time(2);
+          pro_on += w * (time(0) - t);
+        }
+        if (l >= 1 && l == d->max_level && d->d_ep_res_beg[l] != nullptr) {
+          const double wr = static_cast<double>(1 << (l - 1));  // once per visit of the parent
+          res_off += wr * time(3);
+          res_on += wr * (time(1) - t);
+        }
+      }
+      d->ep_mode = (pro_on < pro_off ? 1 : 0) | (res_on < res_off ? 2 : 0);
+      d->ep_ms[0] = pro_off + res_off;
+      d->ep_ms[1] = std::min(pro_on, pro_off) + std::min(res_on, res_off);
+    }
+  });
+}
+
+int octlb_dev_coupling_epilogues(octlb_dev* d, int32_t* mode, double ms[2]) {
+  return Guard([&] {
+    CheckDev(d);
+    if (!d->prepared) d->Prepare();  // whether the epilogues apply is known once the tables are built
+    if (mode) *mode = d->ep_ok ? d->ep_mode : 0;
+    if (ms) {
+      ms[0] = d->ep_ms[0];
+      ms[1] = d->ep_ms[1];
+    }
+  });
+}
+
+int octlb_dev_set_coupling_epilogues(octlb_dev* d, int32_t mode) {
+  return Guard([&] {
+    CheckDev(d);
+    if (mode < 0 || mode > 3) throw ArgError("epilogue mode must be 0..3");
+    d->ep_mode = mode;
   });
 }
 

@@ -220,6 +220,11 @@ __device__ inline void WriteMacro(T* base, int plane, int c, const T* m, T scali
 
 template <typename T>
 __device__ inline void ProlongPoint(const CoupleArgs<T>& a, const int* list, int count, int half, int j);
+template <typename T>
+struct RestrictLds;
+template <typename T, int Groups>
+__device__ inline void RestrictBodyIn(const CoupleArgs<T>& a, const int* list, int count, int wg,
+                                      RestrictLds<T>* sm);
 
 // FD boundary pass, defined with the FD kernels below.
 template <typename T>
@@ -413,6 +418,33 @@ __global__ __launch_bounds__(1024, BlockOccupancy<T>::kWavesPerSimd) void KBlock
     const size_t ord = static_cast<size_t>(a.block_rank[b]) * ncore + tid;
     a.stat_val[ord] = ssum;
     a.stat_use[ord] = static_cast<uint8_t>(scnt);
+  }
+  // ---- coupling steps on this block's fresh output (the barrier makes the
+  // workgroup's stores visible to all its threads and ends every read of the
+  // box that the restriction reuses)
+  if (a.ep_pro_beg != nullptr) {
+    // LevelCoupler::apply_half_time (level_coupler.cpp:438-478) of the points
+    // whose coarse cell lies in this block
+    const int p0 = a.ep_pro_beg[b], np = a.ep_pro_beg[b + 1] - p0;
+    if (np > 0) {
+      __syncthreads();
+      for (int j = tid; j < np; j += static_cast<int>(blockDim.x))
+        ProlongPoint<T>(a.ep_pro, a.ep_pro_pts + p0, np, 1, j);
+    }
+  }
+  if (a.ep_res_beg != nullptr) {
+    // LevelCoupler::restrict (level_coupler.cpp:613-654) of the points whose
+    // fine cell lies in this block, 32 points per pass
+    const int p0 = a.ep_res_beg[b], np = a.ep_res_beg[b + 1] - p0;
+    if (np > 0) {
+      constexpr int kGroups = 1024 / 32;
+      static_assert(sizeof(plane) >= kGroups * sizeof(RestrictLds<T>), "restriction LDS must fit the box");
+      RestrictLds<T>* rs = reinterpret_cast<RestrictLds<T>*>(&plane[0][0]);
+      for (int w = 0; w * kGroups < np; ++w) {
+        __syncthreads();
+        RestrictBodyIn<T, kGroups>(a.ep_res, a.ep_res_pts + p0, np, w, rs);
+      }
+    }
   }
 }
 
@@ -1495,11 +1527,19 @@ constexpr int kRestrictGroups = 8;
 
 // Groups = threads per workgroup / 32; wg = this workgroup's index among the
 // restriction workgroups.
+template <typename T>
+struct RestrictLds {
+  T fneq[kQ][kQ + 1];
+  T mac[4];
+};
+
+// Groups 32-lane groups, group g working in sm[g].
 template <typename T, int Groups>
-__device__ inline void RestrictBody(const CoupleArgs<T>& a, const int* list, int count, int wg) {
-  __shared__ T fneq[Groups][kQ][kQ + 1];
-  __shared__ T mac[Groups][4];
+__device__ inline void RestrictBodyIn(const CoupleArgs<T>& a, const int* list, int count, int wg,
+                                      RestrictLds<T>* sm) {
   const int g = threadIdx.x >> 5, lane = threadIdx.x & 31;
+  T(*fneq)[kQ + 1] = sm[g].fneq;
+  T* mac = sm[g].mac;
   const int j = wg * Groups + g;
   const bool active = j < count && lane < kQ;
   int fi = 0, fj = 0, fk = 0;
@@ -1515,12 +1555,12 @@ __device__ inline void RestrictBody(const CoupleArgs<T>& a, const int* list, int
       T m[kMacro];
       ReadMacro(fb, kTileCells, (x * kTile + y) * kTile + z, m);
 #pragma unroll
-      for (int q = 0; q < kQ; ++q) fneq[g][lane][q] = m[4 + q];
+      for (int q = 0; q < kQ; ++q) fneq[lane][q] = m[4 + q];
       if (lane == 0) {
-        mac[g][0] = m[0];
-        mac[g][1] = m[1];
-        mac[g][2] = m[2];
-        mac[g][3] = m[3];
+        mac[0] = m[0];
+        mac[1] = m[1];
+        mac[2] = m[2];
+        mac[3] = m[3];
       }
     }
   }
@@ -1532,15 +1572,21 @@ __device__ inline void RestrictBody(const CoupleArgs<T>& a, const int* list, int
   for (int jp = 1; jp < kQ; ++jp) {
     const int x = fi + Cx(jp), y = fj + Cy(jp), z = fk + Cz(jp);
     if (x < 0 || x >= kTile || y < 0 || y >= kTile || z < 0 || z >= kTile) continue;
-    sum += fneq[g][jp][q];
+    sum += fneq[jp][q];
     ++cnt;
   }
-  const T avg = (fneq[g][0][q] + sum) / static_cast<T>(cnt + 1);
-  const T u[3] = {mac[g][1], mac[g][2], mac[g][3]};
+  const T avg = (fneq[0][q] + sum) / static_cast<T>(cnt + 1);
+  const T u[3] = {mac[1], mac[2], mac[3]};
   const T usqr = NormSqr(u);
   T* cb = (a.par.v[r[9]] ? a.pops1 : a.pops0) + static_cast<size_t>(r[0]) * kQ * kTileCells;
   cb[static_cast<size_t>(q) * kTileCells + (r[1] * kTile + r[2]) * kTile + r[3]] =
-      Feq(q, mac[g][0], u, usqr) + a.restrict_scale * avg;
+      Feq(q, mac[0], u, usqr) + a.restrict_scale * avg;
+}
+
+template <typename T, int Groups>
+__device__ inline void RestrictBody(const CoupleArgs<T>& a, const int* list, int count, int wg) {
+  __shared__ RestrictLds<T> sm[Groups];
+  RestrictBodyIn<T, Groups>(a, list, count, wg, sm);
 }
 
 template <typename T>

@@ -225,6 +225,17 @@ struct TileArgs {
   const int* pc_list;
   int pc_count;
   int pc_wgs;
+  // coupling steps that follow this update, done by the coupled blocks' own
+  // workgroups after their stores (non-null: per block b, the active points
+  // ep_*_pts[ep_*_beg[b] .. ep_*_beg[b + 1])): the half-time prolongation
+  // whose coarse cells this launch updates, the restriction whose fine cells
+  // it updates
+  const int* ep_pro_beg;
+  const int* ep_pro_pts;
+  CoupleArgs<T> ep_pro;
+  const int* ep_res_beg;
+  const int* ep_res_pts;
+  CoupleArgs<T> ep_res;
   SimConst sc;
 };
 

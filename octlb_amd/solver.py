@@ -145,6 +145,19 @@ class GpuSolver:
         push, the rest pull (default)."""
         abi.check(self._lib.octlb_dev_set_kernel_variant(self._h, int(variant)), "set_kernel_variant")
 
+    def set_coupling_epilogues(self, mode):
+        """Bit 0: the half-time prolongation runs in the preceding update's
+        launch; bit 1: the restriction runs in the finest level's second
+        update launch (default 3; the autotuner decides per workload)."""
+        abi.check(self._lib.octlb_dev_set_coupling_epilogues(self._h, int(mode)), "set_coupling_epilogues")
+
+    def coupling_epilogues(self):
+        """(mode in effect, autotuner's [ms without, ms with] per coarse step)."""
+        m = C.c_int32()
+        ms = np.zeros(2)
+        abi.check(self._lib.octlb_dev_coupling_epilogues(self._h, C.byref(m), abi.f64p(ms)), "coupling_epilogues")
+        return int(m.value), ms
+
     def kernel_variant(self):
         """(variant in use, blocks updated by push)."""
         v, n = C.c_int32(), C.c_int32()

@@ -79,6 +79,27 @@ def test_cavity64_from_rest():
     assert abs(g.average_rho() - o.average_rho()) <= 1e-13
 
 
+def test_cavity64_1000_steps():
+    """The plain reference configuration itself: the 64^3 lid-driven cavity
+    (tau 0.6, lid 0.05, halfway bounce-back walls) from rest for 1,000 steps,
+    FP64, against the oracle (OpenMP on the host): populations and the
+    ConstRho-free average density to 1e-12."""
+    import os
+    case = cases.cavity(64, tau=0.6, u_lid=0.05)
+    o = make_oracle(case)
+    o.set_threads(max(1, min(16, os.cpu_count() or 1)))
+    g = make_gpu(case)
+    print("cavity 64^3: 1,000 steps on the oracle and the device", flush=True)
+    o.advance(1000)
+    g.advance(1000)
+    a, b = g.download_core(), o.download_core()
+    err = rel_err(a, b)
+    print(f"max relative population difference {err:.3e}", flush=True)
+    assert err <= 1e-12
+    assert abs(g.average_rho() - o.average_rho()) <= 1e-13
+    assert abs(g.mass() - o.mass()) <= 1e-12 * o.mass()
+
+
 def test_uniform_multiblock_bitwise():
     case = cases.cylinder_channel(bricks_x=4, bricks_yz=2, base_level=0, fine=False, xc=0.5,
                                   radius=0.15)

@@ -86,3 +86,31 @@ def test_autotune_keeps_state_and_picks_a_variant():
     np.testing.assert_array_equal(sim.download_core(), ref.download_core())
     sim.close()
     ref.close()
+
+
+def test_coupling_epilogues_are_bit_identical():
+    """The half-time prolongation and the restriction run inside the update
+    launches or as separate kernels: the 2-level channel's FP64 and FP32
+    states after 12 coarse steps (eager) and after 16 more (graph replay) are
+    the same bit for bit, and the autotuner leaves the state unchanged."""
+    case = cases.cylinder_channel(bricks_x=2, base_level=1, fine=True, xc=0.5, radius=0.1,
+                                  refine_lo=0.25, refine_hi=1.0)
+    for prec in (64, 32):
+        out = {}
+        for mode in (0, 3):
+            sim = case.apply(GpuSolver(case.topo, case.params(prec)))
+            sim.set_coupling_epilogues(mode)
+            assert sim.coupling_epilogues()[0] == mode
+            sim.advance(12)
+            a = sim.download_core()
+            sim.prepare_graphs()
+            sim.advance_graph(16)
+            out[mode] = (a, sim.download_core(), sim.drag(case.drag_bbox))
+            if mode == 3:
+                before = sim.download_core()
+                sim.autotune(reps=2)
+                np.testing.assert_array_equal(sim.download_core(), before)
+                print(f"fp{prec}: autotuned epilogues {sim.coupling_epilogues()}", flush=True)
+            sim.close()
+        for k in range(3):
+            np.testing.assert_array_equal(out[0][k], out[3][k])
