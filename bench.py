@@ -367,6 +367,57 @@ FP64_VALU_PEAK_GFLOPS = 78600.0
 FLOPS_PER_CELL_TRIANGLE = 101  # 3 edges x (closest point 23 + distance 8) + distance 8
 
 
+def run_migrate(precision=32, mark_frac=0.1, seed=3, reps=3):
+    """State migration after a forest change (octlb_dev_migrate) on the
+    synthetic 3-level forest: a seeded 10% of its level-1 blocks refined
+    once more and 2:1 balanced, the state moved onto it (copies +
+    refinements) and back (copies + coarsenings).  Wall time per migration,
+    host block matching, task upload and the device launch included; the
+    rate counts each old block's populations read once and each new
+    block's written once."""
+    from octlb_amd import cases
+    from octlb_amd.solver import GpuSolver
+
+    case = build_case("synthetic")
+    f = cases.synthetic_forest(bricks=8, max_level=2, frac=(0.5, 0.25), seed=7)
+    ta = f.topology(8, 1)
+    assert ta.num_blocks == case.topo.num_blocks
+    rng = np.random.default_rng(seed)
+    mark = ((ta.level == 1) & (rng.random(ta.num_blocks) < mark_frac)).astype(np.uint8)
+    f.refine_marked(mark)
+    f.balance()
+    tb = f.topology(8, 1)
+    src = case.apply(GpuSolver(case.topo, case.params(precision)))
+    src.advance(2)
+    dst = GpuSolver(tb, case.params(precision))
+    dst.set_bc_kinds(cases.walls_kinds(tb))
+    back = GpuSolver(case.topo, case.params(precision))
+    back.set_bc_kinds(case.kinds)
+    dst.migrate_from(src)
+    back.migrate_from(dst)
+    fw, bw = [], []
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        dst.migrate_from(src)
+        t1 = time.perf_counter()
+        back.migrate_from(dst)
+        fw.append(t1 - t0)
+        bw.append(time.perf_counter() - t1)
+    for sim in (src, dst, back):
+        sim.close()
+    elem = 4 if precision == 32 else 8
+    per_block = 19 * 512 * elem
+    na, nb_ = ta.num_blocks, tb.num_blocks
+    fms, bms = 1e3 * min(fw), 1e3 * min(bw)
+    return {"workload": ("synthetic 3-level forest (8^3 bricks) -> 10% of its level-1 blocks refined "
+                         "once more, 2:1 balanced -> back"),
+            "precision": f"fp{precision}", "blocks": [na, nb_],
+            "marked_level1_blocks": int(mark.sum()),
+            "forward_ms": round(fms, 3), "back_ms": round(bms, 3),
+            "forward_gbs": round((na + nb_) * per_block / (fms * 1e-3) / 1e9, 1),
+            "back_gbs": round((na + nb_) * per_block / (bms * 1e-3) / 1e9, 1)}
+
+
 def run_link_build(with_cpu, segments=512, rings=8, reps=3, cpu_blocks=24):
     """Bouzidi link build from a triangle soup (BouzidiLinkData::Build) on
     the headline forest: z-spanning cylinder of D = 16 fine cells as
@@ -439,6 +490,8 @@ def main():
     ap.add_argument("--synthetic-steps", type=int, default=40)
     ap.add_argument("--no-links", action="store_true",
                     help="skip the Bouzidi link-build measurement")
+    ap.add_argument("--no-migrate", action="store_true",
+                    help="skip the state-migration (forest change) measurement")
     ap.add_argument("--no-ensemble", action="store_true",
                     help="skip the 64-cavity ensemble measurement")
     ap.add_argument("--ensemble-steps", type=int, default=40)
@@ -532,6 +585,12 @@ def main():
         log("bouzidi link build")
         links = run_link_build(not args.no_cpu)
         log(f"  {links['links']} links from {links['triangles']} triangles in {links['gpu_ms']} ms")
+    mig = None
+    if not args.no_migrate and args.workload == "cylinder":
+        log("state migration after a forest change")
+        mig = run_migrate(head)
+        log(f"  {mig['blocks'][0]} -> {mig['blocks'][1]} blocks: {mig['forward_ms']} ms "
+            f"({mig['forward_gbs']} GB/s), back {mig['back_ms']} ms ({mig['back_gbs']} GB/s)")
     adjo = adjc = None
     if not args.no_adjoint and args.workload == "cylinder":
         log("adjoint")
@@ -601,6 +660,7 @@ def main():
         "synthetic_amr": synth,
         "ensemble": ens,
         "link_build": links,
+        "migration": mig,
         "adjoint": adjc,
         "adjoint_single_level": adjo,
         "drag_query_us": {f"fp{p}": (round(res[p]["drag_query_us"], 2) if res[p]["drag_query_us"] else None)

@@ -340,7 +340,10 @@ int octlb_build_links_noalloc(int32_t num_blocks, int32_t n, const double* bound
  * coupler's prolongation rule, level_coupler.cpp:163-214); a block
  * coarsened from eight src blocks takes the mean rho, the
  * momentum-conserving mean u and s_r times the mean fneq of its fine cells
- * (s_r = tau/(tau - 1/4)).  Host round trip: once per adapt. */
+ * (s_r = tau/(tau - 1/4)).  The host matches blocks by hashed (level,
+ * integer origin) keys and uploads one task per new block; one device
+ * launch (a workgroup per new block) moves the populations from src's
+ * buffers into dst's, FP64 arithmetic in both precisions. */
 int octlb_dev_migrate(octlb_dev* dst, octlb_dev* src);
 
 /* ------------------------------------------------------------------ */

@@ -301,15 +301,9 @@ def fourier_velocity(topo: abi.TopologyArrays, A, kmax: int = 4):
     return u
 
 
-def synthetic_amr(bricks: int = 8, max_level: int = 2, frac: tuple = (0.5, 0.25), seed: int = 7,
-                  amp: float = 0.01, kmax: int = 4, tau: float = 0.6) -> Case:
-    """Unit cube of bricks^3 level-0 blocks refined where a seeded sum of 8
-    Gaussian blobs is largest: level-0 octants whose centre value lies in the
-    top frac[0] of the domain volume are refined, then level-1 octants
-    likewise until frac[1] of the volume is at level 2; 2:1 balanced.  The
-    initial velocity is a divergence-free random Fourier field (|k| <= kmax)
-    scaled to max |u| = amp, loaded as feq(1, u).  Walls on all six faces
-    (no spec: no-slip -> kBounceBack, bc_installer.h:44-58)."""
+def synthetic_forest(bricks: int = 8, max_level: int = 2, frac: tuple = (0.5, 0.25), seed: int = 7) -> Forest:
+    """The forest of synthetic_amr: unit cube of bricks^3 level-0 blocks
+    refined where a seeded sum of 8 Gaussian blobs is largest."""
     centres, sigmas = gaussian_blobs(seed)
     f = Forest([0.0, 0.0, 0.0, 1.0, 1.0, 1.0], (bricks, bricks, bricks))
     for lv in range(max_level):
@@ -327,9 +321,27 @@ def synthetic_amr(bricks: int = 8, max_level: int = 2, frac: tuple = (0.5, 0.25)
         mark[np.flatnonzero(at)[take]] = 1
         f.refine_marked(mark)
         f.balance()
-    topo = f.topology(8, 1)
-    kinds = np.zeros((topo.num_blocks, 8, 8, 8), np.uint8)
+    return f
+
+
+def walls_kinds(topo) -> np.ndarray:
+    """BcKind bytes of a forest with no-slip walls on all six faces."""
+    kinds = np.zeros((topo.num_blocks, topo.n, topo.n, topo.n), np.uint8)
     stamp_tree_boundary(kinds, topo.tree_boundary, [])
+    return kinds
+
+
+def synthetic_amr(bricks: int = 8, max_level: int = 2, frac: tuple = (0.5, 0.25), seed: int = 7,
+                  amp: float = 0.01, kmax: int = 4, tau: float = 0.6) -> Case:
+    """Unit cube of bricks^3 level-0 blocks refined where a seeded sum of 8
+    Gaussian blobs is largest: level-0 octants whose centre value lies in the
+    top frac[0] of the domain volume are refined, then level-1 octants
+    likewise until frac[1] of the volume is at level 2; 2:1 balanced.  The
+    initial velocity is a divergence-free random Fourier field (|k| <= kmax)
+    scaled to max |u| = amp, loaded as feq(1, u).  Walls on all six faces
+    (no spec: no-slip -> kBounceBack, bc_installer.h:44-58)."""
+    topo = synthetic_forest(bricks, max_level, frac, seed).topology(8, 1)
+    kinds = walls_kinds(topo)
     u = fourier_velocity(topo, fourier_spectrum(seed + 1, kmax), kmax)
     u *= amp / np.sqrt((u ** 2).sum(-1)).max()
     rho = np.ones(kinds.shape)

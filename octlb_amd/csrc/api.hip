@@ -1847,31 +1847,9 @@ int octlb_dev_migrate(octlb_dev* dst, octlb_dev* src) {
     if (dst->nb == 0 || src->nb == 0) throw ArgError("migrate: empty forest");
     src->Sync();
     dst->Sync();
-    const int n = dst->n, NP = dst->NP, h = dst->h;
-    const size_t view = static_cast<size_t>(dst->NS) * kQ;
-    auto slot = [&](int x, int y, int z) { return static_cast<size_t>(((x + h) * NP + y + h) * NP + z + h); };
-    auto get = [&](octlb_dev* d, int b, std::vector<double>& v) {
-      v.assign(view, 0.0);
-      if (d->prec == 64) ToHostBlock<double>(d, b, v.data()); else ToHostBlock<float>(d, b, v.data());
-    };
-    auto put = [&](int b, const std::vector<double>& v) {
-      if (dst->prec == 64) FromHostBlock<double>(dst, b, v.data()); else FromHostBlock<float>(dst, b, v.data());
-    };
-    // olb::lbm::computeRhoU + fneq of one cell
-    auto macro = [](const double* f, double& rho, double* u, double* fneq) {
-      rho = 0.0;
-      for (int i = 0; i < kQ; ++i) rho += f[i];
-      rho += 1.0;
-      u[0] = u[1] = u[2] = 0.0;
-      for (int i = 0; i < kQ; ++i)
-        for (int d = 0; d < 3; ++d) u[d] += f[i] * Ci(i, d);
-      for (int d = 0; d < 3; ++d) u[d] /= rho;
-      const double usqr = NormSqr(u);
-      for (int i = 0; i < kQ; ++i) fneq[i] = f[i] - Feq<double>(i, rho, u, usqr);
-    };
     // Blocks of both forests keyed by (level, integer origin in that level's
-    // block widths): a hash lookup per destination block, so migration is
-    // linear in the block count.
+    // block widths): a hash lookup per destination block, so the matching is
+    // linear in the block count.  The populations move on the device.
     const double* sb = src->bounds.data();
     const double* db = dst->bounds.data();
     double gmin[3] = {db[0], db[1], db[2]};
@@ -1880,121 +1858,125 @@ int octlb_dev_migrate(octlb_dev* dst, octlb_dev* src) {
       for (int b = 0; b < src->nb; ++b) gmin[k] = std::min(gmin[k], sb[6 * static_cast<size_t>(b) + k]);
     }
     const double w0 = (db[3] - db[0]) * std::ldexp(1.0, dst->level[0]);  // level-0 block width
+    // key = level (5 bits) | x, y, z index of the origin in that level's
+    // block widths (19 bits each)
+    constexpr int64_t kIdxLimit = int64_t(1) << 19;
+    auto pack = [](int64_t lv, int64_t ix, int64_t iy, int64_t iz) {
+      return (static_cast<uint64_t>(lv) << 57) | (static_cast<uint64_t>(ix) << 38) |
+             (static_cast<uint64_t>(iy) << 19) | static_cast<uint64_t>(iz);
+    };
     auto key_of = [&](const double* bb, int lv) {
       const double w = std::ldexp(w0, -lv);
       if (std::fabs((bb[3] - bb[0]) - w) > 1e-9 * w0) throw ArgError("migrate: block width does not follow its level");
-      int64_t ix[3];
+      if (lv < 0 || lv >= 31) throw ArgError("migrate: level out of range");
+      std::array<int64_t, 4> key{lv, 0, 0, 0};
       for (int k = 0; k < 3; ++k) {
         const double c = (bb[k] - gmin[k]) / w;
-        ix[k] = std::llround(c);
-        if (std::fabs(c - static_cast<double>(ix[k])) > 1e-6) throw ArgError("migrate: block origin off its level's grid");
+        key[1 + k] = std::llround(c);
+        if (std::fabs(c - static_cast<double>(key[1 + k])) > 1e-6) throw ArgError("migrate: block origin off its level's grid");
+        if (key[1 + k] < 0 || key[1 + k] >= kIdxLimit / 2) throw ArgError("migrate: forest wider than 2^18 blocks per level");
       }
-      return std::array<int64_t, 4>{lv, ix[0], ix[1], ix[2]};
+      return key;
     };
-    struct KeyHash {
-      size_t operator()(const std::array<int64_t, 4>& k) const {
-        uint64_t hsh = 1469598103934665603ull;
-        for (int64_t v : k) hsh = (hsh ^ static_cast<uint64_t>(v)) * 1099511628211ull;
-        return static_cast<size_t>(hsh);
+    // open-addressing table of src blocks (linear probing, power-of-two size)
+    size_t cap = 16;
+    while (cap < static_cast<size_t>(src->nb) * 2) cap <<= 1;
+    constexpr uint64_t kEmpty = ~uint64_t(0);
+    std::vector<uint64_t> keys(cap, kEmpty);
+    std::vector<int> vals(cap, -1);
+    const int shift = 64 - __builtin_ctzll(cap);
+    auto slot_of = [&](uint64_t k) { return static_cast<size_t>((k * 0x9E3779B97F4A7C15ull) >> shift); };
+    for (int s = 0; s < src->nb; ++s) {
+      const auto k4 = key_of(sb + 6 * static_cast<size_t>(s), src->level[s]);
+      const uint64_t k = pack(k4[0], k4[1], k4[2], k4[3]);
+      size_t i = slot_of(k);
+      while (keys[i] != kEmpty && keys[i] != k) i = (i + 1) & (cap - 1);
+      if (keys[i] == k) throw ArgError("migrate: two source blocks share level and bounds");
+      keys[i] = k;
+      vals[i] = s;
+    }
+    auto find = [&](const std::array<int64_t, 4>& k4) {
+      const uint64_t k = pack(k4[0], k4[1], k4[2], k4[3]);
+      for (size_t i = slot_of(k);; i = (i + 1) & (cap - 1)) {
+        if (keys[i] == k) return vals[i];
+        if (keys[i] == kEmpty) return -1;
       }
     };
-    std::unordered_map<std::array<int64_t, 4>, int, KeyHash> src_at;
-    src_at.reserve(static_cast<size_t>(src->nb) * 2);
-    for (int s = 0; s < src->nb; ++s) src_at[key_of(sb + 6 * static_cast<size_t>(s), src->level[s])] = s;
-    auto find = [&](const std::array<int64_t, 4>& k) {
-      auto it = src_at.find(k);
-      return it == src_at.end() ? -1 : it->second;
-    };
-    const double tau = 1.0 / dst->sc.omega;
-    const double sp = (tau - 0.25) / tau, sr = tau / (tau - 0.25);
-    std::vector<double> in, out;
+    std::vector<MigTask> task(static_cast<size_t>(dst->nb));
     for (int b = 0; b < dst->nb; ++b) {
-      const double* bb = db + 6 * static_cast<size_t>(b);
-      const int lv = dst->level[b];
-      const auto key = key_of(bb, lv);
-      const int match = find(key);
-      if (match >= 0) {
-        get(src, match, in);
-        put(b, in);
+      const auto key = key_of(db + 6 * static_cast<size_t>(b), dst->level[b]);
+      const int lv = static_cast<int>(key[0]);
+      MigTask& t = task[static_cast<size_t>(b)];
+      std::fill(t.src, t.src + 8, -1);
+      t.oct[0] = t.oct[1] = t.oct[2] = 0;
+      t.src[0] = find(key);
+      if (t.src[0] >= 0) {
+        t.mode = kMigCopy;
         continue;
       }
-      const int parent = lv > 0 ? find({lv - 1, key[1] >> 1, key[2] >> 1, key[3] >> 1}) : -1;
-      std::vector<int> kids;
-      if (parent < 0)
-        for (int c = 0; c < 8; ++c) {
-          const int s = find({lv + 1, 2 * key[1] + (c & 1), 2 * key[2] + ((c >> 1) & 1), 2 * key[3] + (c >> 2)});
-          if (s >= 0) kids.push_back(s);
-        }
-      out.assign(view, 0.0);
-      if (parent >= 0) {
-        // every slot of the fine block, halo included, from the parent's
-        // halo-inclusive view (a fine halo slot lies in the parent's core or
-        // one-slot halo)
-        get(src, parent, in);
-        const double* pb = sb + 6 * static_cast<size_t>(parent);
-        const double pcw = (pb[3] - pb[0]) / n, fcw = (bb[3] - bb[0]) / n;
-        for (int x = -h; x < n + h; ++x)
-          for (int y = -h; y < n + h; ++y)
-            for (int z = -h; z < n + h; ++z) {
-              const int cx = static_cast<int>(std::floor((bb[0] + (x + 0.5) * fcw - pb[0]) / pcw));
-              const int cy = static_cast<int>(std::floor((bb[1] + (y + 0.5) * fcw - pb[1]) / pcw));
-              const int cz = static_cast<int>(std::floor((bb[2] + (z + 0.5) * fcw - pb[2]) / pcw));
-              if (cx < -h || cx >= n + h || cy < -h || cy >= n + h || cz < -h || cz >= n + h)
-                throw StateError("migrate: fine cell outside its parent's view");
-              double rho, u[3], fneq[kQ];
-              macro(&in[slot(cx, cy, cz) * kQ], rho, u, fneq);
-              const double usqr = NormSqr(u);
-              double* f = &out[slot(x, y, z) * kQ];
-              for (int i = 0; i < kQ; ++i) f[i] = Feq<double>(i, rho, u, usqr) + sp * fneq[i];
-            }
-      } else if (kids.size() == 8) {
-        const double cw = (bb[3] - bb[0]) / n;
-        std::vector<double> acc(static_cast<size_t>(n) * n * n * (kQ + 4), 0.0);  // rho, rho u, fneq sums
-        for (int s : kids) {
-          get(src, s, in);
-          const double* kb = sb + 6 * static_cast<size_t>(s);
-          const double kcw = (kb[3] - kb[0]) / n;
-          for (int x = 0; x < n; ++x)
-            for (int y = 0; y < n; ++y)
-              for (int z = 0; z < n; ++z) {
-                const int cx = static_cast<int>(std::floor((kb[0] + (x + 0.5) * kcw - bb[0]) / cw));
-                const int cy = static_cast<int>(std::floor((kb[1] + (y + 0.5) * kcw - bb[1]) / cw));
-                const int cz = static_cast<int>(std::floor((kb[2] + (z + 0.5) * kcw - bb[2]) / cw));
-                if (cx < 0 || cx >= n || cy < 0 || cy >= n || cz < 0 || cz >= n)
-                  throw StateError("migrate: child cell outside its parent");
-                double rho, u[3], fneq[kQ];
-                macro(&in[slot(x, y, z) * kQ], rho, u, fneq);
-                double* a = &acc[((static_cast<size_t>(cx) * n + cy) * n + cz) * (kQ + 4)];
-                a[0] += rho;
-                for (int d = 0; d < 3; ++d) a[1 + d] += rho * u[d];
-                for (int i = 0; i < kQ; ++i) a[4 + i] += fneq[i];
-              }
-        }
-        for (int x = 0; x < n; ++x)
-          for (int y = 0; y < n; ++y)
-            for (int z = 0; z < n; ++z) {
-              const double* a = &acc[((static_cast<size_t>(x) * n + y) * n + z) * (kQ + 4)];
-              const double rho = a[0] / 8.0;
-              const double u[3] = {a[1] / a[0], a[2] / a[0], a[3] / a[0]};
-              const double usqr = NormSqr(u);
-              double* f = &out[slot(x, y, z) * kQ];
-              for (int i = 0; i < kQ; ++i) f[i] = Feq<double>(i, rho, u, usqr) + sr * (a[4 + i] / 8.0);
-            }
-        // halo slots repeat the nearest core cell (the block's own boundary
-        // layer): a defined value for the shell slots no exchange refills
-        for (int x = -h; x < n + h; ++x)
-          for (int y = -h; y < n + h; ++y)
-            for (int z = -h; z < n + h; ++z) {
-              if (x >= 0 && x < n && y >= 0 && y < n && z >= 0 && z < n) continue;
-              const int cx = std::min(std::max(x, 0), n - 1), cy = std::min(std::max(y, 0), n - 1),
-                        cz = std::min(std::max(z, 0), n - 1);
-              std::copy_n(&out[slot(cx, cy, cz) * kQ], kQ, &out[slot(x, y, z) * kQ]);
-            }
-      } else {
-        throw ArgError("migrate: block " + std::to_string(b) + " has no source block, parent or eight children");
+      t.src[0] = lv > 0 ? find({lv - 1, key[1] >> 1, key[2] >> 1, key[3] >> 1}) : -1;
+      if (t.src[0] >= 0) {
+        t.mode = kMigRefine;
+        for (int k = 0; k < 3; ++k) t.oct[k] = static_cast<int>(key[1 + k] & 1);
+        continue;
       }
-      put(b, out);
+      t.mode = kMigCoarsen;
+      for (int c = 0; c < 8; ++c) {
+        t.src[c] = find({lv + 1, 2 * key[1] + (c & 1), 2 * key[2] + ((c >> 1) & 1), 2 * key[3] + (c >> 2)});
+        if (t.src[c] < 0)
+          throw ArgError("migrate: block " + std::to_string(b) + " has no source block, parent or eight children");
+      }
     }
+    const double tau = 1.0 / dst->sc.omega;
+    // the new blocks grouped by rule: copies, refinements, coarsenings
+    int count[3] = {0, 0, 0};
+    for (const MigTask& t : task) ++count[t.mode];
+    if (dst->padded && count[0] != dst->nb)
+      throw ArgError("migrate: refinement and coarsening need the blocked layout (8^3 blocks, halo 1)");
+    // one upload: the tasks, then the block list of each rule
+    const size_t task_bytes = task.size() * sizeof(MigTask);
+    std::vector<char> stage(task_bytes + task.size() * sizeof(int));
+    std::memcpy(stage.data(), task.data(), task_bytes);
+    int* list = reinterpret_cast<int*>(stage.data() + task_bytes);
+    int at[3] = {0, count[0], count[0] + count[1]};
+    for (int b = 0; b < dst->nb; ++b) list[at[task[static_cast<size_t>(b)].mode]++] = b;
+    char* d_stage = DevAlloc<char>(stage.size());
+    MigTask* d_task = reinterpret_cast<MigTask*>(d_stage);
+    int* d_list = reinterpret_cast<int*>(d_stage + task_bytes);
+    try {
+      HIP_CHECK(hipMemcpyAsync(d_stage, stage.data(), stage.size(), hipMemcpyHostToDevice, dst->stream));
+      auto launch = [&](auto zero) {
+        using T = decltype(zero);
+        MigArgs<T> a{};
+        a.task = d_task;
+        a.src.bt = src->d_bt;
+        a.dst.bt = dst->d_bt;
+        for (int k = 0; k < 2; ++k) {
+          a.src.pops[k] = static_cast<T*>(src->pops[k]);
+          a.src.shells[k] = static_cast<T*>(src->shells[k]);
+          a.dst.pops[k] = static_cast<T*>(dst->pops[k]);
+          a.dst.shells[k] = static_cast<T*>(dst->shells[k]);
+        }
+        a.src.par = src->Par();
+        a.dst.par = dst->Par();
+        a.n = dst->n;
+        a.h = dst->h;
+        a.NP = dst->NP;
+        a.NS = dst->NS;
+        a.plane = dst->plane;
+        a.padded = dst->padded ? 1 : 0;
+        a.sp = (tau - 0.25) / tau;
+        a.sr = tau / (tau - 0.25);
+        LaunchMigrate<T>(a, d_list, count, dst->stream);
+        HIP_CHECK(hipGetLastError());
+      };
+      if (dst->prec == 64) launch(0.0); else launch(0.0f);
+      dst->Sync();
+    } catch (...) {
+      (void)hipFree(d_stage);
+      throw;
+    }
+    HIP_CHECK(hipFree(d_stage));
     // the time state moves with the populations: the step counter (inlet
     // ramps) and the lagged ConstRho average
     dst->step = src->step;

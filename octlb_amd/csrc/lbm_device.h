@@ -507,6 +507,38 @@ template <typename T>
 void LaunchMacro(const T* p0, const T* p1, const ParArray& par, const BlockTab* bt, const SimConst& sc, int nb,
                  double* rho, double* u, hipStream_t s);
 
+// ------------------------------------------------------ state migration
+// What one block of the adapted forest takes from the old forest.
+enum MigMode : int { kMigCopy = 0, kMigRefine = 1, kMigCoarsen = 2 };
+struct MigTask {
+  int mode;
+  // copy: src[0] = the block with the same level and bounds; refine:
+  // src[0] = the parent, oct = this block's octant in it (0/1 per axis);
+  // coarsen: src[c] = the child at octant c = ox + 2 oy + 4 oz
+  int src[8];
+  int oct[3];
+};
+// One forest's populations as the kernel reads (old) or writes (new) them:
+// block b lives in buffer par[bt[b].level].
+template <typename T>
+struct MigSide {
+  const BlockTab* bt;
+  T* pops[2];
+  T* shells[2];
+  ParArray par;
+};
+template <typename T>
+struct MigArgs {
+  const MigTask* task;
+  MigSide<T> src, dst;
+  int n, h, NP, NS, plane, padded;
+  double sp, sr;  // (tau - 1/4)/tau and its inverse
+};
+// list: the new blocks of each rule, count[0] copies, then count[1]
+// refinements, then count[2] coarsenings.
+template <typename T>
+void LaunchMigrate(const MigArgs<T>& a, const int* list, const int count[3], hipStream_t s);
+
 }  // namespace octlb
 
 #endif  // OCTLB_AMD_CSRC_LBM_DEVICE_H_

@@ -63,6 +63,8 @@ def lib():
             "ol_apply_restriction": ([vp, C.c_int32], C.c_int),
             "ol_build_links": ([C.c_int32, C.c_int32, f64p, u8p, C.c_int64, f64p, C.c_int64, i32p, f64p],
                                C.c_int64),
+            "ol_migrate_block": ([C.c_int32, C.c_int32, C.c_int32, i32p, C.POINTER(f64p), C.c_double, f64p],
+                                 C.c_int),
             "ol_adjoint": ([vp, C.c_int32, C.c_int32, C.c_int32, f64p, f64p, C.c_double, f64p, C.c_uint32,
                             C.c_int64, i32p, f64p, f64p, f64p], C.c_int),
         }
@@ -228,6 +230,21 @@ def build_links(bounds, n, material, triangles):
     q = np.zeros(max(cnt, 1))
     h.ol_build_links(*args, cnt, link.ctypes.data_as(i32p), q.ctypes.data_as(f64p))
     return link[:cnt], q[:cnt]
+
+
+def migrate_block(mode, n, h, oct, views, tau):
+    """ol_migrate_block: the halo-inclusive view [(n+2h)^3, 19] of one block
+    of an adapted forest.  mode 0 copy / 1 refine (views = [parent], oct =
+    octant) / 2 coarsen (views = the eight children in octant order)."""
+    E = n + 2 * h
+    vs = [np.ascontiguousarray(v, np.float64).reshape(E ** 3, Q) for v in views]
+    f64p = C.POINTER(C.c_double)
+    arr = (f64p * 8)(*([v.ctypes.data_as(f64p) for v in vs] + [None] * (8 - len(vs))))
+    oc = np.ascontiguousarray(oct, np.int32)
+    out = np.zeros((E ** 3, Q))
+    _ok(lib().ol_migrate_block(int(mode), int(n), int(h), oc.ctypes.data_as(C.POINTER(C.c_int32)), arr,
+                               float(tau), out.ctypes.data_as(f64p)), "migrate_block")
+    return out
 
 
 def setup(sim, case, precision_note=None):

@@ -1948,3 +1948,73 @@ int ol_adjoint(ol_sim* s, int32_t steps, int32_t win_begin, int32_t win_end, con
   free(xs); free(ys); free(lx); free(ly); free(ls); free(dq);
   return 0;
 }
+
+/* ---------------- state migration after a forest change ------------------
+ * The reference has no migration code; its dynamic-AMR rebuild is planned
+ * (doc/prd/octlb-framework.md:180,521).  The rules rescale the
+ * non-equilibrium part the way the coupler does between levels:
+ * prolong_scale_ = (tau - 1/4)/tau, restrict_scale_ = tau/(tau - 1/4)
+ * (level_coupler.cpp:226-228), with olb's computeRhoU / computeFneq of each
+ * cell (level_coupler.cpp:187).
+ *   refine:  every slot of the fine block from the parent cell under it,
+ *            feq(rho, u) + s_p fneq;
+ *   coarsen: each coarse cell from its eight fine cells (children in octant
+ *            order, each child's cells in raster order): mean rho, u = sum
+ *            rho u / sum rho, s_r times the mean fneq; halo slots repeat
+ *            the nearest core cell.                                        */
+static void migrate_macro(const double* f, double* rho, double* u, double* fneq) {
+  olb_rho_u(f, rho, u);
+  const double usqr = norm_sqr(u);
+  for (int i = 0; i < Q; ++i) fneq[i] = f[i] - feq2(i, *rho, u, usqr);
+}
+
+static int floor_half(int g) { return g >= 0 ? g / 2 : -((1 - g) / 2); }
+
+int ol_migrate_block(int32_t mode, int32_t n, int32_t h, const int32_t oct[3], const double* const* views,
+                     double tau, double* out) {
+  if (n <= 0 || h < 0 || !views || !out) return fail("ol_migrate_block: bad arguments");
+  if (mode < 0 || mode > 2) return fail("ol_migrate_block: mode out of range");
+  const int E = n + 2 * h;
+  const double sp = (tau - 0.25) / tau, sr = tau / (tau - 0.25);
+#define VSLOT(x, y, z) ((size_t)(((x) + h) * E + (y) + h) * E + (z) + h)
+  for (int x = -h; x < n + h; ++x)
+    for (int y = -h; y < n + h; ++y)
+      for (int z = -h; z < n + h; ++z) {
+        double* o = out + VSLOT(x, y, z) * Q;
+        if (mode == 0) {
+          memcpy(o, views[0] + VSLOT(x, y, z) * Q, sizeof(double) * Q);
+        } else if (mode == 1) {
+          const int px = floor_half(oct[0] * n + x), py = floor_half(oct[1] * n + y),
+                    pz = floor_half(oct[2] * n + z);
+          double rho, u[3], fneq[Q];
+          migrate_macro(views[0] + VSLOT(px, py, pz) * Q, &rho, u, fneq);
+          const double usqr = norm_sqr(u);
+          for (int i = 0; i < Q; ++i) o[i] = feq2(i, rho, u, usqr) + sp * fneq[i];
+        } else {
+          const int cx = x < 0 ? 0 : (x >= n ? n - 1 : x);
+          const int cy = y < 0 ? 0 : (y >= n ? n - 1 : y);
+          const int cz = z < 0 ? 0 : (z >= n ? n - 1 : z);
+          double acc[4 + Q] = {0.0};
+          for (int c = 0; c < 8; ++c) {
+            const int ox = c & 1, oy = (c >> 1) & 1, oz = c >> 2;
+            /* this child's fine cells under (cx,cy,cz), in its raster order */
+            for (int gx = 2 * cx; gx < 2 * cx + 2; ++gx)
+              for (int gy = 2 * cy; gy < 2 * cy + 2; ++gy)
+                for (int gz = 2 * cz; gz < 2 * cz + 2; ++gz) {
+                  if (gx / n != ox || gy / n != oy || gz / n != oz) continue;
+                  double rho, u[3], fneq[Q];
+                  migrate_macro(views[c] + VSLOT(gx % n, gy % n, gz % n) * Q, &rho, u, fneq);
+                  acc[0] += rho;
+                  for (int d = 0; d < 3; ++d) acc[1 + d] += rho * u[d];
+                  for (int i = 0; i < Q; ++i) acc[4 + i] += fneq[i];
+                }
+          }
+          const double rho = acc[0] / 8.0;
+          const double u[3] = {acc[1] / acc[0], acc[2] / acc[0], acc[3] / acc[0]};
+          const double usqr = norm_sqr(u);
+          for (int i = 0; i < Q; ++i) o[i] = feq2(i, rho, u, usqr) + sr * (acc[4 + i] / 8.0);
+        }
+      }
+#undef VSLOT
+  return 0;
+}

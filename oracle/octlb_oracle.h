@@ -94,6 +94,17 @@ int ol_adjoint(ol_sim* sim, int32_t steps, int32_t win_begin, int32_t win_end, c
                const double bbox[6], double rho0, const double u0[3], uint32_t kind_mask, int64_t nlinks,
                const int32_t* link, double* J, double dJ_du0[3], double* dJ_dq);
 
+/* State of one block of an adapted forest from the old forest's
+ * halo-inclusive views (AoS [(n+2h)^3][19] in slot order
+ * ((x+h)(n+2h) + y+h)(n+2h) + z+h).  mode 0: copy views[0]; mode 1:
+ * refine, views[0] = the parent, oct = this block's octant in it; mode 2:
+ * coarsen, views[c] = the child at octant c = ox + 2 oy + 4 oz.  Writes
+ * every slot of out.  Not part of the reference (its dynamic-AMR rebuild
+ * is a plan, doc/prd/octlb-framework.md:180,521): the rules apply the
+ * coupler's rescaling (level_coupler.cpp:187, 226-228) to whole blocks. */
+int ol_migrate_block(int32_t mode, int32_t n, int32_t h, const int32_t oct[3], const double* const* views,
+                     double tau, double* out);
+
 #ifdef __cplusplus
 }
 #endif

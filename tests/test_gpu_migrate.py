@@ -19,20 +19,79 @@ C = cases.C
 TAU = 0.6
 
 
-def _topo(refine):
+def _topo(refine, n=8):
     f = Forest(np.array([0.0, 0.0, 0.0, 2.0, 1.0, 1.0]), (2, 1, 1))
     f.refine_uniform(1)
     if refine:
         f.refine_box([0.01, 0.01, 0.01, 0.49, 0.49, 0.49], 2)
         f.balance()
-    return f.topology(8, 1)
+    return f.topology(n, 1)
 
 
-def _solver(topo):
+def _solver(topo, precision=64):
     from octlb_amd import abi
-    s = GpuSolver(topo, abi.make_params(1.0 / TAU, precision=64))
-    s.set_bc_kinds(np.zeros((topo.num_blocks, 8, 8, 8), np.uint8))
+    s = GpuSolver(topo, abi.make_params(1.0 / TAU, precision=precision))
+    s.set_bc_kinds(np.zeros((topo.num_blocks, topo.n, topo.n, topo.n), np.uint8))
     return s
+
+
+def _tasks(old, new):
+    """(mode, old blocks, octant) of every block of `new`: the same block,
+    the parent (octant of the child in it) or the eight children."""
+    out = []
+    for j in range(new.num_blocks):
+        bb, lv = new.bounds[j], new.level[j]
+        same = [i for i in range(old.num_blocks) if old.level[i] == lv and np.all(old.bounds[i] == bb)]
+        if same:
+            out.append((0, same, (0, 0, 0)))
+            continue
+        par = [i for i in range(old.num_blocks) if old.level[i] == lv - 1 and np.all(old.bounds[i][:3] <= bb[:3])
+               and np.all(old.bounds[i][3:] >= bb[3:])]
+        if par:
+            pb = old.bounds[par[0]]
+            oct = tuple(int(v) for v in np.rint((bb[:3] - pb[:3]) / (bb[3] - bb[0])))
+            out.append((1, par, oct))
+            continue
+        w = (bb[3] - bb[0]) / 2
+        kids = []
+        for c in range(8):
+            lo = bb[:3] + w * np.array([c & 1, (c >> 1) & 1, c >> 2])
+            kids += [i for i in range(old.num_blocks) if old.level[i] == lv + 1 and np.allclose(old.bounds[i][:3], lo)]
+        assert len(kids) == 8
+        out.append((2, kids, (0, 0, 0)))
+    return out
+
+
+@pytest.mark.parametrize("precision,n", [(64, 8), (32, 8)])
+def test_device_migration_matches_the_oracle(precision, n):
+    # every new block's core cells, refine and then coarsen, against the
+    # oracle's statement of the rules fed with the old handle's downloaded
+    # halo-inclusive views: bit for bit in both precisions
+    from oracle import binding
+    ta, tb = _topo(False, n), _topo(True, n)
+    src = _solver(ta, precision)
+    nc = ta.num_blocks * n ** 3
+    rng = np.random.default_rng(11)
+    src.init_field(1.0 + 0.01 * rng.random(nc), 0.02 * (rng.random(nc * 3) - 0.5))
+    src.advance(2)
+    dst = _solver(tb, precision)
+    back = _solver(ta, precision)
+    modes = set()
+    for old, new, a, b in ((ta, tb, src, dst), (tb, ta, dst, back)):
+        print(f"migrate precision {precision} n {n}: {old.num_blocks} -> {new.num_blocks} blocks", flush=True)
+        b.migrate_from(a)
+        got = b.download_core()
+        views = [a.download_block(i) for i in range(old.num_blocks)]
+        for j, (mode, srcs, oct) in enumerate(_tasks(old, new)):
+            modes.add(mode)
+            want = binding.migrate_block(mode, n, 1, oct, [views[i] for i in srcs], TAU)
+            want = want.reshape(n + 2, n + 2, n + 2, 19)[1:-1, 1:-1, 1:-1]
+            if precision == 32:
+                want = want.astype(np.float32).astype(np.float64)
+            np.testing.assert_array_equal(got[j], want, err_msg=f"block {j} mode {mode}")
+    assert modes == {0, 1, 2}
+    for s in (src, dst, back):
+        s.close()
 
 
 def _macro(f):
@@ -162,5 +221,18 @@ def test_migrate_rejects_ensembles():
     b = GpuSolver.ensemble(ens, 64)
     with pytest.raises(abi.OctlbError, match="ensemble"):
         b.migrate_from(a)
+    for s in (a, b):
+        s.close()
+
+
+def test_padded_block_copies_whole_view():
+    # the padded layout (one block, any edge) keeps every halo slot: a
+    # migration onto the same forest copies the whole view
+    case = cases.cavity(16, tau=TAU, u_lid=0.05)
+    a = case.apply(GpuSolver(case.topo, case.params(64)))
+    a.advance(3)
+    b = case.apply(GpuSolver(case.topo, case.params(64)))
+    b.migrate_from(a)
+    np.testing.assert_array_equal(b.download_block(0), a.download_block(0))
     for s in (a, b):
         s.close()
