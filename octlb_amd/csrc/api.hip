@@ -317,14 +317,17 @@ struct octlb_dev {
   template <typename T>
   void Couple(CoupleArgs<T> c, const int* list, int count, int mode, int next_level);
   bool fd_pending = false;   // blocked mode: FD pass of the last streamed level not launched yet
-  const int* fd_pend_idx = nullptr;  // its FD block indices (into d_fd_blocks) and count
+  const int4* fd_pend_idx = nullptr;  // its FD blocks as (FD block, block, level, 0) and count
   int fd_pend_cnt = 0;
+  int fd_pend_cls[kFdClasses] = {};  // its blocks per FD-cell class (listed class by class)
   int fd_pend_level = -1;            // level whose own FD blocks are pending, -1 = all levels
   // per level: FD blocks of that level / of the other levels, as index runs in d_fd_idx
   int* d_fd_idx = nullptr;
+  int4* d_fd_desc = nullptr;  // d_fd_idx's entries as (FD block, block, level, 0)
   int fd_same_off[kMaxLevels] = {}, fd_same_cnt[kMaxLevels] = {};
   int fd_other_off[kMaxLevels] = {}, fd_other_cnt[kMaxLevels] = {};
   int fd_all_off = 0;
+  int fd_same_cls[kMaxLevels][kFdClasses] = {}, fd_all_cls[kFdClasses] = {};
   bool fd_merge_ok = false;  // FD blocks and coupled blocks are disjoint
   template <typename T>
   void CoarseStep();
@@ -657,6 +660,7 @@ void octlb_dev::Prepare() {
   fd_pending = false;
   fd_pend_cnt = 0;
   d_fd_idx = nullptr;
+  d_fd_desc = nullptr;
   for (int lv = 0; lv < kMaxLevels; ++lv) fd_same_cnt[lv] = fd_other_cnt[lv] = 0;
   d_fd_blocks = nullptr;
   d_fd_slot = nullptr;
@@ -951,11 +955,20 @@ void octlb_dev::Prepare() {
       d_fd_count = PAlloc<int4>(counts.size());
       Upload(d_fd_count, counts, stream);
       {
+        // FD-cell class of each FD block: the pass's LDS is sized by it
+        auto cls = [&](int k) {
+          const int cells = counts[static_cast<size_t>(k)].z;
+          return cells <= kFdClassCells[0] ? 0 : (cells <= kFdClassCells[1] ? 1 : 2);
+        };
         std::vector<int> idx;
         for (int lv = 0; lv <= max_level; ++lv) {
           fd_same_off[lv] = static_cast<int>(idx.size());
-          for (int k = 0; k < static_cast<int>(fdb.size()); ++k)
-            if (level[fdb[k]] == lv) idx.push_back(k);
+          for (int c = 0; c < kFdClasses; ++c) {
+            const size_t before = idx.size();
+            for (int k = 0; k < static_cast<int>(fdb.size()); ++k)
+              if (level[fdb[k]] == lv && cls(k) == c) idx.push_back(k);
+            fd_same_cls[lv][c] = static_cast<int>(idx.size() - before);
+          }
           fd_same_cnt[lv] = static_cast<int>(idx.size()) - fd_same_off[lv];
           fd_other_off[lv] = static_cast<int>(idx.size());
           for (int k = 0; k < static_cast<int>(fdb.size()); ++k)
@@ -963,9 +976,18 @@ void octlb_dev::Prepare() {
           fd_other_cnt[lv] = static_cast<int>(idx.size()) - fd_other_off[lv];
         }
         fd_all_off = static_cast<int>(idx.size());
-        for (int k = 0; k < static_cast<int>(fdb.size()); ++k) idx.push_back(k);
+        for (int c = 0; c < kFdClasses; ++c) {
+          const size_t before = idx.size();
+          for (int k = 0; k < static_cast<int>(fdb.size()); ++k)
+            if (cls(k) == c) idx.push_back(k);
+          fd_all_cls[c] = static_cast<int>(idx.size() - before);
+        }
         d_fd_idx = PAlloc<int>(idx.size());
         Upload(d_fd_idx, idx, stream);
+        std::vector<int4> desc(idx.size());
+        for (size_t j = 0; j < idx.size(); ++j) desc[j] = make_int4(idx[j], fdb[idx[j]], level[fdb[idx[j]]], 0);
+        d_fd_desc = PAlloc<int4>(desc.size());
+        Upload(d_fd_desc, desc, stream);
       }
       // the FD pass may share a launch with a coupling step only if no FD
       // block is read or written by a coupling point
@@ -1334,8 +1356,9 @@ void octlb_dev::Level(int l) {
       // the level has none to update) follows the update: it is launched
       // with the coupling step that comes next, or alone
       const bool split = cnt > 0;
-      fd_pend_idx = d_fd_idx + (split ? fd_same_off[l] : fd_all_off);
+      fd_pend_idx = d_fd_desc + (split ? fd_same_off[l] : fd_all_off);
       fd_pend_cnt = split ? fd_same_cnt[l] : nfd_blocks;
+      for (int c = 0; c < kFdClasses; ++c) fd_pend_cls[c] = split ? fd_same_cls[l][c] : fd_all_cls[c];
       fd_pend_level = split ? l : -1;
       fd_pending = fd_pend_cnt > 0;
     }
@@ -1399,7 +1422,7 @@ void octlb_dev::FlushFd() {
   fd_pending = false;
   FdArgs<T> f = MakeFdArgs<T>();
   f.step_next = TakeInc(true);
-  LaunchFdBlocks<T>(f, d_fd_blocks, d_fd_slot, fd_pend_idx, fd_pend_cnt, stream);
+  LaunchFdBlocks<T>(f, d_fd_slot, fd_pend_idx, fd_pend_cls, stream);
 }
 
 // One coupling step (mode 1 = prolongation at half time, 0 = at full time,
@@ -1434,7 +1457,7 @@ void octlb_dev::Couple(CoupleArgs<T> c, const int* list, int count, int mode, in
     fd_pending = false;
     FdArgs<T> f = MakeFdArgs<T>();
     f.step_next = TakeInc(true);
-    LaunchFdCouple<T>(f, d_fd_blocks, d_fd_slot, fd_pend_idx, fd_pend_cnt, c, list, count, mode, stream);
+    LaunchFdCouple<T>(f, d_fd_slot, fd_pend_idx, fd_pend_cnt, c, list, count, mode, stream);
     return;
   }
   FlushFd<T>();

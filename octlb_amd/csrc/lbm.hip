@@ -227,11 +227,14 @@ __device__ inline void RestrictBodyIn(const CoupleArgs<T>& a, const int* list, i
                                       RestrictLds<T>* sm);
 
 // FD boundary pass, defined with the FD kernels below.
-template <typename T>
+constexpr int kMaxFdCells = 6 * 36 + 12 * 6 + 8;  // face, edge and corner cells of an 8^3 block
+template <typename T, int MaxFd = kMaxFdCells>
 struct FdSmem;
+template <typename T, int MaxFd = kMaxFdCells, bool kEarly = false>
+__device__ inline void FdBlockBody(const FdArgs<T>& a, const short* fd_slot, int4 d, FdSmem<T, MaxFd>& sm);
 template <typename T>
-__device__ inline void FdBlockBody(const FdArgs<T>& a, const int* fd_blocks, const short* fd_slot, int fb,
-                                   FdSmem<T>& sm);
+__device__ inline void FdBlockBodyAt(const FdArgs<T>& a, const int* fd_blocks, const short* fd_slot, int fb,
+                                     FdSmem<T>& sm);
 
 // FP32: two workgroups per CU (2 x 77 KB LDS) need <= 64 VGPRs (8 waves per
 // SIMD); FP64: one workgroup per CU (153 KB LDS), 4 waves per SIMD.
@@ -257,7 +260,7 @@ __global__ __launch_bounds__(1024, BlockOccupancy<T>::kWavesPerSimd) void KBlock
       const int fb = a.fd_idx[blockIdx.x];
       const int reps = (a.fd_rep_level >= 0 && a.fd.level[a.fd_blocks[fb]] == a.fd_rep_level) ? 2 : 1;
       for (int r = 0; r < reps; ++r)  // each pass ends on a barrier: the next one sees its stores
-        FdBlockBody<T>(a.fd, a.fd_blocks, a.fd_slot, fb, *reinterpret_cast<FdSmem<T>*>(&plane[0][0]));
+        FdBlockBodyAt<T>(a.fd, a.fd_blocks, a.fd_slot, fb, *reinterpret_cast<FdSmem<T>*>(&plane[0][0]));
     }
     return;
   }
@@ -1127,8 +1130,6 @@ __device__ inline T FdPopulation(const int* c, T t, T rho, const T* u, T usqr, c
   return feq + fneq;
 }
 
-constexpr int kMaxFdCells = 6 * 36 + 12 * 6 + 8;  // face, edge and corner cells of an 8^3 block
-
 // Blocked mode: one workgroup per boundary block.  Every cell's density and
 // velocity as the FD stencils see them (BoundaryLatticeView::ComputeRho /
 // ComputeU, interpolated_velocity.h:352-434) are computed once into LDS and
@@ -1138,29 +1139,34 @@ constexpr int kMaxFdCells = 6 * 36 + 12 * 6 + 8;  // face, edge and corner cells
 // and corners extrapolate densities that earlier phases rewrote, as the
 // reference's in-place (ix,iy,iz) sweep does), and the 19 populations of every
 // cell of the phase are written by all 512 threads as (cell, population) tasks.
-// LDS of one FD block pass; carved from the caller's shared buffer so the
-// pass can also run inside the fused kernel's workgroups (aliasing its box).
-template <typename T>
+// LDS of one FD block pass over at most MaxFd FD cells; carved from the
+// caller's shared buffer so the pass can also run inside the fused kernel's
+// workgroups (aliasing its box).
+template <typename T, int MaxFd>
 struct FdSmem {
   T rho_pre[kTileCells], rho_post[kTileCells];
   T ux[kTileCells], uy[kTileCells], uz[kTileCells];
-  T crho[kMaxFdCells], cusq[kMaxFdCells];
-  T cu[kMaxFdCells][3], cpi[kMaxFdCells][6];
-  T fbuf[kMaxFdCells][kQ];
+  T crho[MaxFd], cusq[MaxFd];
+  T cu[MaxFd][3], cpi[MaxFd][6];
+  T fbuf[MaxFd][kQ];
   T wtab[kQ];
   int ctab[kQ][3];
   short slot[kTileCells];
-  short lst[kMaxFdCells];
+  short lst[MaxFd];
   uint8_t kind[kTileCells];
 };
 
 // One FD block pass by a workgroup of 512 or more threads (threads beyond
 // the block's 512 cells take part in the barriers and the task loop only).
-// The pass over block fd_blocks[fb] whose populations are
+// The pass over FD block fb (block b) whose populations are
 // cur[i * plane + cell] (plane = 512 in the production layout), at time t.
-template <typename T>
-__device__ inline void FdBlockCore(const FdArgs<T>& a, const int* fd_blocks, const short* fd_slot, int fb, T* cur,
-                                   int plane, double t, FdSmem<T>& sm) {
+// kEarly: every global input is loaded before the first barrier (the FD
+// cell list and phase ends with the populations); off where the pass runs
+// inside the fused kernel, whose register budget the longer live ranges
+// would exceed.
+template <typename T, int MaxFd = kMaxFdCells, bool kEarly = false>
+__device__ inline void FdBlockCore(const FdArgs<T>& a, const short* fd_slot, int fb, int b, T* cur, int plane,
+                                   double t, FdSmem<T, MaxFd>& sm) {
   T* rho_pre = sm.rho_pre;
   T* rho_post = sm.rho_post;
   T* ux = sm.ux;
@@ -1185,9 +1191,16 @@ __device__ inline void FdBlockCore(const FdArgs<T>& a, const int* fd_blocks, con
     ctab[i][2] = kCzT[i];
     wtab[i] = T(1) / T(kWDenT[i]);
   }
-  const int b = fd_blocks[fb];
   const int tid = threadIdx.x;
   const SimConst& sc = *(const SimConst*)&a.scm[a.bt[b].member];
+  // the FD cells in plane|edge|corner order and the phase ends, issued with
+  // the population loads; the list reaches LDS after them
+  int4 cnt = make_int4(0, 0, 0, 0);
+  short lraw = -1;
+  if constexpr (kEarly) {
+    cnt = a.fd_count[fb];
+    if (tid < MaxFd) lraw = a.fd_list[static_cast<size_t>(fb) * kTileCells + tid];
+  }
   if (act) {
     const int ix = tid >> 6, iy = (tid >> 3) & 7, iz = tid & 7;
     const int k = a.kind[static_cast<size_t>(b) * kTileCells + tid];
@@ -1218,6 +1231,9 @@ __device__ inline void FdBlockCore(const FdArgs<T>& a, const int* fd_blocks, con
       uy[tid] = u[1];
       uz[tid] = u[2];
     }
+  }
+  if constexpr (kEarly) {
+    if (tid < MaxFd) lst[tid] = lraw;
   }
   __syncthreads();
   {
@@ -1258,7 +1274,7 @@ __device__ inline void FdBlockCore(const FdArgs<T>& a, const int* fd_blocks, con
   __syncthreads();
 
   // ---- gradients of the FD cell this thread owns
-  const int4 cnt = a.fd_count[fb];
+  if constexpr (!kEarly) cnt = a.fd_count[fb];
   const int ph_end[3] = {cnt.x, cnt.y, cnt.z};
   // plane, edge and corner cells start on separate waves, so the three
   // stencil shapes run side by side instead of one after another in a wave
@@ -1276,8 +1292,12 @@ __device__ inline void FdBlockCore(const FdArgs<T>& a, const int* fd_blocks, con
   T myu[3] = {T(0), T(0), T(0)};
   int fc = 0, xn = 0, yn = 0, zn = 0, ix = 0, iy = 0, iz = 0;
   if (e >= 0) {
-    my = a.fd_list[static_cast<size_t>(fb) * kTileCells + e];
-    lst[e] = static_cast<short>(my);
+    if constexpr (kEarly) {
+      my = lst[e];
+    } else {
+      my = a.fd_list[static_cast<size_t>(fb) * kTileCells + e];
+      lst[e] = static_cast<short>(my);
+    }
     myph = e < cnt.x ? 0 : (e < cnt.y ? 1 : 2);
     ix = my >> 6;
     iy = (my >> 3) & 7;
@@ -1422,20 +1442,40 @@ __device__ inline void FdBlockCore(const FdArgs<T>& a, const int* fd_blocks, con
   }
 }
 
+// The pass over FD block fb as the fused kernel runs it (its register
+// budget is tight: the block and its level are loaded here, in this order).
 template <typename T>
-__device__ inline void FdBlockBody(const FdArgs<T>& a, const int* fd_blocks, const short* fd_slot, int fb,
-                                   FdSmem<T>& sm) {
+__device__ inline void FdBlockBodyAt(const FdArgs<T>& a, const int* fd_blocks, const short* fd_slot, int fb,
+                                     FdSmem<T>& sm) {
   const int b = fd_blocks[fb];
   T* cur = (a.par.v[a.level[b]] ? a.pops1 : a.pops0) + static_cast<size_t>(b) * kQ * kTileCells;
-  FdBlockCore<T>(a, fd_blocks, fd_slot, fb, cur, kTileCells, static_cast<double>(*a.step), sm);
+  FdBlockCore<T>(a, fd_slot, fb, b, cur, kTileCells, static_cast<double>(*a.step), sm);
 }
 
-template <typename T>
-__global__ __launch_bounds__(512) void KFdBlock(FdArgs<T> a, const int* fd_blocks, const short* fd_slot,
-                                                const int* fd_idx) {
-  __shared__ FdSmem<T> sm;
+// d = (FD block, block, its level, 0): one load gives everything the
+// population loads depend on.
+template <typename T, int MaxFd, bool kEarly>
+__device__ inline void FdBlockBody(const FdArgs<T>& a, const short* fd_slot, int4 d, FdSmem<T, MaxFd>& sm) {
+  T* cur = (a.par.v[d.z] ? a.pops1 : a.pops0) + static_cast<size_t>(d.y) * kQ * kTileCells;
+  FdBlockCore<T, MaxFd, kEarly>(a, fd_slot, d.x, d.y, cur, kTileCells, static_cast<double>(*a.step), sm);
+}
+
+// MaxFd: the most FD cells any block of the launch holds (the LDS the pass
+// needs, and so how many workgroups share a CU).
+// Small classes: FP32 at 64 VGPRs (8 waves per SIMD, four 512-thread
+// workgroups per CU, 20-28 KB LDS each); FP64 at 4 waves per SIMD, the most
+// it reaches without spilling (two workgroups per CU).
+template <typename T, int MaxFd>
+struct FdOccupancy {
+  static constexpr int kWavesPerSimd = MaxFd == kMaxFdCells ? 1 : (sizeof(T) == 4 ? 8 : 4);
+};
+
+template <typename T, int MaxFd>
+__global__ __launch_bounds__(512, (FdOccupancy<T, MaxFd>::kWavesPerSimd)) void KFdBlock(
+    FdArgs<T> a, const short* fd_slot, const int4* fd_idx) {
+  __shared__ FdSmem<T, MaxFd> sm;
   BumpStep(a.step, a.step_next);
-  FdBlockBody<T>(a, fd_blocks, fd_slot, fd_idx[blockIdx.x], sm);
+  FdBlockBody<T, MaxFd, true>(a, fd_slot, fd_idx[blockIdx.x], sm);
 }
 
 template <typename T>
@@ -1602,14 +1642,14 @@ __global__ __launch_bounds__(256) void KRestrict(CoupleArgs<T> a, const int* lis
 constexpr int kMergedGroups = 16;  // 512 threads / 32-lane restriction groups
 
 template <typename T>
-__global__ __launch_bounds__(512) void KFdCouple(FdArgs<T> fa, const int* fd_blocks, const short* fd_slot,
-                                                 const int* fd_idx, int nfd, CoupleArgs<T> ca, const int* list,
+__global__ __launch_bounds__(512) void KFdCouple(FdArgs<T> fa, const short* fd_slot, const int4* fd_idx, int nfd,
+                                                 CoupleArgs<T> ca, const int* list,
                                                  int count, int mode) {
   __shared__ FdSmem<T> sm;
   BumpStep(fa.step, fa.step_next);
   const int w = static_cast<int>(blockIdx.x);
   if (w < nfd) {
-    FdBlockBody<T>(fa, fd_blocks, fd_slot, fd_idx[w], sm);
+    FdBlockBody<T, kMaxFdCells, true>(fa, fd_slot, fd_idx[w], sm);
   } else if (mode == 2) {
     RestrictBody<T, kMergedGroups>(ca, list, count, w - nfd);
   } else {
@@ -1804,10 +1844,19 @@ void LaunchInitField(T* pops, int nb, const SimConst& sc, const double* rho, con
 }
 
 template <typename T>
-void LaunchFdBlocks(const FdArgs<T>& a, const int* fd_blocks, const short* fd_slot, const int* fd_idx, int nfd,
+void LaunchFdBlocks(const FdArgs<T>& a, const short* fd_slot, const int4* fd_idx, const int ncls[kFdClasses],
                     hipStream_t s) {
-  if (nfd <= 0) return;
-  hipLaunchKernelGGL(KFdBlock<T>, dim3(nfd), dim3(kTileCells), 0, s, a, fd_blocks, fd_slot, fd_idx);
+  const int4* idx = fd_idx;
+  // one launch per FD-cell class (fd_idx lists the blocks of class 0 first):
+  // the small classes fit more workgroups per CU
+  if (ncls[0] > 0)
+    hipLaunchKernelGGL((KFdBlock<T, kFdClassCells[0]>), dim3(ncls[0]), dim3(kTileCells), 0, s, a, fd_slot, idx);
+  idx += ncls[0];
+  if (ncls[1] > 0)
+    hipLaunchKernelGGL((KFdBlock<T, kFdClassCells[1]>), dim3(ncls[1]), dim3(kTileCells), 0, s, a, fd_slot, idx);
+  idx += ncls[1];
+  if (ncls[2] > 0)
+    hipLaunchKernelGGL((KFdBlock<T, kMaxFdCells>), dim3(ncls[2]), dim3(kTileCells), 0, s, a, fd_slot, idx);
 }
 
 template <typename T>
@@ -1831,13 +1880,12 @@ void LaunchProlong(const CoupleArgs<T>& a, const int* list, int count, int half,
 }
 
 template <typename T>
-void LaunchFdCouple(const FdArgs<T>& fa, const int* fd_blocks, const short* fd_slot, const int* fd_idx, int nfd,
-                    const CoupleArgs<T>& ca, const int* list, int count, int mode, hipStream_t s) {
+void LaunchFdCouple(const FdArgs<T>& fa, const short* fd_slot, const int4* fd_idx, int nfd, const CoupleArgs<T>& ca,
+                    const int* list, int count, int mode, hipStream_t s) {
   const int per = mode == 2 ? kMergedGroups : 512;
   const int ncw = (count + per - 1) / per;
   if (nfd + ncw <= 0) return;
-  hipLaunchKernelGGL(KFdCouple<T>, dim3(nfd + ncw), dim3(512), 0, s, fa, fd_blocks, fd_slot, fd_idx, nfd, ca, list,
-                     count, mode);
+  hipLaunchKernelGGL(KFdCouple<T>, dim3(nfd + ncw), dim3(512), 0, s, fa, fd_slot, fd_idx, nfd, ca, list, count, mode);
 }
 
 template <typename T>
@@ -1897,12 +1945,12 @@ void LaunchMacro(const T* p0, const T* p1, const ParArray& par, const BlockTab* 
   template void LaunchInitField<T>(T*, int, const SimConst&, const double*, const double*,        \
                                    hipStream_t);                                                  \
   template void LaunchFd<T>(const FdArgs<T>&, const int*, hipStream_t);                           \
-  template void LaunchFdBlocks<T>(const FdArgs<T>&, const int*, const short*, const int*, int, hipStream_t); \
+  template void LaunchFdBlocks<T>(const FdArgs<T>&, const short*, const int4*, const int*, hipStream_t);  \
   template void LaunchCouplerInit<T>(const CoupleArgs<T>&, int, hipStream_t);                     \
   template void LaunchProlong<T>(const CoupleArgs<T>&, const int*, int, int, hipStream_t);        \
   template void LaunchRestrict<T>(const CoupleArgs<T>&, const int*, int, hipStream_t);            \
-  template void LaunchFdCouple<T>(const FdArgs<T>&, const int*, const short*, const int*, int,     \
-                                  const CoupleArgs<T>&, const int*, int, int, hipStream_t);         \
+  template void LaunchFdCouple<T>(const FdArgs<T>&, const short*, const int4*, int, const CoupleArgs<T>&, \
+                                  const int*, int, int, hipStream_t);                               \
   template void LaunchDrag<T>(const DragArgs<T>&, int, const int*, int, double*, double*,         \
                               hipStream_t);                                                       \
   template void LaunchMass<T>(const T*, const T*, const ParArray&, const BlockTab*,               \

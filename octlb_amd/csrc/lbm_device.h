@@ -483,14 +483,21 @@ void LaunchInitField(T* pops, int nb, const SimConst& sc, const double* rho, con
                      hipStream_t s);
 template <typename T>
 void LaunchFd(const FdArgs<T>& a, const int* phase_begin, hipStream_t s);
+// FD passes by FD-cell count: class c holds the blocks with at most
+// kFdClassCells[c] FD cells (c < 2) and the rest (c = 2); ncls[c] blocks
+// each, listed class by class in fd_idx, one launch per class with the LDS
+// it needs (more workgroups share a CU).
+constexpr int kFdClasses = 3;
+constexpr int kFdClassCells[2] = {64, 128};
+// fd_idx entries: (FD block, block, its level, 0).
 template <typename T>
-void LaunchFdBlocks(const FdArgs<T>& a, const int* fd_blocks, const short* fd_slot, const int* fd_idx, int nfd,
+void LaunchFdBlocks(const FdArgs<T>& a, const short* fd_slot, const int4* fd_idx, const int ncls[kFdClasses],
                     hipStream_t s);
 // FD pass + the coupling step after it in one launch; mode 0/1 = prolongation
 // (full / half time), 2 = restriction.
 template <typename T>
-void LaunchFdCouple(const FdArgs<T>& fa, const int* fd_blocks, const short* fd_slot, const int* fd_idx, int nfd,
-                    const CoupleArgs<T>& ca, const int* list, int count, int mode, hipStream_t s);
+void LaunchFdCouple(const FdArgs<T>& fa, const short* fd_slot, const int4* fd_idx, int nfd, const CoupleArgs<T>& ca,
+                    const int* list, int count, int mode, hipStream_t s);
 template <typename T>
 void LaunchCouplerInit(const CoupleArgs<T>& a, int npts, hipStream_t s);
 template <typename T>
