@@ -419,19 +419,36 @@ def run_migrate(precision=32, mark_frac=0.1, seed=3, reps=3):
 
 
 def run_link_build(with_cpu, segments=512, rings=8, reps=3, cpu_blocks=24):
-    """Bouzidi link build from a triangle soup (BouzidiLinkData::Build) on
-    the headline forest: z-spanning cylinder of D = 16 fine cells as
-    2 * segments * rings triangles, material from the analytic inside test.
-    GPU: wall time of octlb_build_links (uploads, kernels, downloads).  CPU:
-    the oracle restatement (the reference's loop: every fluid cell against
-    every triangle), timed on `cpu_blocks` blocks and scaled per fluid cell."""
+    """The geometry pipeline on the headline forest: a z-spanning cylinder of
+    D = 16 fine cells as 2 * segments * rings triangles, voxelized into
+    MaterialKind per cell (voxelize_part, octlb_voxelize_part) and turned
+    into Bouzidi links (BouzidiLinkData::Build, octlb_build_links).  GPU:
+    wall time of each C call (uploads, kernels, downloads).  CPU: the oracle
+    restatements -- the voxelizer on all host threads over the whole
+    forest, the link build (the reference's loop: every fluid cell against
+    every triangle) on `cpu_blocks` blocks, scaled per fluid cell."""
     from octlb_amd import cases, geometry
 
     case = cases.cylinder_channel(bricks_x=16, bricks_yz=4, base_level=0, fine=True, xc=1.0,
                                   radius=0.125, u_in=0.04, tau=0.55)
     topo = case.topo
-    mat = geometry.material_field(topo, lambda x, y, z: (x - 1.0) ** 2 + (y - 0.5) ** 2 < 0.125 ** 2)
     tris = geometry.cylinder_soup(1.0, 0.5, 0.125, 0.0, 1.0, segments, rings)
+    mat, nexact = geometry.voxelize_part(topo.bounds, 8, tris, geometry.PART_EXTERNAL_OBSTACLE)  # warm-up
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        mat, nexact = geometry.voxelize_part(topo.bounds, 8, tris, geometry.PART_EXTERNAL_OBSTACLE)
+    vox_s = (time.perf_counter() - t0) / reps
+    vox = {"gpu_ms": round(vox_s * 1e3, 3), "cells": int(mat.size),
+           "fluid_solid_boundary": [int((mat == k).sum()) for k in range(3)],
+           "exact_fallback_cells": nexact}
+    if with_cpu:
+        from oracle.binding import voxelize_part as oracle_voxelize
+
+        t0 = time.perf_counter()
+        omat = oracle_voxelize(topo.bounds, 8, tris, geometry.PART_EXTERNAL_OBSTACLE)
+        vox["cpu_baseline"] = {"value_s": round(time.perf_counter() - t0, 3), "cores": cpu_threads(), "kind": "port",
+                               "sample": "oracle restatement (exact arithmetic), whole forest, OpenMP over blocks"}
+        vox["equal_to_oracle"] = bool(np.array_equal(omat, mat))
     link, q = geometry.build_links(topo.bounds, 8, mat, tris)  # warm-up (code object, allocator)
     t0 = time.perf_counter()
     for _ in range(reps):
@@ -448,11 +465,13 @@ def run_link_build(with_cpu, segments=512, rings=8, reps=3, cpu_blocks=24):
     gflops = ncand * tris.shape[0] * FLOPS_PER_CELL_TRIANGLE / gpu_s / 1e9
     out = {
         "workload": ("cylinder-channel forest (1,152 blocks of 8^3), z-spanning cylinder D=16 fine cells "
-                     f"as {tris.shape[0]} triangles ({segments} segments x {rings} rings)"),
+                     f"as {tris.shape[0]} triangles ({segments} segments x {rings} rings); material voxelized "
+                     "from the soup on the device, links from that material"),
         "triangles": int(tris.shape[0]),
         "candidate_cells": ncand,
         "links": int(len(link)),
         "gpu_ms": round(gpu_s * 1e3, 3),
+        "voxelize": vox,
         "roofline": {"bound": "fp64 valu", "achieved": round(gflops, 1), "peak": FP64_VALU_PEAK_GFLOPS,
                      "unit": "GFLOP/s", "frac": round(gflops / FP64_VALU_PEAK_GFLOPS, 4),
                      "flops_per_cell_triangle": FLOPS_PER_CELL_TRIANGLE},
@@ -584,7 +603,9 @@ def main():
     if not args.no_links and args.workload == "cylinder":
         log("bouzidi link build")
         links = run_link_build(not args.no_cpu)
-        log(f"  {links['links']} links from {links['triangles']} triangles in {links['gpu_ms']} ms")
+        log(f"  voxelized {links['voxelize']['cells']} cells in {links['voxelize']['gpu_ms']} ms "
+            f"({links['voxelize']['exact_fallback_cells']} by the exact fallback); "
+            f"{links['links']} links from {links['triangles']} triangles in {links['gpu_ms']} ms")
     mig = None
     if not args.no_migrate and args.workload == "cylinder":
         log("state migration after a forest change")

@@ -295,6 +295,31 @@ int octlb_dev_create_ensemble(const octlb_topology* topo, int32_t num_members, c
 /* ConstRho average of one member (TimeLoop::average_rho of that case). */
 int octlb_dev_member_average_rho(octlb_dev* dev, int32_t member, double* out);
 
+/* ---- Material field of a geometry part --------------------------------
+ * Replaces voxelize_part(forest, part, config) (part_voxelizer.cpp:71-114)
+ * with its checks (validate_geometry_part, :51-69), run on the current HIP
+ * device.  The merge of several parts (merge_material_field, :116-146, in
+ * priority order, geometry_engine.cpp:28-39) is byte work left to the
+ * caller.
+ *   bounds, n   as for octlb_build_links (cell_width = n)
+ *   triangles   the part's soup, 9 doubles each
+ *   inner_*     the inner cavity soup (role INTERNAL_CHANNEL only)
+ *   role        OCTLB_PART_INTERNAL_CHANNEL / _EXTERNAL_OBSTACLE
+ *               (GeometryPartRole values)
+ * Output: material, n^3 MaterialKind bytes per block in MaterialField order
+ * (k*n + j)*n + i; *num_exact (optional) = cells whose tests needed the
+ * exact-arithmetic fallback.  Inside tests are ray parity along +x, +y, +z
+ * (majority of three) and cell-surface tests closed triangle-box
+ * intersection, both decided exactly on the double inputs as CGAL's exact
+ * predicates decide them.  OCTLB_EINVAL for an empty soup, a channel
+ * without inner soup, or a channel whose outer mesh leaves less than 5 % of
+ * 5^3 samples outside. */
+#define OCTLB_PART_INTERNAL_CHANNEL 0
+#define OCTLB_PART_EXTERNAL_OBSTACLE 1
+int octlb_voxelize_part(int32_t num_blocks, int32_t n, const double* bounds, int64_t num_triangles,
+                        const double* triangles, int64_t num_inner_triangles, const double* inner_triangles,
+                        int32_t role, uint8_t* material, int64_t* num_exact);
+
 /* ---- Bouzidi wall fractions from a triangle soup ----------------------
  * Replaces BouzidiLinkData::Build(forest, material, assembly)
  * (bouzidi_link_data.cpp:139-233), run on the current HIP device.

@@ -195,6 +195,8 @@ _SIGNATURES = {
     "octlb_build_links_noalloc": ([C.c_int32, C.c_int32, _VP, _VP, C.c_int64, _VP, C.c_int64,
                                    C.POINTER(C.c_int64), _VP, _VP, _VP], C.c_int),
     "octlb_dev_migrate": ([_VP, _VP], C.c_int),
+    "octlb_voxelize_part": ([C.c_int32, C.c_int32, _F64P, C.c_int64, _F64P, C.c_int64, _F64P, C.c_int32, _U8P,
+                             C.POINTER(C.c_int64)], C.c_int),
     "octlb_dev_macro_fields": ([_VP, _F64P, _F64P], C.c_int),
     "octlb_dev_write_vtk": ([_VP, C.c_int32, C.c_char_p, C.c_char_p], C.c_int),
     "octlb_vtk_write_vts": ([C.c_char_p, C.c_int32, C.c_int32, C.c_int32, _F64P, C.c_int32,

@@ -20,7 +20,8 @@ CSRC = PKG / "csrc"
 OUT = PKG / "liboctlb_gpu.so"
 ARCH = os.environ.get("OCTLB_OFFLOAD_ARCH", "gfx950")
 
-SOURCES = ["lbm.hip", "api.hip", "links.hip", "adjoint.hip", "migrate.hip", "forest.cpp", "vtk_out.cpp"]
+SOURCES = ["lbm.hip", "api.hip", "links.hip", "adjoint.hip", "migrate.hip", "voxelize.hip", "forest.cpp",
+           "vtk_out.cpp"]
 COMMON = ["-O3", "-std=c++17", "-fPIC", "-ffp-contract=off", "-fno-fast-math",
           f"-I{ROOT / 'include'}", f"-I{CSRC}"]
 HIP_ONLY = [f"--offload-arch={ARCH}", "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]

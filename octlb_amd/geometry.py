@@ -92,6 +92,97 @@ def material_field(topo: abi.TopologyArrays, inside) -> np.ndarray:
     return mat
 
 
+PART_INTERNAL_CHANNEL, PART_EXTERNAL_OBSTACLE = 0, 1  # GeometryPartRole
+
+
+def reference_box_soup(lo, hi, inward=False) -> np.ndarray:
+    """The reference's test box (AddAxisAlignedBox, tests/unit/mesh/
+    geometry_fixtures.h:39-70): its twelve triangles in its order and
+    orientation, as they are -- they do not close the box (one x-max
+    triangle appears twice in both orientations, one triangle cuts through
+    the inside, half of the x-max and z-max faces is missing), and the
+    reference's ray-parity inside test classifies against exactly this soup."""
+    x0, y0, z0 = lo
+    x1, y1, z1 = hi
+    p = {"000": (x0, y0, z0), "100": (x1, y0, z0), "010": (x0, y1, z0), "110": (x1, y1, z0),
+         "001": (x0, y0, z1), "101": (x1, y0, z1), "011": (x0, y1, z1), "111": (x1, y1, z1)}
+    order = [("000", "100", "010"), ("100", "110", "010"), ("100", "110", "101"), ("100", "101", "110"),
+             ("110", "011", "101"), ("110", "011", "111"), ("010", "110", "011"), ("010", "011", "001"),
+             ("001", "011", "101"), ("001", "101", "100"), ("000", "010", "001"), ("000", "001", "100")]
+    tris = [p[a] + (p[c] + p[b] if inward else p[b] + p[c]) for a, b, c in order]
+    return np.array(tris, np.float64)
+
+
+def reference_sphere_soup(cx, cy, cz, r, segments) -> np.ndarray:
+    """CenteredSphere (tests/unit/mesh/geometry_fixtures.h:88-116): the
+    same angles and operation order, so the same vertex doubles."""
+    import math
+    tris = []
+    for lat in range(segments):
+        th0 = float(lat) * math.pi / segments
+        th1 = float(lat + 1) * math.pi / segments
+        for lon in range(segments):
+            ph0 = float(lon) * 2.0 * math.pi / segments
+            ph1 = float(lon + 1) * 2.0 * math.pi / segments
+
+            def sph(t, f):
+                st = math.sin(t)
+                return (cx + r * st * math.cos(f), cy + r * st * math.sin(f), cz + r * math.cos(t))
+
+            p00, p01, p10, p11 = sph(th0, ph0), sph(th0, ph1), sph(th1, ph0), sph(th1, ph1)
+            tris.append(p00 + p10 + p01)
+            tris.append(p01 + p10 + p11)
+    return np.array(tris, np.float64)
+
+
+def voxelize_part(bounds, n: int, soup, role: int, inner=None):
+    """Device voxelize_part (part_voxelizer.cpp:71-114): MaterialKind bytes
+    [block][k][j][i] of one part; returns (material, cells that needed the
+    exact-arithmetic fallback)."""
+    h = abi.lib()
+    bounds = np.ascontiguousarray(bounds, np.float64).reshape(-1, 6)
+    tri = np.ascontiguousarray(soup, np.float64).reshape(-1, 9)
+    inn = np.ascontiguousarray(inner if inner is not None else np.zeros((0, 9)), np.float64).reshape(-1, 9)
+    nb = bounds.shape[0]
+    out = np.zeros((nb, n, n, n), np.uint8)
+    nexact = C.c_int64(0)
+    abi.check(h.octlb_voxelize_part(nb, n, abi.f64p(bounds), tri.shape[0], abi.f64p(tri) if tri.size else None,
+                                    inn.shape[0], abi.f64p(inn) if inn.size else None, int(role),
+                                    abi.u8p(out.reshape(-1)), C.byref(nexact)), "octlb_voxelize_part")
+    return out, int(nexact.value)
+
+
+def merge_material(base, overlay, role: int) -> None:
+    """merge_material_field (part_voxelizer.cpp:116-146), in place: an
+    internal channel overwrites every cell, an external obstacle only the
+    cells it classifies solid or boundary."""
+    if base.shape != overlay.shape:
+        raise ValueError("MaterialField merge size mismatch")
+    if role == PART_EXTERNAL_OBSTACLE:
+        claim = overlay != MAT_FLUID
+        base[claim] = overlay[claim]
+    else:
+        base[...] = overlay
+
+
+def build_material(bounds, n: int, parts, voxelize=None):
+    """GeometryEngine::build's voxelize-and-merge (geometry_engine.cpp:28-39,
+    without the forest refinement): parts are dicts with soup, role,
+    priority and (channels) inner, merged in ascending priority, equal
+    priorities in the given order.  voxelize(bounds, n, soup, role, inner)
+    -> material defaults to the device voxelizer."""
+    if not parts:
+        raise ValueError("GeometryAssembly has no parts")
+    if voxelize is None:
+        voxelize = lambda b, m, s, r, i: voxelize_part(b, m, s, r, i)[0]  # noqa: E731
+    nb = np.asarray(bounds).reshape(-1, 6).shape[0]
+    merged = np.zeros((nb, n, n, n), np.uint8)
+    for part in sorted(parts, key=lambda q: q.get("priority", 0)):
+        field = voxelize(bounds, n, part["soup"], part["role"], part.get("inner"))
+        merge_material(merged, field, part["role"])
+    return merged
+
+
 def build_links(bounds, n: int, material, triangles):
     """Device link build; returns (links int32 [L, 3] = block, cell
     (ix*n+iy)*n+iz, population; q float64 [L])."""

@@ -65,6 +65,8 @@ def lib():
                                C.c_int64),
             "ol_migrate_block": ([C.c_int32, C.c_int32, C.c_int32, i32p, C.POINTER(f64p), C.c_double, f64p],
                                  C.c_int),
+            "ol_voxelize_part": ([C.c_int32, C.c_int32, f64p, C.c_int64, f64p, C.c_int64, f64p, C.c_int32, u8p],
+                                 C.c_int),
             "ol_adjoint": ([vp, C.c_int32, C.c_int32, C.c_int32, f64p, f64p, C.c_double, f64p, C.c_uint32,
                             C.c_int64, i32p, f64p, f64p, f64p], C.c_int),
         }
@@ -244,6 +246,22 @@ def migrate_block(mode, n, h, oct, views, tau):
     out = np.zeros((E ** 3, Q))
     _ok(lib().ol_migrate_block(int(mode), int(n), int(h), oc.ctypes.data_as(C.POINTER(C.c_int32)), arr,
                                float(tau), out.ctypes.data_as(f64p)), "migrate_block")
+    return out
+
+
+def voxelize_part(bounds, n, soup, role, inner=None):
+    """ol_voxelize_part: MaterialKind bytes [block][k][j][i] of one part
+    (role 0 internal channel, 1 external obstacle)."""
+    bounds = np.ascontiguousarray(bounds, np.float64).reshape(-1, 6)
+    tri = np.ascontiguousarray(soup, np.float64).reshape(-1, 9)
+    inn = np.ascontiguousarray(inner if inner is not None else np.zeros((0, 9)), np.float64).reshape(-1, 9)
+    nb = bounds.shape[0]
+    out = np.zeros((nb, n, n, n), np.uint8)
+    f64p, u8p = C.POINTER(C.c_double), C.POINTER(C.c_uint8)
+    _ok(lib().ol_voxelize_part(nb, n, bounds.ctypes.data_as(f64p), tri.shape[0],
+                               tri.ctypes.data_as(f64p) if tri.size else None, inn.shape[0],
+                               inn.ctypes.data_as(f64p) if inn.size else None, int(role),
+                               out.ctypes.data_as(u8p)), "voxelize_part")
     return out
 
 

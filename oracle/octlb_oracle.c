@@ -2018,3 +2018,293 @@ int ol_migrate_block(int32_t mode, int32_t n, int32_t h, const int32_t oct[3], c
 #undef VSLOT
   return 0;
 }
+
+/* ---------------- voxelizer (part_voxelizer.cpp:71-114) --------------------
+ * The reference decides its geometric tests with CGAL's exact predicates.
+ * Here every test is a sign of a small determinant of the input doubles,
+ * always formed exactly as a nonoverlapping expansion (Shewchuk, "Adaptive
+ * Precision Floating-Point Arithmetic and Fast Robust Geometric
+ * Predicates", 1997: Two-Sum, Two-Product with fma, Expansion-Sum,
+ * Scale-Expansion); the sign is that of the largest nonzero component.    */
+static void vx_two_sum(double a, double b, double* x, double* y) {
+  *x = a + b;
+  const double bv = *x - a, av = *x - bv;
+  *y = (a - av) + (b - bv);
+}
+static void vx_fast_two_sum(double a, double b, double* x, double* y) {
+  *x = a + b;
+  *y = b - (*x - a);
+}
+/* h = e + f, h distinct from e and f; returns n + m */
+static int vx_sum(const double* e, int n, const double* f, int m, double* h) {
+  for (int i = 0; i < n; ++i) h[i] = e[i];
+  int len = n;
+  for (int j = 0; j < m; ++j) {
+    double q = f[j];
+    for (int i = j; i < len; ++i) {
+      double s, t;
+      vx_two_sum(q, h[i], &s, &t);
+      h[i] = t;
+      q = s;
+    }
+    h[len++] = q;
+  }
+  return len;
+}
+/* h = e * b; returns 2 n */
+static int vx_scale(const double* e, int n, double b, double* h) {
+  double q = e[0] * b;
+  h[0] = fma(e[0], b, -q);
+  for (int i = 1; i < n; ++i) {
+    const double p1 = e[i] * b, p0 = fma(e[i], b, -p1);
+    double s;
+    vx_two_sum(q, p0, &s, &h[2 * i - 1]);
+    vx_fast_two_sum(p1, s, &q, &h[2 * i]);
+  }
+  h[2 * n - 1] = q;
+  return 2 * n;
+}
+/* h = e * f; returns 2 n m (h needs that many slots, tmp 4 n m) */
+static int vx_mul(const double* e, int n, const double* f, int m, double* h, double* tmp) {
+  int len = vx_scale(e, n, f[0], h);
+  for (int j = 1; j < m; ++j) {
+    double* s = tmp;
+    double* r = tmp + 2 * n;
+    const int ls = vx_scale(e, n, f[j], s);
+    len = vx_sum(h, len, s, ls, r);
+    memcpy(h, r, sizeof(double) * (size_t)len);
+  }
+  return len;
+}
+static int vx_sign(const double* e, int n) {
+  for (int i = n - 1; i >= 0; --i)
+    if (e[i] != 0.0) return e[i] > 0.0 ? 1 : -1;
+  return 0;
+}
+static void vx_diff(double a, double b, double* e) { /* e = a - b exactly, (low, high) */
+  const double x = a - b, bv = a - x, av = x + bv;
+  e[1] = x;
+  e[0] = (a - av) + (bv - b);
+}
+/* sign of (p1u - q1u)(p2v - q2v) - (p1v - q1v)(p2u - q2u) */
+static int vx_det2(double p1u, double p1v, double q1u, double q1v, double p2u, double p2v, double q2u,
+                   double q2v) {
+  double a[2], b[2], c[2], d[2], ab[8], cd[8], s[16], tmp[16];
+  vx_diff(p1u, q1u, a);
+  vx_diff(p2v, q2v, b);
+  vx_diff(p1v, q1v, c);
+  vx_diff(p2u, q2u, d);
+  vx_mul(a, 2, b, 2, ab, tmp);
+  vx_mul(c, 2, d, 2, cd, tmp);
+  for (int i = 0; i < 8; ++i) cd[i] = -cd[i];
+  return vx_sign(s, vx_sum(ab, 8, cd, 8, s));
+}
+/* sign of det[A - D; B - D; C - D] by cofactors along the first row */
+static int vx_det3(const double* A, const double* B, const double* C, const double* D) {
+  double r[3][3][2];
+  for (int k = 0; k < 3; ++k) {
+    vx_diff(A[k], D[k], r[0][k]);
+    vx_diff(B[k], D[k], r[1][k]);
+    vx_diff(C[k], D[k], r[2][k]);
+  }
+  double total[192], tmp[192];
+  int len = 0;
+  for (int col = 0; col < 3; ++col) {
+    const int c1 = (col + 1) % 3, c2 = (col + 2) % 3; /* cyclic cofactor: + for every column */
+    double p[8], q[8], minor[16], term[64], t2[32];
+    vx_mul(r[1][c1], 2, r[2][c2], 2, p, t2);
+    vx_mul(r[1][c2], 2, r[2][c1], 2, q, t2);
+    for (int i = 0; i < 8; ++i) q[i] = -q[i];
+    const int lm = vx_sum(p, 8, q, 8, minor);
+    double big[128];
+    const int lt = vx_mul(minor, lm, r[0][col], 2, term, big);
+    len = vx_sum(total, len, term, lt, tmp);
+    memcpy(total, tmp, sizeof(double) * (size_t)len);
+  }
+  return vx_sign(total, len);
+}
+
+typedef struct {
+  double v[3][3];
+  double lo[3], hi[3];
+  int ns[3]; /* signs of the normal (B - A) x (C - A) */
+  int degenerate;
+} vx_tri;
+
+static void vx_prep(const double* t9, vx_tri* t) {
+  for (int k = 0; k < 3; ++k)
+    for (int d = 0; d < 3; ++d) t->v[k][d] = t9[k * 3 + d];
+  for (int d = 0; d < 3; ++d) {
+    t->lo[d] = fmin(fmin(t->v[0][d], t->v[1][d]), t->v[2][d]);
+    t->hi[d] = fmax(fmax(t->v[0][d], t->v[1][d]), t->v[2][d]);
+  }
+  for (int a = 0; a < 3; ++a) {
+    const int u = (a + 1) % 3, w = (a + 2) % 3;
+    t->ns[a] = vx_det2(t->v[1][u], t->v[1][w], t->v[0][u], t->v[0][w], t->v[2][u], t->v[2][w], t->v[0][u],
+                       t->v[0][w]);
+  }
+  t->degenerate = t->ns[0] == 0 && t->ns[1] == 0 && t->ns[2] == 0;
+}
+
+/* closed ray p + t e_a (t >= 0) meets the closed non-degenerate triangle
+ * (CGAL::do_intersect(Ray_3, Triangle_3)) */
+static int vx_ray_hit(const double* p, int a, const vx_tri* t) {
+  const double *A = t->v[0], *B = t->v[1], *C = t->v[2];
+  const int u = (a + 1) % 3, w = (a + 2) % 3;
+  const int na = t->ns[a];
+  const int s = vx_det3(B, C, p, A); /* normal . (p - A) */
+  if (na != 0) {
+    if (s != 0 && s == na) return 0; /* the plane lies behind the origin */
+    const int e1 = vx_det2(B[u], B[w], A[u], A[w], p[u], p[w], A[u], A[w]);
+    const int e2 = vx_det2(C[u], C[w], B[u], B[w], p[u], p[w], B[u], B[w]);
+    const int e3 = vx_det2(A[u], A[w], C[u], C[w], p[u], p[w], C[u], C[w]);
+    return e1 != -na && e2 != -na && e3 != -na;
+  }
+  if (s != 0) return 0;
+  /* ray in the triangle's plane: the coordinate plane (a, q) it projects onto */
+  const int q = t->ns[w] != 0 ? u : w;
+  for (int k = 0; k < 3; ++k) {
+    const double *X = t->v[k], *Y = t->v[(k + 1) % 3];
+    if ((X[q] > p[q] && Y[q] > p[q]) || (X[q] < p[q] && Y[q] < p[q])) continue;
+    if (X[q] == p[q] && Y[q] == p[q]) {
+      if (fmax(X[a], Y[a]) >= p[a]) return 1;
+      continue;
+    }
+    const int num = vx_det2(X[a], X[q], p[a], p[q], Y[a], Y[q], X[a], X[q]);
+    const int dq = Y[q] > X[q] ? 1 : -1;
+    if (num == 0 || num == dq) return 1;
+  }
+  return 0;
+}
+
+/* the closed triangle (degenerate ones too) meets the closed box: no
+ * separating axis among the box axes, the normal and the nine e_k x edge */
+static int vx_box_hit(const double* lo, const double* hi, const vx_tri* t) {
+  for (int k = 0; k < 3; ++k)
+    if (t->hi[k] < lo[k] || t->lo[k] > hi[k]) return 0;
+  const double *A = t->v[0], *B = t->v[1], *C = t->v[2];
+  if (!t->degenerate) {
+    double cmin[3], cmax[3];
+    for (int k = 0; k < 3; ++k) {
+      cmin[k] = t->ns[k] > 0 ? lo[k] : hi[k];
+      cmax[k] = t->ns[k] > 0 ? hi[k] : lo[k];
+    }
+    if (vx_det3(B, C, cmin, A) > 0 || vx_det3(B, C, cmax, A) < 0) return 0;
+  }
+  for (int e = 0; e < 3; ++e) {
+    const double *X = t->v[e], *Y = t->v[(e + 1) % 3], *Z = t->v[(e + 2) % 3];
+    for (int k = 0; k < 3; ++k) {
+      const int i1 = (k + 1) % 3, i2 = (k + 2) % 3;
+      if (Y[i1] == X[i1] && Y[i2] == X[i2]) continue;
+      const int l1 = Y[i2] > X[i2] ? -1 : (Y[i2] < X[i2] ? 1 : 0);
+      const int l2 = Y[i1] > X[i1] ? 1 : (Y[i1] < X[i1] ? -1 : 0);
+      const double mn1 = l1 > 0 ? lo[i1] : hi[i1], mx1 = l1 > 0 ? hi[i1] : lo[i1];
+      const double mn2 = l2 > 0 ? lo[i2] : hi[i2], mx2 = l2 > 0 ? hi[i2] : lo[i2];
+      if (vx_det2(Y[i1], Y[i2], X[i1], X[i2], X[i1], X[i2], mn1, mn2) < 0 &&
+          vx_det2(Y[i1], Y[i2], X[i1], X[i2], Z[i1], Z[i2], mn1, mn2) < 0)
+        return 0;
+      if (vx_det2(Y[i1], Y[i2], X[i1], X[i2], X[i1], X[i2], mx1, mx2) > 0 &&
+          vx_det2(Y[i1], Y[i2], X[i1], X[i2], Z[i1], Z[i2], mx1, mx2) > 0)
+        return 0;
+    }
+  }
+  return 1;
+}
+
+/* RayParityInsideAabb: odd hit counts on at least two of the three axes */
+static int vx_inside(const double* p, const vx_tri* tris, int64_t count) {
+  int votes = 0;
+  for (int a = 0; a < 3; ++a) {
+    const int u = (a + 1) % 3, w = (a + 2) % 3;
+    int odd = 0;
+    for (int64_t i = 0; i < count; ++i) {
+      const vx_tri* t = &tris[i];
+      if (t->degenerate || t->hi[a] < p[a] || t->lo[u] > p[u] || t->hi[u] < p[u] || t->lo[w] > p[w] ||
+          t->hi[w] < p[w])
+        continue;
+      odd ^= vx_ray_hit(p, a, t);
+    }
+    votes += odd;
+  }
+  return votes >= 2;
+}
+
+int ol_voxelize_part(int32_t num_blocks, int32_t n, const double* bounds, int64_t num_triangles,
+                     const double* triangles, int64_t num_inner, const double* inner, int32_t role,
+                     uint8_t* material) {
+  if (num_blocks <= 0 || n <= 0 || !bounds || !material) return fail("ol_voxelize_part: bad forest arguments");
+  if (role != 0 && role != 1) return fail("ol_voxelize_part: unknown part role");
+  if (num_triangles <= 0 || !triangles) return fail("geometry part has no triangles");
+  const int channel = role == 0;
+  if (channel && (num_inner <= 0 || !inner)) return fail("kInternalChannel requires inner_cavity_soup");
+  if (!channel) num_inner = 0;
+  vx_tri* outer = (vx_tri*)malloc(sizeof(vx_tri) * (size_t)num_triangles);
+  vx_tri* lumen = (vx_tri*)malloc(sizeof(vx_tri) * (size_t)(num_inner > 0 ? num_inner : 1));
+  if (!outer || !lumen) {
+    free(outer);
+    free(lumen);
+    return fail("ol_voxelize_part: out of memory");
+  }
+  for (int64_t i = 0; i < num_triangles; ++i) vx_prep(triangles + i * 9, &outer[i]);
+  for (int64_t i = 0; i < num_inner; ++i) vx_prep(inner + i * 9, &lumen[i]);
+  if (channel) {
+    /* validate_geometry_part: outside_sample_fraction(5) >= 0.05 over the
+     * outer soup's bounding box (cgal_surface_mesh.cpp) */
+    double bb[6] = {DBL_MAX, DBL_MAX, DBL_MAX, -DBL_MAX, -DBL_MAX, -DBL_MAX};
+    for (int64_t i = 0; i < num_triangles * 3; ++i)
+      for (int d = 0; d < 3; ++d) {
+        bb[d] = fmin(bb[d], triangles[i * 3 + d]);
+        bb[3 + d] = fmax(bb[3 + d], triangles[i * 3 + d]);
+      }
+    const int s = 5;
+    int outside = 0;
+    for (int iz = 0; iz < s; ++iz)
+      for (int iy = 0; iy < s; ++iy)
+        for (int ix = 0; ix < s; ++ix) {
+          const double p[3] = {bb[0] + (double)ix * ((bb[3] - bb[0]) / (double)(s - 1)),
+                               bb[1] + (double)iy * ((bb[4] - bb[1]) / (double)(s - 1)),
+                               bb[2] + (double)iz * ((bb[5] - bb[2]) / (double)(s - 1))};
+          outside += !vx_inside(p, outer, num_triangles);
+        }
+    if ((double)outside / (double)(s * s * s) < 0.05) {
+      free(outer);
+      free(lumen);
+      return fail("kInternalChannel mesh has insufficient exterior/lumen volume (solid-wall channel expected)");
+    }
+  }
+  const int64_t cells = (int64_t)n * n * n;
+#pragma omp parallel for schedule(dynamic, 1)
+  for (int32_t b = 0; b < num_blocks; ++b) {
+    const double* ob = bounds + (size_t)b * 6;
+    for (int k = 0; k < n; ++k)
+      for (int j = 0; j < n; ++j)
+        for (int i = 0; i < n; ++i) {
+          /* CellBounds (part_voxelizer.cpp:33-47) and the centre */
+          const int ijk[3] = {i, j, k};
+          double lo[3], hi[3], c[3];
+          for (int d = 0; d < 3; ++d) {
+            const double dx = (ob[3 + d] - ob[d]) / (double)n;
+            lo[d] = ob[d] + (double)ijk[d] * dx;
+            hi[d] = lo[d] + dx;
+            c[d] = 0.5 * (lo[d] + hi[d]);
+          }
+          int hit = 0;
+          for (int64_t t = 0; t < num_triangles && !hit; ++t) hit = vx_box_hit(lo, hi, &outer[t]);
+          for (int64_t t = 0; t < num_inner && !hit; ++t) hit = vx_box_hit(lo, hi, &lumen[t]);
+          uint8_t m;
+          if (hit) {
+            m = 2; /* kBoundary */
+          } else if (!channel) {
+            m = vx_inside(c, outer, num_triangles) ? 1 : 0; /* MapExternal */
+          } else {
+            const int in_lumen = vx_inside(c, lumen, num_inner);
+            const int in_wall = !in_lumen && vx_inside(c, outer, num_triangles);
+            m = in_lumen ? 0 : (in_wall ? 1 : 0); /* MapInternalChannel */
+          }
+          material[(size_t)b * (size_t)cells + ((size_t)k * n + j) * n + i] = m;
+        }
+  }
+  free(outer);
+  free(lumen);
+  return 0;
+}

@@ -105,6 +105,19 @@ int ol_adjoint(ol_sim* sim, int32_t steps, int32_t win_begin, int32_t win_end, c
 int ol_migrate_block(int32_t mode, int32_t n, int32_t h, const int32_t oct[3], const double* const* views,
                      double tau, double* out);
 
+/* voxelize_part (part_voxelizer.cpp:71-114) with validate_geometry_part
+ * (:51-69): MaterialKind per cell (MaterialField order (k*n + j)*n + i per
+ * block) of one part.  role 0 = kInternalChannel (inner soup required),
+ * 1 = kExternalObstacle.  Every geometric test is decided exactly on the
+ * double inputs (expansion arithmetic, no floating-point filter), as CGAL's
+ * exact predicates decide them: cell box vs closed triangle, and ray
+ * parity along +x, +y, +z from the cell centre (CgalSurfaceMesh::is_inside,
+ * cgal_surface_mesh.cpp).  Returns 0, or -1 with ol_last_error() set for
+ * the reference's GeometryBuildError cases. */
+int ol_voxelize_part(int32_t num_blocks, int32_t n, const double* bounds, int64_t num_triangles,
+                     const double* triangles, int64_t num_inner, const double* inner, int32_t role,
+                     uint8_t* material);
+
 #ifdef __cplusplus
 }
 #endif
