@@ -178,6 +178,7 @@ struct octlb_dev {
   double* d_dterm = nullptr;
   uint8_t* d_duse = nullptr;
   double* d_dblock_out = nullptr;
+  unsigned int* d_drag_done = nullptr;  // drag workgroups finished (the last one resets it)
   int ndlinks = 0, ndblocks = 0;
   double* d_stats = nullptr;
   double* d_stat_val = nullptr;
@@ -1045,6 +1046,8 @@ void octlb_dev::Prepare() {
   d_dterm = PAlloc<double>(dlinks.size());
   d_duse = PAlloc<uint8_t>(dlinks.size());
   d_dblock_out = PAlloc<double>(3 * static_cast<size_t>(std::max(ndblocks, 1)));
+  d_drag_done = PAlloc<unsigned int>(1);
+  HIP_CHECK(hipMemsetAsync(d_drag_done, 0, sizeof(unsigned int), stream));
   // ConstRho statistics of the finest level, per member: the member's
   // tiles are contiguous in tile order (octant order under ConstRho)
   {
@@ -1645,9 +1648,13 @@ void Drag(octlb_dev* d, const double* bbox, double* out) {
   for (int k = 0; k < 6; ++k) a.bbox[k] = bbox[k];
   a.par = d->Par();
   a.sc = d->sc;
-  LaunchDrag<T>(a, d->ndlinks, d->d_dblock_begin, d->ndblocks, d->d_dblock_out, d->d_out, d->stream);
+  if (d->ndblocks == 0) {  // no link in any block: the sum of nothing
+    for (int k = 0; k < 3; ++k) out[k] = 0.0;
+    return;
+  }
+  // the last drag workgroup writes the three sums straight into pinned host memory
+  LaunchDrag<T>(a, d->d_dblock_begin, d->ndblocks, d->d_dblock_out, d->d_drag_done, d->h_out, d->stream);
   HIP_CHECK(hipGetLastError());
-  HIP_CHECK(hipMemcpyAsync(d->h_out, d->d_out, 3 * sizeof(double), hipMemcpyDeviceToHost, d->stream));
   d->Sync();
   for (int k = 0; k < 3; ++k) out[k] = d->h_out[k];
 }
@@ -2264,7 +2271,7 @@ int octlb_dev_drag(octlb_dev* d, const double bbox[6], double out[3]) {
     CheckDev(d);
     if (!bbox || !out) throw ArgError("null argument");
     if (!d->prepared) d->Prepare();
-    d->Sync();
+    // the drag launch follows the steps on the solver's stream: no wait first
     if (d->prec == 64) {
       Drag<double>(d, bbox, out);
     } else {

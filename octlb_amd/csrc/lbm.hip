@@ -1670,8 +1670,40 @@ __global__ __launch_bounds__(512) void KFdCouple(FdArgs<T> fa, const short* fd_s
 constexpr int kDragThreads = 256;
 constexpr int kDragChunk = 2048;
 
+// The cylinder case's sum of the per-block forces in block order
+// (cylinder3d_case.h:492-497), staged through LDS so the sequential chain
+// reads no global memory.  Blocks without links add -0.0, which leaves any
+// sum unchanged, so only blocks with links are listed.
+// st: kDragChunk doubles of LDS.
+__device__ inline void SumBlocks3(const double* v, int count, double* out, double* st) {
+  constexpr int kPer = kDragChunk / 3;
+  const int tid = threadIdx.x;
+  double s0 = 0.0, s1 = 0.0, s2 = 0.0;
+  for (int c0 = 0; c0 < count; c0 += kPer) {
+    const int m = min(kPer, count - c0);
+    for (int e = tid; e < 3 * m; e += kDragThreads) st[e] = v[3 * c0 + e];
+    __syncthreads();
+    if (tid == 0)
+      for (int e = 0; e < m; ++e) {
+        s0 += st[3 * e];
+        s1 += st[3 * e + 1];
+        s2 += st[3 * e + 2];
+      }
+    __syncthreads();
+  }
+  if (tid == 0) {
+    out[0] = s0;
+    out[1] = s1;
+    out[2] = s2;
+  }
+}
+
+// One workgroup per block with drag links; the last workgroup to finish
+// (a device-scope counter, reset by it) sums the block forces and writes
+// the result to `result`, host-visible pinned memory.
 template <typename T>
-__global__ __launch_bounds__(kDragThreads) void KDragBlock(DragArgs<T> a, const int* block_begin, double* out) {
+__global__ __launch_bounds__(kDragThreads) void KDragBlock(DragArgs<T> a, const int* block_begin, double* out,
+                                                           unsigned int* done, int nblk, double* result) {
   // per link: the three products t * c_i (the reference's operands, so the
   // sequential sum below adds exactly what its loop adds), +0.0 for a link
   // outside the bbox predicate: a sum that starts at +0.0 is never -0.0
@@ -1717,39 +1749,24 @@ __global__ __launch_bounds__(kDragThreads) void KDragBlock(DragArgs<T> a, const 
     }
     __syncthreads();
   }
+  __shared__ int last;
   if (tid == 0) {
     out[3 * j] = -fo0;
     out[3 * j + 1] = -fo1;
     out[3 * j + 2] = -fo2;
+    __threadfence();  // the block force before the count that publishes it
+    last = atomicAdd(done, 1u) == static_cast<unsigned int>(nblk - 1);
+  }
+  __syncthreads();
+  if (!last) return;
+  __threadfence();  // every block force is visible to the last workgroup
+  SumBlocks3(out, nblk, result, tx);
+  if (tid == 0) {
+    *done = 0u;
+    __threadfence_system();
   }
 }
 
-// The cylinder case's sum of the per-block forces in block order
-// (cylinder3d_case.h:492-497), staged through LDS so the sequential chain
-// reads no global memory.  Blocks without links add -0.0, which leaves any
-// sum unchanged, so only blocks with links are listed.
-__global__ __launch_bounds__(kDragThreads) void KSum3(const double* v, int count, double* out) {
-  __shared__ double st[3 * kDragChunk];
-  const int tid = threadIdx.x;
-  double s0 = 0.0, s1 = 0.0, s2 = 0.0;
-  for (int c0 = 0; c0 < count; c0 += kDragChunk) {
-    const int m = min(kDragChunk, count - c0);
-    for (int e = tid; e < 3 * m; e += kDragThreads) st[e] = v[3 * c0 + e];
-    __syncthreads();
-    if (tid == 0)
-      for (int e = 0; e < m; ++e) {
-        s0 += st[3 * e];
-        s1 += st[3 * e + 1];
-        s2 += st[3 * e + 2];
-      }
-    __syncthreads();
-  }
-  if (tid == 0) {
-    out[0] = s0;
-    out[1] = s1;
-    out[2] = s2;
-  }
-}
 
 // Cylinder3dCase::total_mass (cylinder3d_case.h:601-622): rho over kBulk
 // cells, FP64 partial sums per block.
@@ -1896,11 +1913,11 @@ void LaunchRestrict(const CoupleArgs<T>& a, const int* list, int count, hipStrea
 }
 
 template <typename T>
-void LaunchDrag(const DragArgs<T>& a, int nlinks, const int* block_begin, int nblk, double* block_out,
-                double* out, hipStream_t s) {
-  (void)nlinks;
-  if (nblk > 0) hipLaunchKernelGGL(KDragBlock<T>, dim3(nblk), dim3(kDragThreads), 0, s, a, block_begin, block_out);
-  hipLaunchKernelGGL(KSum3, dim3(1), dim3(kDragThreads), 0, s, block_out, nblk, out);
+void LaunchDrag(const DragArgs<T>& a, const int* block_begin, int nblk, double* block_out, unsigned int* done,
+                double* result, hipStream_t s) {
+  if (nblk > 0)
+    hipLaunchKernelGGL(KDragBlock<T>, dim3(nblk), dim3(kDragThreads), 0, s, a, block_begin, block_out, done, nblk,
+                       result);
 }
 
 template <typename T>
@@ -1951,7 +1968,7 @@ void LaunchMacro(const T* p0, const T* p1, const ParArray& par, const BlockTab* 
   template void LaunchRestrict<T>(const CoupleArgs<T>&, const int*, int, hipStream_t);            \
   template void LaunchFdCouple<T>(const FdArgs<T>&, const short*, const int4*, int, const CoupleArgs<T>&, \
                                   const int*, int, int, hipStream_t);                               \
-  template void LaunchDrag<T>(const DragArgs<T>&, int, const int*, int, double*, double*,         \
+  template void LaunchDrag<T>(const DragArgs<T>&, const int*, int, double*, unsigned int*, double*,  \
                               hipStream_t);                                                       \
   template void LaunchMass<T>(const T*, const T*, const ParArray&, const BlockTab*,               \
                               const uint8_t*, const SimConst&, int, double*, double*, hipStream_t); \

@@ -505,8 +505,8 @@ void LaunchProlong(const CoupleArgs<T>& a, const int* list, int count, int half,
 template <typename T>
 void LaunchRestrict(const CoupleArgs<T>& a, const int* list, int count, hipStream_t s);
 template <typename T>
-void LaunchDrag(const DragArgs<T>& a, int nlinks, const int* block_begin, int nblk, double* block_out,
-                double* out, hipStream_t s);
+void LaunchDrag(const DragArgs<T>& a, const int* block_begin, int nblk, double* block_out, unsigned int* done,
+                double* result, hipStream_t s);
 template <typename T>
 void LaunchMass(const T* p0, const T* p1, const ParArray& par, const BlockTab* bt, const uint8_t* kind,
                 const SimConst& sc, int nb, double* partial, double* out, hipStream_t s);
