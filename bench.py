@@ -529,8 +529,8 @@ def main():
                          "counter scale: a plain copy reads and writes exactly that many bytes)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(json.dumps({"error": "this build runs on one GPU; multi-GPU sharding is a design "
-                                   "note only (DESIGN.md), no figure at N > 1 is produced",
+        print(json.dumps({"error": "this build runs on one GPU; sharding over several GPUs is a design "
+                                   "note only, no figure at N > 1 is produced",
                           "n_gpus": args.gpus}))
         sys.exit(2)
 

@@ -356,7 +356,8 @@ int octlb_build_links_noalloc(int32_t num_blocks, int32_t n, const double* bound
                               int64_t num_triangles, const double* triangles, int64_t capacity,
                               int64_t* num_links, int32_t* links, double* q, void* stream);
 
-/* Re-adaptation (the survey's dynamic-AMR contract; the reference has
+/* Re-adaptation (the reference plans a dynamic-AMR rebuild,
+ * doc/prd/octlb-framework.md:180,521, and has
  * no migration code): after the forest changes and a new handle dst is
  * built for it, move src's state onto dst.  Same block edge, halo,
  * precision and relaxation rate.  Unchanged blocks (same level and
@@ -406,9 +407,9 @@ int octlb_vtk_append_pvd(const char* pvd_path, int32_t iT, const char* vtm_relat
 /* Bouzidi streaming on 8^3 blocks.  One level without FD boundary      */
 /* kinds runs the single-level engine (fused kernels); several levels   */
 /* (Lagrava coupling, level_coupler.cpp) and/or FD inlet / outlet cells */
-/* (interpolated_velocity.h) run the multi-level engine.  The survey's  */
-/* adjoint entry octlb_adj_run(dev, ...) is split into create / run so  */
-/* the transposed stream operator is built once per adapt.              */
+/* (interpolated_velocity.h) run the multi-level engine.  One adjoint   */
+/* entry octlb_adj_run(dev, ...) is split into create / run so the      */
+/* transposed stream operator is built once per adapt.                  */
 /* ------------------------------------------------------------------ */
 typedef struct octlb_adj octlb_adj;
 

@@ -4,8 +4,8 @@
 // coarsen).  The host only matches blocks (level + integer origin, hashed)
 // and uploads one task per new block with the block lists of the rules.
 //
-// The rules (no reference counterpart; the survey's dynamic-AMR contract,
-// doc/prd/octlb-framework.md:180,521):
+// The rules (no reference counterpart; the reference plans its dynamic-AMR
+// rebuild in doc/prd/octlb-framework.md:180,521):
 //   copy     same level and bounds: the halo-inclusive view as stored
 //   refine   every slot of the fine block, halo included, from the parent
 //            cell under it: feq(rho, u) + s_p fneq with s_p = (tau - 1/4)/tau,
