@@ -420,17 +420,22 @@ __device__ __forceinline__ Origin CellOrigin(const double* ocw, int b) {
   return Origin{o[0], o[1], o[2], o[3]};
 }
 
-// y = C(x) over cells [c0, c1) (halo slots are carried: y = x)
+// y = C(x) over cells [c0, c1) (halo slots are carried: y = x); with a
+// tape, x is also written there (the cells' entries are the level's window)
 template <typename S>
 __global__ __launch_bounds__(kThreads) void KMLCollide(const S* x, S* y, const Cell* cells, const uint8_t* ckind,
                                                        const int* cblock, const int* boff, const double* ocw,
-                                                       const SimConst* scp, double t, int c0, int c1) {
+                                                       const SimConst* scp, double t, int c0, int c1, S* tape) {
   const SimConst& sc = *scp;  // by pointer: a by-value copy would be staged in scratch per lane
   const int cl = c0 + static_cast<int>(blockIdx.x) * kThreads + static_cast<int>(threadIdx.x);
   if (cl >= c1) return;
   const Cell c = cells[cl];
   S f[kQ];
   LoadCell(x, c, f);
+  if (tape) {
+#pragma unroll
+    for (int i = 0; i < kQ; ++i) tape[c.base + i * c.stride] = f[i];
+  }
   const uint8_t k = ckind[cl];
   if (k != kHalo) {
     const int b = cblock[cl];
@@ -1347,7 +1352,8 @@ struct octlb_adj {
     const size_t pm = static_cast<size_t>(npts) * kMacro;
     const int2 op = ops[k];
     const int l = op.y;
-    if (tape && win_hi[k] > win_lo[k])
+    // a visit's window is the entries of its level's cells: its collide tapes them
+    if (tape && op.x != kOpVisit && win_hi[k] > win_lo[k])
       ADJ_CHECK(hipMemcpyAsync(tape + win_lo[k], x + win_lo[k], static_cast<size_t>(win_hi[k] - win_lo[k]) * sizeof(S),
                                hipMemcpyDeviceToDevice, stream));
     switch (op.x) {
@@ -1355,7 +1361,7 @@ struct octlb_adj {
         const int c0 = lv_cell[l], c1 = lv_cell[l + 1];
         if (c1 <= c0) break;
         hipLaunchKernelGGL(KMLCollide<S>, dim3(Blocks(c1 - c0)), dim3(kThreads), 0, stream, x, y, d_cells, d_ckind,
-                           d_cblock, d_boff, d_ocw, d_scm, t, c0, c1);
+                           d_cblock, d_boff, d_ocw, d_scm, t, c0, c1, tape);
         hipLaunchKernelGGL(KMLGather<S>, dim3(Blocks(c1 - c0)), dim3(kThreads), 0, stream, y, x, d_src, d_links,
                            d_cells, c0, c1);
         break;
