@@ -414,6 +414,7 @@ int VoxelizePart(int32_t num_blocks, int32_t n, const double* bounds, int64_t nu
   if (channel && (num_inner <= 0 || !inner)) throw std::invalid_argument("kInternalChannel requires inner_cavity_soup");
   if (!channel) num_inner = 0;
   const int64_t ntri = num_triangles + num_inner;
+  if (ntri > static_cast<int64_t>(INT32_MAX)) throw std::invalid_argument("too many triangles (> 2^31 - 1)");
   const size_t cells = static_cast<size_t>(n) * n * n;
   const size_t total = static_cast<size_t>(num_blocks) * cells;
   if (total > static_cast<size_t>(INT32_MAX)) throw std::invalid_argument("forest too large (cells > 2^31)");
